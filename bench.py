@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: RK3 steps/s and Poisson solves/s of the 2nd-order full-implicit
+channel step at 768x384x576 (Re_tau=550, Lx=2pi, Lz=pi, tanh-stretched y),
+whole domain on one MI355X, inputs resident in HBM.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("RK3 steps/s & Poisson solves/s at 768×384×576; FP32+refine vs FP64 at "
+          "equal error")
+HBM_PEAK_GBS = 8000.0
+
+PROD = dict(nxc=768, nyc=384, nzc=576, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
+            xnu=9.9369e-5, dt=0.002, gamma=1.5, ubulk=1.0)
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads(requested):
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except Exception:
+        avail = os.cpu_count() or 1
+    n = requested or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    return max(1, min(n, avail, 16))
+
+
+def run_cpu_baseline(case, ux, uy, uz, threads):
+    """the oracle on the host cores: one PPE solve and one RK3 substep at the
+    full size, scaled to the metric's units"""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    o = O.Oracle(case)
+    o.field(O.UX)[...] = ux
+    o.field(O.UY)[...] = uy
+    o.field(O.UZ)[...] = uz
+    log(f"cpu: oracle allocated ({time.perf_counter() - t0:.1f}s), projecting")
+    o.project()
+    src = np.asfortranarray(o.field(O.ARR1).copy())
+    phi = np.zeros_like(o.field(O.HALO))
+    t0 = time.perf_counter()
+    o.call("clcPPE", src, phi)
+    t_ppe = time.perf_counter() - t0
+    log(f"cpu: one PPE solve {t_ppe:.2f}s")
+    t0 = time.perf_counter()
+    o.substep(1)
+    t_sub = time.perf_counter() - t0
+    log(f"cpu: one RK3 substep {t_sub:.2f}s")
+    o.close()
+    return t_ppe, t_sub
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ppe-solves", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--small", action="store_true", help="64^3 workload (quick check)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs "
+              "(--gpus > 1) are not part of it", file=sys.stderr)
+        return 2
+
+    from channel3d_amd import Channel, synth, lib as L
+
+    case = dict(PROD)
+    if a.small:
+        case.update(nxc=64, nyc=64, nzc=64)
+    nx, ny, nz = case["nxc"], case["nyc"], case["nzc"]
+    ncell = nx * ny * nz
+    log(f"workload {nx}x{ny}x{nz} ({ncell / 1e6:.1f}M cells), FP64")
+
+    t0 = time.perf_counter()
+    g = Channel(case)
+    log(f"context created, {g.device_bytes() / 2**30:.1f} GiB device memory "
+        f"({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    ux, uy, uz = synth.laminar_noise(nx, ny, nz, g.yp, yly=case["yly"], ubulk=1.0, seed=0)
+    g.upload(L.UX, ux)
+    g.upload(L.UY, uy)
+    g.upload(L.UZ, uz)
+    g.project()
+    g.sync()
+    log(f"laminar+noise field generated, uploaded, projected ({time.perf_counter() - t0:.1f}s)")
+
+    for _ in range(a.warmup):
+        g.step()
+    g.sync()
+    d0 = g.diagnostics()
+    log(f"warmup {a.warmup} steps done, divmax {d0['divmax2']:.2e}, cfl {d0['cfl']:.3f}")
+
+    # timed region: K steps, per-stage HIP events on the library stream
+    g.profile(True)
+    g.profile_reset()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.step()
+    g.sync()
+    t_steps = time.perf_counter() - t0
+    stages = g.profile_stats()
+    g.profile(False)
+    d1 = g.diagnostics()
+    ms_step = 1e3 * t_steps / a.steps
+    steps_per_s = a.steps / t_steps
+    log(f"{a.steps} steps: {ms_step:.2f} ms/step, {steps_per_s:.2f} steps/s, divmax "
+        f"{d1['divmax2']:.2e}")
+    if not (d1["divmax2"] < 1e-10 and math.isfinite(d1["vmax_x"])):
+        log("WARNING: divergence / velocity check failed")
+
+    # Poisson solves/s on the same context (source: the last pressure source)
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.ppe_solves):
+        g.ppe(L.ARR1, L.HALOARR)
+    g.sync()
+    t_ppe = time.perf_counter() - t0
+    ppe_per_s = a.ppe_solves / t_ppe
+    log(f"{a.ppe_solves} PPE solves: {1e3 * t_ppe / a.ppe_solves:.2f} ms each")
+
+    # dominant kernel roofline (algorithmic bytes / average launch time)
+    stage_rows = {}
+    dom = None
+    for name, s in stages.items():
+        avg_ms = s["total_ms"] / s["calls"]
+        gbs = s["bytes_per_call"] / (avg_ms * 1e-3) / 1e9 if s["bytes_per_call"] else None
+        stage_rows[name] = dict(calls=s["calls"], avg_ms=round(avg_ms, 4),
+                                share=round(s["total_ms"] / (1e3 * t_steps), 4),
+                                gbs=round(gbs, 1) if gbs else None)
+        if s["bytes_per_call"] and (dom is None or s["total_ms"] > stages[dom]["total_ms"]):
+            dom = name
+    roof = None
+    if dom:
+        s = stages[dom]
+        avg = s["total_ms"] / s["calls"]
+        ach = s["bytes_per_call"] / (avg * 1e-3) / 1e9
+        roof = dict(kernel=dom, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    bytes_per_launch=s["bytes_per_call"], avg_ms=round(avg, 4))
+
+    cpu = None
+    if not a.no_cpu:
+        threads = cpu_threads(a.cpu_threads)
+        log(f"cpu baseline: oracle with {threads} threads")
+        t_ppe_cpu, t_sub_cpu = run_cpu_baseline(case, ux, uy, uz, threads)
+        cpu = dict(value=round(1.0 / (3.0 * t_sub_cpu), 5), unit="RK3 steps/s", cores=threads,
+                   kind="port", poisson_solves_per_s=round(1.0 / t_ppe_cpu, 4),
+                   sample=(f"oracle (C, FP64, OpenMP) on the host: one PPE solve and one RK3 "
+                           f"substep at {nx}x{ny}x{nz}; steps/s = 1/(3 x substep time)"))
+    del ux, uy, uz
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 4),
+        "unit": "RK3 steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2nd-order full-implicit channel RK3 step {nx}x{ny}x{nz}, "
+                               "Re_tau=550, FP64, whole domain on one GPU",
+                   "nxc": nx, "nyc": ny, "nzc": nz, "dt": case["dt"], "xnu": case["xnu"],
+                   "precision": "fp64"},
+        "poisson_solves_per_s": round(ppe_per_s, 3),
+        "grid_points_steps_per_s": round(ncell * steps_per_s, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages": stage_rows,
+        "divmax_after_step": d1["divmax2"],
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,272 @@
+"""Host-side mirror of the reference's channel time-step interface.
+
+``Channel`` owns one device context of libchannel3d.so.  Its methods carry
+the reference's routine names and argument order (m_TScheme.f90:28-30,
+m_Poisson.f90:16, m_ChannelSystem.f90:91-130); arrays move in and out in
+the reference's Fortran layouts:
+
+  halo fields      (nxc+3, nyc+3, nzc+3), view[ic, jc, kc] == ux(ic, jc, kc)
+  interior fields  (nxc, nyc, nzc),       view[ic-1, jc-1, kc-1]
+  wall planes      (nxc, nzc)
+
+Device fields are referred to by the state ids of ``lib`` (UX, UY, ...) or by
+handles returned from ``alloc``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import lib as L
+
+
+def mesh_coords(nyc, yly, section_length, nyc_section, stret_type, stret_option, cstret):
+    """yp(1..nyc+1) of the y grid (m_MeshAndMetries.f90:144-160, 292-360);
+    returns nyc+2 values indexed like the reference (entry 0 unused)."""
+    lib = L.load()
+    ns = len(section_length)
+    sl = np.ascontiguousarray(section_length, dtype=np.float64)
+    cs = np.ascontiguousarray(cstret, dtype=np.float64)
+    ints = [np.ascontiguousarray(v, dtype=np.int32) for v in (nyc_section, stret_type,
+                                                             stret_option)]
+    out = np.zeros(nyc + 2)
+    L.check(lib.c3d_mesh_coords(int(nyc), float(yly), ns, sl.ctypes.data_as(L._DP),
+                                *(v.ctypes.data_as(L._IP) for v in ints),
+                                cs.ctypes.data_as(L._DP), out.ctypes.data_as(L._DP)),
+            "c3d_mesh_coords")
+    return out
+
+
+def channel_yp(nyc, yly, gamma):
+    """Two tanh sections (bottom, top), StretType 1, as the channel inputs use
+    (Input/TurbulentChannel_2nd.prm MeshOptions)."""
+    h = nyc // 2
+    return mesh_coords(nyc, yly, [1.0, 1.0], [h, nyc - h], [1, 1], [0, 1], [gamma, gamma])
+
+
+def make_params(case: dict, precision: int = L.FP64) -> L.Params:
+    p = L.Params()
+    p.nxc, p.nyc, p.nzc = int(case["nxc"]), int(case["nyc"]), int(case["nzc"])
+    p.xlx, p.yly, p.zlz = float(case["xlx"]), float(case["yly"]), float(case["zlz"])
+    p.xnu, p.dt = float(case["xnu"]), float(case["dt"])
+    bc = case.get("bc", (0, 0, L.BC_NOSLIP, L.BC_NOSLIP, 0, 0))
+    for d in range(6):
+        p.bc[d] = int(bc[d])
+    for name in ("uxbc", "uybc", "uzbc"):
+        vals = case.get(name, (0.0,) * 6)
+        arr = getattr(p, name)
+        for d in range(6):
+            arr[d] = float(vals[d])
+    for d, g in enumerate(case.get("gravity", (0.0, 0.0, 0.0))):
+        p.gravity[d] = float(g)
+    p.is_ux_const = int(case.get("is_ux_const", True))
+    p.ubulk = float(case.get("ubulk", 1.0))
+    p.ybulk1 = float(case.get("ybulk1", 0.0))
+    p.ybulk2 = float(case.get("ybulk2", case["yly"]))
+    p.ischeme = int(case.get("ischeme", 3))
+    p.precision = int(precision)
+    return p
+
+
+class Channel:
+    def __init__(self, case: dict, yp=None, precision: int = L.FP64):
+        self.lib = L.load()
+        self.case = dict(case)
+        self.params = make_params(case, precision)
+        if yp is None:
+            yp = channel_yp(self.params.nyc, self.params.yly, case.get("gamma", 1.5))
+        self.yp = np.ascontiguousarray(yp, dtype=np.float64)
+        h = ctypes.c_void_p()
+        L.check(self.lib.c3d_create(ctypes.byref(self.params), self.yp.ctypes.data_as(L._DP),
+                                    ctypes.byref(h)), "c3d_create")
+        self._h = h
+        self.nxc, self.nyc, self.nzc = self.params.nxc, self.params.nyc, self.params.nzc
+        self._state = {}
+        for fid in range(12):
+            v = ctypes.c_void_p()
+            L.check(self.lib.c3d_state_field(self._h, fid, ctypes.byref(v)), "c3d_state_field")
+            self._state[fid] = v
+
+    # lifetime ----------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.c3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # fields ------------------------------------------------------------------
+    def shape(self, layout):
+        if layout == L.HALO:
+            return (self.nxc + 3, self.nyc + 3, self.nzc + 3)
+        if layout == L.INTERIOR:
+            return (self.nxc, self.nyc, self.nzc)
+        return (self.nxc, self.nzc)
+
+    def ptr(self, f):
+        """device pointer of a state id or an alloc() handle"""
+        if isinstance(f, int):
+            return self._state[f]
+        return f
+
+    def alloc(self, layout):
+        v = ctypes.c_void_p()
+        L.check(self.lib.c3d_field_alloc(self._h, layout, ctypes.byref(v)), "c3d_field_alloc")
+        return v
+
+    def free(self, handle):
+        L.check(self.lib.c3d_field_free(self._h, handle), "c3d_field_free")
+
+    def upload(self, f, host, layout=None):
+        layout = L.STATE_LAYOUT[f] if layout is None else layout
+        a = np.asfortranarray(host, dtype=np.float64)
+        if a.shape != self.shape(layout):
+            raise ValueError(f"shape {a.shape} != {self.shape(layout)}")
+        L.check(self.lib.c3d_upload(self._h, layout, self.ptr(f), L.dptr(a)), "c3d_upload")
+
+    def download(self, f, layout=None):
+        layout = L.STATE_LAYOUT[f] if layout is None else layout
+        a = np.zeros(self.shape(layout), dtype=np.float64, order="F")
+        L.check(self.lib.c3d_download(self._h, layout, self.ptr(f), L.dptr(a)), "c3d_download")
+        return a
+
+    def metric(self, mid, lo, hi):
+        out = np.zeros(hi - lo + 1)
+        L.check(self.lib.c3d_get_metric(self._h, mid, lo, hi, out.ctypes.data_as(L._DP)),
+                "c3d_get_metric")
+        return out
+
+    def sync(self):
+        L.check(self.lib.c3d_sync(self._h), "c3d_sync")
+
+    def set_dt(self, dt):
+        L.check(self.lib.c3d_set_dt(self._h, float(dt)), "c3d_set_dt")
+
+    # reference routines --------------------------------------------------------
+    def _call(self, name, *fields):
+        fn = getattr(self.lib, "c3d_" + name)
+        L.check(fn(self._h, *[self.ptr(f) for f in fields]), "c3d_" + name)
+
+    def PMcoeUpdate(self, ns):
+        L.check(self.lib.c3d_PMcoeUpdate(self._h, int(ns)), "c3d_PMcoeUpdate")
+
+    def Update_uy_ym(self, uy_ym=L.UY_YM, duy_ym=L.DUY_YM):
+        self._call("Update_uy_ym", uy_ym, duy_ym)
+
+    def clcRhsX(self, ux, uy, uz, RhsX, HistXOld, pressure):
+        self._call("clcRhsX", ux, uy, uz, RhsX, HistXOld, pressure)
+
+    def clcRhsY(self, ux, uy, uz, RhsY, HistYOld, pressure):
+        self._call("clcRhsY", ux, uy, uz, RhsY, HistYOld, pressure)
+
+    def clcRhsZ(self, ux, uy, uz, RhsZ, HistZOld, pressure):
+        self._call("clcRhsZ", ux, uy, uz, RhsZ, HistZOld, pressure)
+
+    def clcU1Hat(self, ux, RhsX):
+        self._call("clcU1Hat", ux, RhsX)
+
+    def clcU2Hat(self, uy, RhsY, duy_ym):
+        self._call("clcU2Hat", uy, RhsY, duy_ym)
+
+    def clcU3Hat(self, uz, RhsZ):
+        self._call("clcU3Hat", uz, RhsZ)
+
+    def SetBC_and_UpdateHaloForPrSrc(self, ux, uy, uz, uy_ym):
+        self._call("SetBC_and_UpdateHaloForPrSrc", ux, uy, uz, uy_ym)
+
+    def SetBC_and_UpdateHalo_pr(self, pressure):
+        self._call("SetBC_and_UpdateHalo_pr", pressure)
+
+    def SetBC_and_UpdateHalo(self, ux, uy, uz, uy_ym):
+        self._call("SetBC_and_UpdateHalo", ux, uy, uz, uy_ym)
+
+    def clcPrSrc(self, ux, uy, uz, prsrc, pressure):
+        out = ctypes.c_double()
+        L.check(self.lib.c3d_clcPrSrc(self._h, *[self.ptr(f) for f in (ux, uy, uz, prsrc,
+                                                                        pressure)],
+                                      ctypes.byref(out)), "c3d_clcPrSrc")
+        return out.value
+
+    def clcPPE(self, prsrc, prphiHalo):
+        self._call("clcPPE", prsrc, prphiHalo)
+
+    def FluidVelUpdate(self, prphiHalo, ux, uy, uz):
+        self._call("FluidVelUpdate", prphiHalo, ux, uy, uz)
+
+    def PressureUpdate(self, pressure, prphiHalo):
+        self._call("PressureUpdate", pressure, prphiHalo)
+
+    def _diag(self, name, fields, n):
+        out = (ctypes.c_double * n)()
+        fn = getattr(self.lib, "c3d_" + name)
+        L.check(fn(self._h, *[self.ptr(f) for f in fields], out), "c3d_" + name)
+        return list(out) if n > 1 else out[0]
+
+    def CalcMaxCFL(self, ux=L.UX, uy=L.UY, uz=L.UZ):
+        return self._diag("CalcMaxCFL", (ux, uy, uz), 1)
+
+    def CheckDivergence(self, ux=L.UX, uy=L.UY, uz=L.UZ):
+        return self._diag("CheckDivergence", (ux, uy, uz), 1)
+
+    def CalcVmax(self, ux=L.UX, uy=L.UY, uz=L.UZ):
+        return self._diag("CalcVmax", (ux, uy, uz), 3)
+
+    def CalcUxAver(self, ux=L.UX):
+        return self._diag("CalcUxAver", (ux,), 1)
+
+    # fused, device-resident --------------------------------------------------
+    def substep(self, ns):
+        L.check(self.lib.c3d_substep(self._h, int(ns)), "c3d_substep")
+
+    def step(self):
+        L.check(self.lib.c3d_step(self._h), "c3d_step")
+
+    def project(self):
+        L.check(self.lib.c3d_project(self._h), "c3d_project")
+
+    def ppe(self, prsrc, phi):
+        L.check(self.lib.c3d_ppe(self._h, self.ptr(prsrc), self.ptr(phi)), "c3d_ppe")
+
+    def diagnostics(self):
+        out = (ctypes.c_double * 9)()
+        L.check(self.lib.c3d_step_diagnostics(self._h, out), "c3d_step_diagnostics")
+        keys = ("cfl", "divmax1", "divmax2", "vmax_x", "vmax_y", "vmax_z", "PrGradNow",
+                "PrGradAver", "SimTime")
+        return dict(zip(keys, list(out)))
+
+    # profiling ----------------------------------------------------------------
+    def profile(self, on=True):
+        L.check(self.lib.c3d_profile_enable(self._h, int(on)), "c3d_profile_enable")
+
+    def profile_reset(self):
+        L.check(self.lib.c3d_profile_reset(self._h), "c3d_profile_reset")
+
+    def profile_stats(self):
+        n = self.lib.c3d_profile_count(self._h)
+        out = {}
+        for i in range(n):
+            name = ctypes.c_char_p()
+            calls = ctypes.c_long()
+            ms = ctypes.c_double()
+            b = ctypes.c_double()
+            L.check(self.lib.c3d_profile_get(self._h, i, ctypes.byref(name), ctypes.byref(calls),
+                                             ctypes.byref(ms), ctypes.byref(b)),
+                    "c3d_profile_get")
+            if calls.value:
+                out[name.value.decode()] = dict(calls=calls.value, total_ms=ms.value,
+                                                bytes_per_call=b.value)
+        return out
+
+    def device_bytes(self):
+        return int(self.lib.c3d_device_bytes(self._h))

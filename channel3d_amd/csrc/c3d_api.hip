@@ -1,0 +1,1053 @@
+// C ABI of the MI355X Channel3d core: context set-up (grid metrics, FFT
+// twiddles, implicit-operator tables), device fields, the per-routine entry
+// points that mirror the reference's procedure pointers, the fused
+// device-resident substep/step, and stage profiling with HIP events.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "c3d_internal.h"
+
+using namespace c3d;
+
+namespace c3d {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return C3D_ERR_HIP;
+}
+
+int prof_begin(c3d_ctx* c, int stage, hipEvent_t* ev) {
+  (void)stage;
+  *ev = nullptr;
+  if (!c->prof) return C3D_OK;
+  if (c->ev_free.empty()) {
+    hipEvent_t e;
+    C3D_HIP(hipEventCreate(&e));
+    c->ev_free.push_back(e);
+  }
+  *ev = c->ev_free.back();
+  c->ev_free.pop_back();
+  C3D_HIP(hipEventRecord(*ev, c->stream));
+  return C3D_OK;
+}
+
+static int prof_harvest(c3d_ctx* c) {
+  if (c->pending.empty()) return C3D_OK;
+  C3D_HIP(hipEventSynchronize(c->pending.back().b));
+  for (auto& q : c->pending) {
+    float ms = 0.f;
+    C3D_HIP(hipEventElapsedTime(&ms, q.a, q.b));
+    c->prof_ms[q.stage] += ms;
+    c->prof_calls[q.stage] += 1;
+    c->ev_free.push_back(q.a);
+    c->ev_free.push_back(q.b);
+  }
+  c->pending.clear();
+  return C3D_OK;
+}
+
+int prof_end(c3d_ctx* c, int stage, hipEvent_t ev, double bytes) {
+  if (!c->prof || !ev) return C3D_OK;
+  if (c->ev_free.empty()) {
+    hipEvent_t e;
+    C3D_HIP(hipEventCreate(&e));
+    c->ev_free.push_back(e);
+  }
+  hipEvent_t e2 = c->ev_free.back();
+  c->ev_free.pop_back();
+  C3D_HIP(hipEventRecord(e2, c->stream));
+  c->pending.push_back({stage, ev, e2});
+  c->prof_bytes[stage] = bytes;
+  if (c->pending.size() >= 4096) return prof_harvest(c);
+  return C3D_OK;
+}
+
+}  // namespace c3d
+
+// ===========================================================================
+// mesh: m_MeshAndMetries.f90 clcCoord (:292-360) and sections (:144-160)
+// ===========================================================================
+static void clc_coord(double* y, int n0, int n1, double s0, double s1, int type, double cs,
+                      int opt) {
+  const int m = n1 - n0;
+  const double len = s1 - s0;
+  const double PI = 3.141592653589793238462643383279502884;
+  y[n0] = s0;
+  y[n1] = s1;
+  switch (type) {
+    case 0:
+      for (int j = 0; j <= m; ++j) y[n0 + j] = s0 + (double)j / (double)m * len;
+      break;
+    case 1: {
+      const double tstr = std::tanh(cs);
+      for (int j = 0; j <= m; ++j) {
+        const double xi = (double)j / (double)(float)m;
+        const double sc = std::tanh(cs * (xi - 1.0)) / tstr + 1.0;
+        if (opt == 0) y[n0 + j] = s0 + sc * len;
+        else y[n1 - j] = s1 - sc * len;
+      }
+    } break;
+    case 2: {
+      const double tstr = std::sin(0.5 * cs * PI);
+      for (int j = 0; j <= m; ++j) {
+        const double xi = (double)j / (double)m - 1.0;
+        const double sc = std::sin(cs * xi * PI * 0.5) / tstr + 1.0;
+        if (opt == 0) y[n0 + j] = s0 + sc * len;
+        else y[n1 - j] = s1 - sc * len;
+      }
+    } break;
+    case 3:
+      if (cs == 1.0) {
+        for (int j = 0; j <= m; ++j) y[n0 + j] = s0 + (double)j / (double)m * len;
+      } else {
+        const double tstr = (std::pow(cs, m) - 1.0) / (cs - 1.0);
+        for (int j = 0; j <= m - 1; ++j) {
+          const double sc = std::pow(cs, j) / tstr;
+          if (opt == 0) y[n0 + j + 1] = y[n0 + j] + sc * len;
+          else y[n1 - j - 1] = y[n1 - j] - sc * len;
+        }
+      }
+      break;
+  }
+  y[n0] = s0;
+  y[n1] = s1;
+}
+
+extern "C" int c3d_mesh_coords(int nyc, double yly, int nsection, const double* section_length,
+                               const int* nyc_section, const int* stret_type,
+                               const int* stret_option, const double* cstret, double* yp_out) {
+  if (nsection < 1 || !section_length || !nyc_section || !stret_type || !stret_option ||
+      !cstret || !yp_out) {
+    set_error("c3d_mesh_coords: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  int tot = 0;
+  double sum = 0.0;
+  for (int s = 0; s < nsection; ++s) {
+    if (nyc_section[s] < 1 || stret_type[s] < 0 || stret_type[s] > 3 || section_length[s] < 0 ||
+        cstret[s] < 0) {
+      set_error("c3d_mesh_coords: bad section");
+      return C3D_ERR_ARG;
+    }
+    tot += nyc_section[s];
+    sum += section_length[s];
+  }
+  if (tot != nyc) {
+    set_error("c3d_mesh_coords: sum(nycSection) /= nyc");
+    return C3D_ERR_ARG;
+  }
+  std::vector<int> nid(nsection + 1);
+  std::vector<double> ys(nsection + 1);
+  nid[0] = 1;
+  ys[0] = 0.0;
+  for (int s = 0; s < nsection; ++s) {
+    nid[s + 1] = nid[s] + nyc_section[s];
+    ys[s + 1] = ys[s] + section_length[s] / sum * yly;
+  }
+  ys[nsection] = yly;
+  for (int s = 0; s < nsection; ++s)
+    clc_coord(yp_out, nid[s], nid[s + 1], ys[s], ys[s + 1], stret_type[s], cstret[s],
+              stret_option[s]);
+  return C3D_OK;
+}
+
+extern "C" const char* c3d_last_error(void) { return g_err.c_str(); }
+
+// ===========================================================================
+// metrics (m_MeshAndMetries.f90:77-286, m_TScheme.f90:73-85,
+// m_Poisson.f90:36-66), host side, reference indexing
+// ===========================================================================
+static void compute_metrics(c3d_ctx* c, const double* yp_in) {
+  const int nx = c->p.nxc, ny = c->p.nyc, nz = c->p.nzc;
+  const int nyp = ny + 1;
+  c->dx = c->p.xlx / (double)nx;
+  c->dy = c->p.yly / (double)ny;
+  c->dz = c->p.zlz / (double)nz;
+  c->rdx = (double)nx / c->p.xlx;
+  c->rdy = (double)ny / c->p.yly;
+  c->rdz = (double)nz / c->p.zlz;
+  c->rdx2 = c->rdx * c->rdx;
+  c->rdy2 = c->rdy * c->rdy;
+  c->rdz2 = c->rdz * c->rdz;
+  const int L = ny + 3;
+  for (int id = 0; id < C3D_NMETRICS; ++id) c->met[id].assign(L, 0.0);
+  c->met[C3D_M_WAVEX].assign(nx + 1, 0.0);
+  c->met[C3D_M_WAVEZ].assign(nz + 1, 0.0);
+  auto& yp = c->met[C3D_M_YP];
+  auto& yc = c->met[C3D_M_YC];
+  auto& dyp = c->met[C3D_M_DYP];
+  auto& dyc = c->met[C3D_M_DYC];
+  auto& rdyp = c->met[C3D_M_RDYP];
+  auto& rdyc = c->met[C3D_M_RDYC];
+  auto& yint = c->met[C3D_M_YINTERP];
+  auto& am2Pr = c->met[C3D_M_AM2PR];
+  auto& ac2Pr = c->met[C3D_M_AC2PR];
+  auto& ap2Pr = c->met[C3D_M_AP2PR];
+  auto& am2c = c->met[C3D_M_AM2C];
+  auto& ac2c = c->met[C3D_M_AC2C];
+  auto& ap2c = c->met[C3D_M_AP2C];
+  auto& am2p = c->met[C3D_M_AM2P];
+  auto& ac2p = c->met[C3D_M_AC2P];
+  auto& ap2p = c->met[C3D_M_AP2P];
+  auto& amcn = c->met[C3D_M_AM2CCN];
+  auto& apcn = c->met[C3D_M_AP2CCN];
+  for (int j = 1; j <= nyp; ++j) yp[j] = yp_in[j];
+  for (int j = 1; j <= ny; ++j) yc[j] = 0.5 * (yp[j] + yp[j + 1]);
+  yc[0] = 2.0 * yp[1] - yc[1];
+  yc[nyp] = 2.0 * yp[nyp] - yc[ny];
+  for (int j = 1; j <= ny; ++j) dyp[j] = yp[j + 1] - yp[j];
+  dyp[0] = dyp[1];
+  dyp[nyp] = dyp[ny];
+  for (int j = 2; j <= ny; ++j) dyc[j] = yc[j] - yc[j - 1];
+  dyc[1] = dyp[1];
+  dyc[nyp] = dyp[ny];
+  for (int j = 0; j <= nyp; ++j) rdyp[j] = 1.0 / dyp[j];
+  for (int j = 1; j <= nyp; ++j) rdyc[j] = 1.0 / dyc[j];
+  for (int j = 1; j <= nyp; ++j) yint[j] = dyp[j] / (dyp[j] + dyp[j - 1]);
+  for (int j = 1; j <= ny; ++j) {
+    am2Pr[j] = rdyp[j] * rdyc[j];
+    ap2Pr[j] = rdyp[j] * rdyc[j + 1];
+  }
+  am2Pr[1] = 0.0;                    // y walls are never periodic here
+  ap2Pr[1] = rdyp[1] * rdyc[2];
+  am2Pr[ny] = rdyp[ny] * rdyc[ny];
+  ap2Pr[ny] = 0.0;
+  for (int j = 1; j <= ny; ++j) ac2Pr[j] = -(am2Pr[j] + ap2Pr[j]);
+  for (int j = 1; j <= ny; ++j) {
+    am2c[j] = rdyp[j] * rdyc[j];
+    ap2c[j] = rdyp[j] * rdyc[j + 1];
+  }
+  if (c->p.bc[2] == C3D_BC_NOSLIP) {
+    am2c[1] = 4.0 * rdyc[1] / (dyc[1] + 2.0 * dyc[2]);
+    ap2c[1] = 4.0 * rdyc[2] / (dyc[1] + 2.0 * dyc[2]);
+  }
+  if (c->p.bc[3] == C3D_BC_NOSLIP) {
+    am2c[ny] = 4.0 * rdyc[ny] / (dyc[nyp] + 2.0 * dyc[ny]);
+    ap2c[ny] = 4.0 * rdyc[nyp] / (dyc[nyp] + 2.0 * dyc[ny]);
+  }
+  for (int j = 1; j <= ny; ++j) ac2c[j] = -(am2c[j] + ap2c[j]);
+  for (int j = 1; j <= ny; ++j) {
+    am2p[j] = rdyc[j] * rdyp[j - 1];
+    ap2p[j] = rdyc[j] * rdyp[j];
+    ac2p[j] = -(am2p[j] + ap2p[j]);
+  }
+  for (int j = 1; j <= ny; ++j) {
+    amcn[j] = am2c[j];
+    apcn[j] = ap2c[j];
+  }
+  amcn[1] = c->p.bc[2] == C3D_BC_NOSLIP ? 2.0 * am2c[1] : 0.0;
+  apcn[ny] = c->p.bc[3] == C3D_BC_NOSLIP ? 2.0 * ap2c[ny] : 0.0;
+  // modified wavenumbers for the periodic x and z (indices kx = i-1, kz = k-1)
+  const double twoPi = 6.283185307179586476925286766559005768;
+  for (int kx = 0; kx < nx; ++kx)
+    c->met[C3D_M_WAVEX][kx] = 2.0 * c->rdx2 * (std::cos(twoPi * (double)kx / (double)nx) - 1.0);
+  for (int kz = 0; kz < nz; ++kz)
+    c->met[C3D_M_WAVEZ][kz] = 2.0 * c->rdz2 * (std::cos(twoPi * (double)kz / (double)nz) - 1.0);
+  double normfft = 1.0;
+  normfft = normfft * 1.0 * (double)nx;
+  normfft = normfft * 1.0 * (double)nz;
+  c->normfft = 1.0 / normfft;
+  c->s1coe.assign(L, 0.0);
+  for (int j = 1; j <= ny; ++j)
+    c->s1coe[j] = (yc[j] < c->p.ybulk1 || yc[j] >= c->p.ybulk2) ? 0.0 : dyp[j];
+}
+
+// periodic Green's function of tridiag(a, b, a) of size n (the circulant
+// system InversePTriFixedCoe solves), cut where the dropped tail is below
+// 2^-64 of the centre tap, else all n taps
+static void periodic_taps(double a, double b, int n, int* lo, int* cnt, std::vector<double>& taps) {
+  taps.clear();
+  if (a == 0.0) {
+    *lo = 0;
+    *cnt = 1;
+    taps.push_back(1.0 / b);
+    return;
+  }
+  const long double A = a, B = b;
+  const long double disc = sqrtl(B * B - 4.0L * A * A);
+  const long double r = -2.0L * A / (B + disc);
+  const long double C = 1.0L / disc;
+  const long double ar = fabsl(r);
+  const long double rn = powl(r, (long double)n);
+  auto g = [&](int m) { return C * (powl(r, (long double)m) + powl(r, (long double)(n - m))) / (1.0L - rn); };
+  int M = 0;
+  long double rm = ar;
+  const long double eps = ldexpl(1.0L, -65) * (1.0L - ar);
+  while (M < n && rm > eps) {
+    ++M;
+    rm *= ar;
+  }
+  if (2 * M + 1 >= n) {
+    *lo = 0;
+    *cnt = n;
+    for (int m = 0; m < n; ++m) taps.push_back((double)g(m));
+  } else {
+    *lo = -M;
+    *cnt = 2 * M + 1;
+    for (int t = 0; t < 2 * M + 1; ++t) taps.push_back((double)g(std::abs(t - M)));
+  }
+}
+
+// shared Thomas factors of InverseTridiagonal (m_Tools.f90:229-239) for
+// per-j coefficients a, b, c (1-based, rows 1..n); out: a, vecm, arrmn
+static void thomas_factors(const std::vector<double>& a, const std::vector<double>& b,
+                           const std::vector<double>& cc, int n, std::vector<double>& vecm,
+                           std::vector<double>& arr) {
+  vecm.assign(n + 2, 0.0);
+  arr.assign(n + 2, 0.0);
+  vecm[1] = b[1];
+  for (int j = 2; j <= n; ++j) {
+    arr[j] = cc[j - 1] / vecm[j - 1];
+    vecm[j] = b[j] - a[j] * arr[j];
+  }
+}
+
+static int build_adi_tables(c3d_ctx* c) {
+  const int ny = c->g.ny, nx = c->g.nx, nz = c->g.nz;
+  const size_t L = (size_t)ny + 2;
+  const size_t per_ns = (size_t)nx + (size_t)nz + 6 * L;
+  std::vector<double> host(3 * per_ns, 0.0);
+  for (int s = 0; s < 3; ++s) {
+    auto& tb = c->adi[s];
+    double* hb = host.data() + s * per_ns;
+    double* db = (double*)c->d_tabs + s * per_ns;
+    const double beta = 0.5 * c->aconst[s] * c->p.xnu * c->dt;
+    std::vector<double> taps;
+    // x: mic = pic = -pmBeta*rdx2, cic = pmBeta*rdx2*2 + 1 (m_TSchemeFIMP_inc.f90:287-289)
+    periodic_taps(-beta * c->rdx2, beta * c->rdx2 * 2.0 + 1.0, nx, &tb.xlo, &tb.xn, taps);
+    std::memcpy(hb, taps.data(), sizeof(double) * taps.size());
+    tb.d_xtaps = db;
+    periodic_taps(-beta * c->rdz2, beta * c->rdz2 * 2.0 + 1.0, nz, &tb.zlo, &tb.zn, taps);
+    std::memcpy(hb + nx, taps.data(), sizeof(double) * taps.size());
+    tb.d_ztaps = db + nx;
+    // y, ux/uz: CN metrics (:327-333)
+    std::vector<double> a(ny + 2, 0.0), b(ny + 2, 0.0), cc(ny + 2, 0.0), vecm, arr;
+    for (int j = 1; j <= ny; ++j) {
+      cc[j] = -beta * c->met[C3D_M_AP2CCN][j];
+      a[j] = -beta * c->met[C3D_M_AM2CCN][j];
+      b[j] = -cc[j] - a[j] + 1.0;
+    }
+    thomas_factors(a, b, cc, ny, vecm, arr);
+    double* yb = hb + nx + nz;
+    std::memcpy(yb + 0 * L, a.data(), sizeof(double) * L);
+    std::memcpy(yb + 1 * L, vecm.data(), sizeof(double) * L);
+    std::memcpy(yb + 2 * L, arr.data(), sizeof(double) * L);
+    for (int q = 0; q < 3; ++q) tb.d_yc[q] = db + nx + nz + q * L;
+    // y, uy: wall row then node metrics (:700-712)
+    cc[1] = 0.0; b[1] = 1.0; a[1] = 0.0;
+    for (int j = 2; j <= ny; ++j) {
+      cc[j] = -beta * c->met[C3D_M_AP2P][j];
+      b[j] = -beta * c->met[C3D_M_AC2P][j] + 1.0;
+      a[j] = -beta * c->met[C3D_M_AM2P][j];
+    }
+    thomas_factors(a, b, cc, ny, vecm, arr);
+    std::memcpy(yb + 3 * L, a.data(), sizeof(double) * L);
+    std::memcpy(yb + 4 * L, vecm.data(), sizeof(double) * L);
+    std::memcpy(yb + 5 * L, arr.data(), sizeof(double) * L);
+    for (int q = 0; q < 3; ++q) tb.d_yp[q] = db + nx + nz + (3 + q) * L;
+  }
+  C3D_HIP(hipMemcpyAsync(c->d_tabs, host.data(), sizeof(double) * host.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  return C3D_OK;
+}
+
+template <typename T>
+static void fill_metric_block(c3d_ctx* c, std::vector<T>& h, Metrics<T>& m, T* dbase) {
+  const size_t L = (size_t)c->g.ny + 3;
+  const int ids[12] = {C3D_M_RDYP, C3D_M_RDYC, C3D_M_YINTERP, C3D_M_AM2C, C3D_M_AC2C, C3D_M_AP2C,
+                       C3D_M_AM2P, C3D_M_AC2P, C3D_M_AP2P, C3D_M_AM2PR, C3D_M_AC2PR, C3D_M_AP2PR};
+  const T** dst[12] = {&m.rdyp, &m.rdyc, &m.yint, &m.am2c, &m.ac2c, &m.ap2c,
+                       &m.am2p, &m.ac2p, &m.ap2p, &m.am2Pr, &m.ac2Pr, &m.ap2Pr};
+  size_t off = 0;
+  for (int q = 0; q < 12; ++q) {
+    for (size_t j = 0; j < L; ++j) h[off + j] = (T)c->met[ids[q]][j];
+    *dst[q] = dbase + off;
+    off += L;
+  }
+  for (size_t j = 0; j < L; ++j) h[off + j] = (T)c->s1coe[j];
+  m.s1coe = dbase + off;
+  off += L;
+  const int nk = c->g.nk, nz = c->g.nz;
+  for (int k = 0; k <= nk; ++k) h[off + k] = (T)c->met[C3D_M_WAVEX][k];
+  m.wavex = dbase + off;
+  off += nk + 1;
+  for (int k = 0; k < nz; ++k) h[off + k] = (T)c->met[C3D_M_WAVEZ][k];
+  m.wavez = dbase + off;
+}
+
+static size_t metric_block_len(const Geom& g) { return 13 * ((size_t)g.ny + 3) + g.nk + 1 + g.nz; }
+
+// twiddles exp(-2 pi i t / n), rounded from long double
+template <typename T>
+static void twiddles(int n, int count, std::vector<T>& out) {
+  out.resize(2 * (size_t)count);
+  const long double tp = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < count; ++t) {
+    const long double a = tp * (long double)t / (long double)n;
+    out[2 * t] = (T)cosl(a);
+    out[2 * t + 1] = (T)(-sinl(a));
+  }
+}
+
+template <typename T>
+static int upload_twiddles(c3d_ctx* c) {
+  const int nk = c->g.nk, nz = c->g.nz;
+  std::vector<T> a, b, z;
+  twiddles<T>(nk, nk, a);
+  twiddles<T>(2 * nk, nk + 1, b);
+  twiddles<T>(nz, nz, z);
+  const size_t bytes = sizeof(T) * (a.size() + b.size() + z.size());
+  C3D_HIP(hipMalloc(&c->d_tw, bytes));
+  c->dev_bytes += bytes;
+  T* d = (T*)c->d_tw;
+  C3D_HIP(hipMemcpy(d, a.data(), sizeof(T) * a.size(), hipMemcpyHostToDevice));
+  C3D_HIP(hipMemcpy(d + a.size(), b.data(), sizeof(T) * b.size(), hipMemcpyHostToDevice));
+  C3D_HIP(hipMemcpy(d + a.size() + b.size(), z.data(), sizeof(T) * z.size(), hipMemcpyHostToDevice));
+  c->tw_x = d;
+  c->tw_xr = d + a.size();
+  c->tw_z = d + a.size() + b.size();
+  return C3D_OK;
+}
+
+static int dalloc(c3d_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " +
+              hipGetErrorString(e));
+    return C3D_ERR_NOMEM;
+  }
+  c->dev_bytes += bytes;
+  return C3D_OK;
+}
+
+static size_t layout_elems(const c3d_ctx* c, int layout) {
+  if (layout == C3D_WALL) return (size_t)c->g.nx * c->g.nz;
+  return (size_t)c->g.nfield;
+}
+
+extern "C" int c3d_destroy(c3d_ctx* c) {
+  if (!c) return C3D_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& q : c->pending) {
+    hipEventDestroy(q.a);
+    hipEventDestroy(q.b);
+  }
+  for (auto e : c->ev_free) hipEventDestroy(e);
+  for (int i = 0; i < C3D_NSTATE; ++i)
+    if (c->state[i]) hipFree(c->state[i]);
+  for (void* p : c->user_fields) hipFree(p);
+  void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
+                  c->d_dyp};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  if (c->staging) hipHostFree(c->staging);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return C3D_OK;
+}
+
+extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) {
+  if (!p || !yp || !out) {
+    set_error("c3d_create: null argument");
+    return C3D_ERR_ARG;
+  }
+  *out = nullptr;
+  if (p->bc[0] != C3D_BC_PERIOD || p->bc[1] != C3D_BC_PERIOD || p->bc[4] != C3D_BC_PERIOD ||
+      p->bc[5] != C3D_BC_PERIOD) {
+    set_error("c3d_create: x and z must be periodic (the FIMP _010 channel path)");
+    return C3D_ERR_ARG;
+  }
+  for (int d = 2; d <= 3; ++d)
+    if (p->bc[d] != C3D_BC_NOSLIP && p->bc[d] != C3D_BC_FREESLIP) {
+      set_error("c3d_create: y boundaries must be no-slip or free-slip walls");
+      return C3D_ERR_ARG;
+    }
+  if (p->nxc < 4 || p->nyc < 3 || p->nzc < 4 || (p->nxc & 1)) {
+    set_error("c3d_create: need even nxc >= 4, nyc >= 3, nzc >= 4");
+    return C3D_ERR_ARG;
+  }
+  if (!fft_size_supported(p->nxc / 2) || !fft_size_supported(p->nzc)) {
+    set_error("c3d_create: nxc/2 and nzc must be supported FFT lengths (2^a 3^b 5^c list)");
+    return C3D_ERR_UNSUPPORTED;
+  }
+  if (p->precision != C3D_FP64 && p->precision != C3D_FP32) {
+    set_error("c3d_create: bad precision");
+    return C3D_ERR_ARG;
+  }
+  if (p->ischeme < 1 || p->ischeme > 3 || !(p->dt > 0) || !(p->xnu >= 0)) {
+    set_error("c3d_create: bad time-scheme parameters");
+    return C3D_ERR_ARG;
+  }
+  if (p->is_ux_const && !(p->ybulk2 > p->ybulk1)) {
+    set_error("c3d_create: ybulk1 OR ybulk2 wrong");
+    return C3D_ERR_ARG;
+  }
+  c3d_ctx* c = new (std::nothrow) c3d_ctx();
+  if (!c) return C3D_ERR_NOMEM;
+  c->p = *p;
+  c->prec = p->precision;
+  c->esize = c->prec == C3D_FP64 ? 8 : 4;
+  Geom& g = c->g;
+  g.nx = p->nxc;
+  g.ny = p->nyc;
+  g.nz = p->nzc;
+  g.nyh = g.ny + 3;
+  g.plane = (long long)g.nx * g.nyh;
+  g.nfield = g.plane * g.nz;
+  g.nk = g.nx / 2;
+  g.P = ((g.nk + 1) + 3) / 4 * 4;
+  g.nspec = (long long)g.P * g.ny * g.nz;
+  compute_metrics(c, yp);
+  // m_Parameters.f90:135-150
+  if (p->ischeme == 1) {
+    c->iadvance = 1;
+    const double gc[3] = {1.5, 0.0, 0.0}, tc[3] = {-0.5, 0.0, 0.0};
+    for (int s = 0; s < 3; ++s) { c->gconst[s] = gc[s]; c->tconst[s] = tc[s]; }
+  } else if (p->ischeme == 2) {
+    c->iadvance = 2;
+    const double gc[3] = {0.5, 1.0, 0.0}, tc[3] = {0.0, -0.5, 0.0};
+    for (int s = 0; s < 3; ++s) { c->gconst[s] = gc[s]; c->tconst[s] = tc[s]; }
+  } else {
+    c->iadvance = 3;
+    const double gc[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+    const double tc[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+    for (int s = 0; s < 3; ++s) { c->gconst[s] = gc[s]; c->tconst[s] = tc[s]; }
+  }
+  for (int s = 0; s < 3; ++s) c->aconst[s] = c->gconst[s] + c->tconst[s];
+  c->dt = p->dt;
+  c->SimTime = 0.0;
+  c->cur_ns = 1;
+
+  int rc = C3D_OK;
+  auto fail = [&](int r) {
+    c3d_destroy(c);
+    return r;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    c->stream = nullptr;
+    return fail(hip_fail(e, "hipStreamCreate"));
+  }
+  // metrics
+  {
+    const size_t L = metric_block_len(g);
+    const size_t bytes = L * (c->prec == C3D_FP64 ? sizeof(double) : sizeof(float));
+    if ((rc = dalloc(c, &c->d_met, bytes))) return fail(rc);
+    if (c->prec == C3D_FP64) {
+      std::vector<double> h(L, 0.0);
+      fill_metric_block<double>(c, h, c->md, (double*)c->d_met);
+      e = hipMemcpy(c->d_met, h.data(), bytes, hipMemcpyHostToDevice);
+    } else {
+      std::vector<float> h(L, 0.f);
+      fill_metric_block<float>(c, h, c->mf, (float*)c->d_met);
+      e = hipMemcpy(c->d_met, h.data(), bytes, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return fail(hip_fail(e, "metric upload"));
+    if ((rc = dalloc(c, (void**)&c->d_dyp, sizeof(double) * (g.ny + 3)))) return fail(rc);
+    e = hipMemcpy(c->d_dyp, c->met[C3D_M_DYP].data(), sizeof(double) * (g.ny + 3),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(hip_fail(e, "dyp upload"));
+  }
+  // implicit-operator tables
+  {
+    const size_t per_ns = (size_t)g.nx + (size_t)g.nz + 6 * ((size_t)g.ny + 2);
+    if ((rc = dalloc(c, &c->d_tabs, 3 * per_ns * sizeof(double)))) return fail(rc);
+    if ((rc = build_adi_tables(c))) return fail(rc);
+  }
+  rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);
+  if (rc) return fail(rc);
+  // state and scratch
+  const size_t fbytes = (size_t)g.nfield * c->esize;
+  const size_t wbytes = (size_t)g.nx * g.nz * c->esize;
+  for (int i = 0; i < C3D_NSTATE; ++i) {
+    const size_t b = (i == C3D_UY_YM || i == C3D_DUY_YM) ? wbytes : fbytes;
+    if ((rc = dalloc(c, &c->state[i], b))) return fail(rc);
+    e = hipMemset(c->state[i], 0, b);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipMemset"));
+  }
+  if ((rc = dalloc(c, &c->tmpA, fbytes))) return fail(rc);
+  if ((rc = dalloc(c, &c->spec, (size_t)g.nspec * 2 * c->esize))) return fail(rc);
+  if ((rc = dalloc(c, &c->sarr, (size_t)g.nspec * c->esize))) return fail(rc);
+  {
+    const long long nb = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 7) / 8);
+    c->max_partials = (int)(10 * nb + 64);
+    if ((rc = dalloc(c, (void**)&c->d_partials, sizeof(double) * c->max_partials))) return fail(rc);
+  }
+  if ((rc = dalloc(c, (void**)&c->d_sc, sizeof(DevScalars)))) return fail(rc);
+  e = hipMemset(c->d_sc, 0, sizeof(DevScalars));
+  if (e != hipSuccess) return fail(hip_fail(e, "hipMemset"));
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(hip_fail(e, "c3d_create"));
+  *out = c;
+  return C3D_OK;
+}
+
+extern "C" int c3d_sync(c3d_ctx* c) {
+  if (!c) return C3D_ERR_ARG;
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  return prof_harvest(c);
+}
+
+extern "C" int c3d_get_metric(c3d_ctx* c, int id, int lo, int hi, double* out) {
+  if (!c || id < 0 || id >= C3D_NMETRICS || !out || lo > hi || lo < 0 ||
+      hi >= (int)c->met[id].size()) {
+    set_error("c3d_get_metric: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  for (int j = lo; j <= hi; ++j) out[j - lo] = c->met[id][j];
+  return C3D_OK;
+}
+
+extern "C" int c3d_set_dt(c3d_ctx* c, double dt) {
+  if (!c || !(dt > 0)) {
+    set_error("c3d_set_dt: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  c->dt = dt;
+  return build_adi_tables(c);
+}
+
+// ===========================================================================
+// fields and layout conversion
+// ===========================================================================
+extern "C" int c3d_field_alloc(c3d_ctx* c, int layout, void** dptr) {
+  if (!c || !dptr || layout < C3D_HALO || layout > C3D_WALL) {
+    set_error("c3d_field_alloc: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  const size_t bytes = layout_elems(c, layout) * c->esize;
+  int rc = dalloc(c, dptr, bytes);
+  if (rc) return rc;
+  C3D_HIP(hipMemset(*dptr, 0, bytes));
+  c->user_fields.push_back(*dptr);
+  return C3D_OK;
+}
+
+extern "C" int c3d_field_free(c3d_ctx* c, void* dptr) {
+  if (!c || !dptr) return C3D_ERR_ARG;
+  for (size_t i = 0; i < c->user_fields.size(); ++i)
+    if (c->user_fields[i] == dptr) {
+      C3D_HIP(hipStreamSynchronize(c->stream));
+      C3D_HIP(hipFree(dptr));
+      c->user_fields.erase(c->user_fields.begin() + i);
+      return C3D_OK;
+    }
+  set_error("c3d_field_free: not a field of this context");
+  return C3D_ERR_ARG;
+}
+
+extern "C" int c3d_state_field(c3d_ctx* c, int id, void** dptr) {
+  if (!c || !dptr || id < 0 || id >= C3D_NSTATE) {
+    set_error("c3d_state_field: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  *dptr = c->state[id];
+  return C3D_OK;
+}
+
+static int ensure_staging(c3d_ctx* c, size_t bytes) {
+  if (c->staging_bytes >= bytes) return C3D_OK;
+  if (c->staging) hipHostFree(c->staging);
+  c->staging = nullptr;
+  c->staging_bytes = 0;
+  C3D_HIP(hipHostMalloc(&c->staging, bytes, hipHostMallocDefault));
+  c->staging_bytes = bytes;
+  return C3D_OK;
+}
+
+template <typename T>
+static void pack(const c3d_ctx* c, int layout, const double* host, T* dev_img) {
+  const Geom& g = c->g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (layout == C3D_WALL) {
+    for (size_t q = 0; q < (size_t)nx * nz; ++q) dev_img[q] = (T)host[q];
+    return;
+  }
+  const size_t hx = (size_t)nx + 3, hy = (size_t)ny + 3;
+  for (int k = 0; k < nz; ++k)
+    for (int jj = 0; jj < g.nyh; ++jj) {
+      T* drow = dev_img + (size_t)fidx(g, 0, jj, k);
+      if (layout == C3D_HALO) {
+        const double* hrow = host + 1 + hx * ((size_t)jj + hy * (size_t)(k + 1));
+        for (int i = 0; i < nx; ++i) drow[i] = (T)hrow[i];
+      } else if (jj >= 1 && jj <= ny) {
+        const double* hrow = host + (size_t)nx * ((size_t)(jj - 1) + (size_t)ny * k);
+        for (int i = 0; i < nx; ++i) drow[i] = (T)hrow[i];
+      } else {
+        for (int i = 0; i < nx; ++i) drow[i] = (T)0;
+      }
+    }
+}
+
+template <typename T>
+static void unpack(const c3d_ctx* c, int layout, const T* dev_img, double* host) {
+  const Geom& g = c->g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (layout == C3D_WALL) {
+    for (size_t q = 0; q < (size_t)nx * nz; ++q) host[q] = (double)dev_img[q];
+    return;
+  }
+  if (layout == C3D_INTERIOR) {
+    for (int k = 0; k < nz; ++k)
+      for (int jj = 1; jj <= ny; ++jj) {
+        const T* drow = dev_img + (size_t)fidx(g, 0, jj, k);
+        double* hrow = host + (size_t)nx * ((size_t)(jj - 1) + (size_t)ny * k);
+        for (int i = 0; i < nx; ++i) hrow[i] = (double)drow[i];
+      }
+    return;
+  }
+  // halo: interior x/z range plus the periodic halo layers the reference keeps
+  const size_t hx = (size_t)nx + 3, hy = (size_t)ny + 3;
+  std::memset(host, 0, sizeof(double) * hx * hy * ((size_t)nz + 3));
+  for (int kc = 0; kc <= nz + 1; ++kc) {
+    const int k = kc == 0 ? nz - 1 : kc == nz + 1 ? 0 : kc - 1;
+    for (int jj = 0; jj < g.nyh; ++jj) {
+      const T* drow = dev_img + (size_t)fidx(g, 0, jj, k);
+      double* hrow = host + hx * ((size_t)jj + hy * (size_t)kc);
+      for (int i = 0; i < nx; ++i) hrow[i + 1] = (double)drow[i];
+      hrow[0] = (double)drow[nx - 1];
+      hrow[nx + 1] = (double)drow[0];
+    }
+  }
+}
+
+extern "C" int c3d_upload(c3d_ctx* c, int layout, void* dptr, const double* host) {
+  if (!c || !dptr || !host || layout < C3D_HALO || layout > C3D_WALL) {
+    set_error("c3d_upload: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  const size_t n = layout_elems(c, layout);
+  int rc = ensure_staging(c, n * c->esize);
+  if (rc) return rc;
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  if (c->prec == C3D_FP64) pack<double>(c, layout, host, (double*)c->staging);
+  else pack<float>(c, layout, host, (float*)c->staging);
+  C3D_HIP(hipMemcpyAsync(dptr, c->staging, n * c->esize, hipMemcpyHostToDevice, c->stream));
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_download(c3d_ctx* c, int layout, const void* dptr, double* host) {
+  if (!c || !dptr || !host || layout < C3D_HALO || layout > C3D_WALL) {
+    set_error("c3d_download: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  const size_t n = layout_elems(c, layout);
+  int rc = ensure_staging(c, n * c->esize);
+  if (rc) return rc;
+  C3D_HIP(hipMemcpyAsync(c->staging, dptr, n * c->esize, hipMemcpyDeviceToHost, c->stream));
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  if (c->prec == C3D_FP64) unpack<double>(c, layout, (const double*)c->staging, host);
+  else unpack<float>(c, layout, (const float*)c->staging, host);
+  return C3D_OK;
+}
+
+// ===========================================================================
+// reference routines
+// ===========================================================================
+#define CHECK_CTX(c)                         \
+  do {                                       \
+    if (!(c)) {                              \
+      set_error("null context");             \
+      return C3D_ERR_ARG;                    \
+    }                                        \
+  } while (0)
+
+extern "C" int c3d_PMcoeUpdate(c3d_ctx* c, int ns) {
+  CHECK_CTX(c);
+  if (ns < 1 || ns > c->iadvance) {
+    set_error("c3d_PMcoeUpdate: ns out of range");
+    return C3D_ERR_ARG;
+  }
+  const double dt = c->dt;
+  c->pmGamma = c->gconst[ns - 1] * dt;
+  c->pmTheta = c->tconst[ns - 1] * dt;
+  c->pmAlpha = c->aconst[ns - 1] * dt;
+  c->pmBeta = 0.5 * c->aconst[ns - 1] * c->p.xnu * dt;
+  c->SimTime = c->SimTime + dt * c->aconst[ns - 1];
+  c->pmAlphaC = c->aconst[ns - 1];
+  c->cur_ns = ns;
+  return C3D_OK;
+}
+
+extern "C" int c3d_Update_uy_ym(c3d_ctx* c, void* uy_ym, void* duy_ym) {
+  CHECK_CTX(c);
+  return launch_update_uy_ym(c, uy_ym, duy_ym);
+}
+
+extern "C" int c3d_clcRhsX(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* rx,
+                           void* hx, const void* pr) {
+  CHECK_CTX(c);
+  int rc = launch_rhs(c, 1, ux, uy, uz, pr, rx, nullptr, nullptr, hx, nullptr, nullptr, 1);
+  if (rc) return rc;
+  if (c->p.is_ux_const) return launch_add_dpmdxns(c, rx);
+  return C3D_OK;
+}
+
+extern "C" int c3d_clcRhsY(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* ry,
+                           void* hy, const void* pr) {
+  CHECK_CTX(c);
+  return launch_rhs(c, 2, ux, uy, uz, pr, nullptr, ry, nullptr, nullptr, hy, nullptr, 0);
+}
+
+extern "C" int c3d_clcRhsZ(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* rz,
+                           void* hz, const void* pr) {
+  CHECK_CTX(c);
+  return launch_rhs(c, 4, ux, uy, uz, pr, nullptr, nullptr, rz, nullptr, nullptr, hz, 0);
+}
+
+extern "C" int c3d_clcU1Hat(c3d_ctx* c, void* ux, void* rx) {
+  CHECK_CTX(c);
+  return launch_adi(c, 0, c->cur_ns, ux, rx, nullptr, 0);
+}
+
+extern "C" int c3d_clcU2Hat(c3d_ctx* c, void* uy, void* ry, const void* duy_ym) {
+  CHECK_CTX(c);
+  return launch_adi(c, 1, c->cur_ns, uy, ry, duy_ym, 0);
+}
+
+extern "C" int c3d_clcU3Hat(c3d_ctx* c, void* uz, void* rz) {
+  CHECK_CTX(c);
+  return launch_adi(c, 2, c->cur_ns, uz, rz, nullptr, 0);
+}
+
+extern "C" int c3d_SetBC_and_UpdateHaloForPrSrc(c3d_ctx* c, void* ux, void* uy, void* uz,
+                                                const void* uy_ym) {
+  CHECK_CTX(c);
+  return launch_bc_vel(c, ux, uy, uz, uy_ym, 0);
+}
+
+extern "C" int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* c, void* pr) {
+  CHECK_CTX(c);
+  return launch_bc_pr(c, pr);
+}
+
+extern "C" int c3d_SetBC_and_UpdateHalo(c3d_ctx* c, void* ux, void* uy, void* uz,
+                                        const void* uy_ym) {
+  CHECK_CTX(c);
+  return launch_bc_vel(c, ux, uy, uz, uy_ym, 1);
+}
+
+static int read_scalars(c3d_ctx* c) {
+  C3D_HIP(hipMemcpyAsync(&c->h_sc, c->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  return prof_harvest(c);
+}
+
+extern "C" int c3d_clcPrSrc(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                            void* prsrc, void* pr, double* divmax) {
+  CHECK_CTX(c);
+  int rc = launch_prsrc(c, ux, uy, uz, prsrc, pr, 0.0);
+  if (rc) return rc;
+  if (divmax) {
+    if ((rc = read_scalars(c))) return rc;
+    *divmax = c->h_sc.divmax1;
+  }
+  return C3D_OK;
+}
+
+extern "C" int c3d_clcPPE(c3d_ctx* c, void* prsrc, void* phi) {
+  CHECK_CTX(c);
+  return run_ppe(c, prsrc, phi);
+}
+
+extern "C" int c3d_ppe(c3d_ctx* c, void* prsrc, void* phi) {
+  CHECK_CTX(c);
+  return run_ppe(c, prsrc, phi);
+}
+
+extern "C" int c3d_FluidVelUpdate(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz) {
+  CHECK_CTX(c);
+  return launch_proj(c, phi, ux, uy, uz, nullptr, 1, 0, 0);
+}
+
+extern "C" int c3d_PressureUpdate(c3d_ctx* c, void* pr, const void* phi) {
+  CHECK_CTX(c);
+  return launch_proj(c, phi, nullptr, nullptr, nullptr, pr, 0, 1, 0);
+}
+
+static int diag_read(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int what) {
+  int rc = launch_diag(c, ux, uy, uz, what, 0);
+  if (rc) return rc;
+  return read_scalars(c);
+}
+
+extern "C" int c3d_CalcMaxCFL(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                              double* uddxmax) {
+  CHECK_CTX(c);
+  if (!uddxmax) return C3D_ERR_ARG;
+  int rc = diag_read(c, ux, uy, uz, 1);
+  if (rc) return rc;
+  *uddxmax = c->h_sc.uddxmax[0];
+  c->cfl_valid = false;  // slot 0 was overwritten
+  return C3D_OK;
+}
+
+extern "C" int c3d_CheckDivergence(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                                   double* divmax) {
+  CHECK_CTX(c);
+  if (!divmax) return C3D_ERR_ARG;
+  int rc = diag_read(c, ux, uy, uz, 2);
+  if (rc) return rc;
+  *divmax = c->h_sc.divmax2;
+  return C3D_OK;
+}
+
+extern "C" int c3d_CalcVmax(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                            double* vmax3) {
+  CHECK_CTX(c);
+  if (!vmax3) return C3D_ERR_ARG;
+  int rc = diag_read(c, ux, uy, uz, 4);
+  if (rc) return rc;
+  for (int q = 0; q < 3; ++q) vmax3[q] = c->h_sc.vmax[q];
+  return C3D_OK;
+}
+
+extern "C" int c3d_CalcUxAver(c3d_ctx* c, const void* ux, double* uxaver) {
+  CHECK_CTX(c);
+  if (!uxaver) return C3D_ERR_ARG;
+  int rc = diag_read(c, ux, ux, ux, 8);
+  if (rc) return rc;
+  *uxaver = c->h_sc.uxaver;
+  return C3D_OK;
+}
+
+// ===========================================================================
+// fused, device-resident step (m_ChannelSystem.f90:73-130)
+// ===========================================================================
+extern "C" int c3d_substep(c3d_ctx* c, int ns) {
+  CHECK_CTX(c);
+  int rc;
+  void** s = c->state;
+  if ((rc = c3d_PMcoeUpdate(c, ns))) return rc;
+  if ((rc = launch_update_uy_ym(c, s[C3D_UY_YM], s[C3D_DUY_YM]))) return rc;
+  // RhsX -> RealArr1, RhsY -> RealArr2, RhsZ -> RealHalo (:97-100)
+  if ((rc = launch_rhs(c, 7, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], s[C3D_ARR1], s[C3D_ARR2],
+                       s[C3D_HALOARR], s[C3D_HISTX], s[C3D_HISTY], s[C3D_HISTZ], 1)))
+    return rc;
+  // clcU1Hat..U3Hat (:104-106); the constant-flow-rate shift of RhsX is
+  // applied while the x sweep loads it
+  if ((rc = launch_adi(c, 0, ns, s[C3D_UX], s[C3D_ARR1], nullptr, c->p.is_ux_const))) return rc;
+  if ((rc = launch_adi(c, 1, ns, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
+  if ((rc = launch_adi(c, 2, ns, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
+  // :110-114
+  if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
+  if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], s[C3D_PR], 0.0))) return rc;
+  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  // :115-121, the Neumann ghost of phi applied inside the projection
+  if ((rc = launch_proj(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], 1, 1, 1)))
+    return rc;
+  // :123-124
+  if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
+  return launch_bc_pr(c, s[C3D_PR]);
+}
+
+extern "C" int c3d_step(c3d_ctx* c) {
+  CHECK_CTX(c);
+  int rc;
+  void** s = c->state;
+  // CalcMaxCFL on the state entering the step (:81); it was computed by the
+  // previous step's closing diagnostics unless the state changed since
+  if (!c->cfl_valid) {
+    if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 1, c->cfl_slot))) return rc;
+  }
+  for (int ns = 1; ns <= c->iadvance; ++ns)
+    if ((rc = c3d_substep(c, ns))) return rc;
+  // CheckDivergence, CalcVmax (:127-130) and the next step's CFL, one pass
+  const int next = c->cfl_slot ^ 1;
+  if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 7, next))) return rc;
+  c->last_cfl_slot = c->cfl_slot;
+  c->cfl_slot = next;
+  c->cfl_valid = true;
+  return C3D_OK;
+}
+
+extern "C" int c3d_step_diagnostics(c3d_ctx* c, double* o) {
+  CHECK_CTX(c);
+  if (!o) return C3D_ERR_ARG;
+  int rc = read_scalars(c);
+  if (rc) return rc;
+  o[0] = c->h_sc.uddxmax[c->last_cfl_slot] * c->dt;
+  o[1] = c->h_sc.divmax1;
+  o[2] = c->h_sc.divmax2;
+  o[3] = c->h_sc.vmax[0];
+  o[4] = c->h_sc.vmax[1];
+  o[5] = c->h_sc.vmax[2];
+  o[6] = c->h_sc.PrGradNow;
+  o[7] = c->h_sc.PrGradAver;
+  o[8] = c->SimTime;
+  return C3D_OK;
+}
+
+extern "C" int c3d_project(c3d_ctx* c) {
+  CHECK_CTX(c);
+  int rc;
+  void** s = c->state;
+  c->cfl_valid = false;
+  const double save = c->pmAlpha;
+  c->pmAlpha = 1.0;
+  if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
+  if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], nullptr, 1.0))) return rc;
+  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  if ((rc = launch_proj(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr, 1, 0, 1)))
+    return rc;
+  c->pmAlpha = save;
+  return launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1);
+}
+
+// ===========================================================================
+// profiling
+// ===========================================================================
+static const char* kStageNames[ST_COUNT] = {
+    "rhs_fused", "rhs_single", "bulk_finalize", "adi_x", "adi_z", "adi_y_update", "bc_ghosts",
+    "prsrc", "fft_x_fwd", "fft_z_fwd", "tridiag_y", "fft_z_inv", "fft_x_inv", "projection",
+    "vel_update", "pr_update", "diagnostics", "misc"};
+
+extern "C" int c3d_profile_enable(c3d_ctx* c, int on) {
+  CHECK_CTX(c);
+  int rc = c3d_sync(c);
+  if (rc) return rc;
+  c->prof = on != 0;
+  return C3D_OK;
+}
+
+extern "C" int c3d_profile_reset(c3d_ctx* c) {
+  CHECK_CTX(c);
+  int rc = c3d_sync(c);
+  if (rc) return rc;
+  for (int i = 0; i < ST_COUNT; ++i) {
+    c->prof_calls[i] = 0;
+    c->prof_ms[i] = 0.0;
+  }
+  return C3D_OK;
+}
+
+extern "C" int c3d_profile_count(c3d_ctx* c) {
+  (void)c;
+  return ST_COUNT;
+}
+
+extern "C" int c3d_profile_get(c3d_ctx* c, int idx, const char** name, long* calls,
+                               double* total_ms, double* bytes_per_call) {
+  CHECK_CTX(c);
+  if (idx < 0 || idx >= ST_COUNT) return C3D_ERR_ARG;
+  int rc = c3d_sync(c);
+  if (rc) return rc;
+  if (name) *name = kStageNames[idx];
+  if (calls) *calls = c->prof_calls[idx];
+  if (total_ms) *total_ms = c->prof_ms[idx];
+  if (bytes_per_call) *bytes_per_call = c->prof_bytes[idx];
+  return C3D_OK;
+}
+
+extern "C" size_t c3d_device_bytes(c3d_ctx* c) { return c ? c->dev_bytes : 0; }

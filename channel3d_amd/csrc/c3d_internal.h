@@ -1,0 +1,240 @@
+// Internal declarations of the MI355X Channel3d core library.
+//
+// Device layout of every 3-D field (FP64 or FP32):
+//   element (i, jj, k) at  i + nx * (jj + nyh * k)
+//   i  in [0, nx)    x cell index (reference ic = i+1); x-periodic by index wrap
+//   jj in [0, ny+2]  the reference's own j index, wall ghost planes included
+//   k  in [0, nz)    z cell index (reference kc = k+1); z-periodic by index wrap
+// The reference's x and z halo layers (m_Decomp2d.f90:1652-1738) are never
+// stored: on one device they are exact copies of the opposite interior plane,
+// so kernels read the wrapped interior index instead.
+//
+// Spectral (PPE) array: complex, element (kx, j0, kz) at kx + P * (j0 + ny * kz)
+//   kx in [0, nx/2] (pitch P >= nx/2+1), j0 = jj-1 in [0, ny), kz in [0, nz).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/channel3d.h"
+
+namespace c3d {
+
+struct Geom {
+  int nx, ny, nz;
+  int nyh;            // ny + 3 stored y planes (jj = 0 .. ny+2)
+  long long plane;    // nx * nyh
+  long long nfield;   // plane * nz
+  int nk;             // nx/2 (complex length of the x transforms)
+  int P;              // spectral kx pitch (complex elements)
+  long long nspec;    // P * ny * nz
+};
+
+__host__ __device__ inline long long fidx(const Geom& g, int i, int jj, int k) {
+  return (long long)i + (long long)g.nx * ((long long)jj + (long long)g.nyh * (long long)k);
+}
+
+// Reproducible sums.  Each double term is truncated to a multiple of 2^-80
+// and split into four signed 32-bit limbs accumulated in int64: integer
+// addition is associative, so the total is independent of the order and of
+// the launch shape (the same on any device or thread count).  Terms must
+// satisfy |t| < 2^47; for up to 2^31 terms the limb sums cannot overflow.
+// The total is rounded to double by fix_value in a fixed order.
+__host__ __device__ inline void fix_add(long long* acc, double t) {
+  union { double d; unsigned long long u; } cv;
+  cv.d = t;
+  const unsigned long long bits = cv.u;
+  const int e = (int)((bits >> 52) & 0x7FF);
+  if (e == 0) return;  // zero / subnormal: below the 2^-80 grid
+  const unsigned long long mant = (bits & 0xFFFFFFFFFFFFFULL) | 0x10000000000000ULL;
+  int sh = e - 995;    // t * 2^80 = mant * 2^sh
+  unsigned long long hi = 0, lo = 0;
+  if (sh >= 0) {
+    if (sh > 74) sh = 74;
+    if (sh == 0) lo = mant;
+    else if (sh < 64) { lo = mant << sh; hi = mant >> (64 - sh); }
+    else hi = mant << (sh - 64);
+  } else {
+    if (-sh >= 53) return;
+    lo = mant >> (-sh);
+  }
+  const long long l0 = (long long)(lo & 0xFFFFFFFFULL), l1 = (long long)(lo >> 32);
+  const long long l2 = (long long)(hi & 0xFFFFFFFFULL), l3 = (long long)(hi >> 32);
+  if (bits >> 63) { acc[0] -= l0; acc[1] -= l1; acc[2] -= l2; acc[3] -= l3; }
+  else            { acc[0] += l0; acc[1] += l1; acc[2] += l2; acc[3] += l3; }
+}
+
+__host__ __device__ inline double fix_value(const long long* acc) {
+  return (((double)acc[3] * 65536.0 + (double)acc[2] * 1.52587890625e-05) +
+          (double)acc[1] * 3.552713678800501e-15) + (double)acc[0] * 8.271806125530277e-25;
+}
+
+// 1-D wall-normal metric vectors on the device, indexed by the reference j
+// (0 .. ny+2), entries outside the reference's own range hold 0.
+template <typename T>
+struct Metrics {
+  const T* rdyp;
+  const T* rdyc;
+  const T* yint;   // YinterpCoe
+  const T* am2c;
+  const T* ac2c;
+  const T* ap2c;
+  const T* am2p;
+  const T* ac2p;
+  const T* ap2p;
+  const T* am2Pr;
+  const T* ac2Pr;
+  const T* ap2Pr;
+  const T* s1coe;  // dyp(j) inside [ybulk1, ybulk2), else 0 (m_TSchemeFIMP_inc.f90:25-27)
+  const T* wavex;  // WaveNumX for kx = 0 .. nx/2
+  const T* wavez;  // WaveNumZ for kz = 0 .. nz-1
+};
+
+// per-substep scalars (m_Parameters.f90:173-186) and constants
+template <typename T>
+struct Coef {
+  T pmGamma, pmTheta, pmAlpha, pmBeta;
+  T rdx, rdz, rdx2, rdz2;
+  T gx, gy, gz;     // gravity
+  T xnu;
+};
+
+// device-resident scalars written by reduction kernels
+struct DevScalars {
+  double s1tot1, dpmdxns, PrGradNow, PrGradAver;
+  double divmax1, divmax2;
+  double uddxmax[2];  // CalcMaxCFL result; two slots so a step can compute the next one
+  double vmax[3];
+  double uxaver;
+  double pad[4];
+};
+
+enum Stage {
+  ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_Y, ST_BC, ST_PRSRC,
+  ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
+  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_COUNT
+};
+
+}  // namespace c3d
+
+struct c3d_ctx {
+  c3d_params p;
+  c3d::Geom g;
+  int prec;               // C3D_FP64 / C3D_FP32
+  size_t esize;           // bytes per real element
+  hipStream_t stream;
+
+  // host metrics (reference indexing, vectors sized ny+3 / nx+1 / nz+1)
+  std::vector<double> met[C3D_NMETRICS];
+  double dx, dy, dz, rdx, rdy, rdz, rdx2, rdy2, rdz2;
+  double normfft;
+  std::vector<double> s1coe;
+
+  // RK tables and current substep scalars
+  double gconst[3], tconst[3], aconst[3];
+  int iadvance;
+  double dt, pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, SimTime;
+  int cur_ns;
+
+  // device metric storage (one block per precision in use)
+  void* d_met = nullptr;
+  double* d_dyp = nullptr;    // dyp(j), j = 0..ny+2 (fp64 always)
+  c3d::Metrics<double> md{};
+  c3d::Metrics<float> mf{};
+
+  // implicit-operator tables, per substep ns (0..2):
+  //  x/z periodic Green's-function taps, y Thomas factors
+  struct AdiTab {
+    int xlo, xn, zlo, zn;     // tap offset range and count
+    double* d_xtaps;          // device, xn entries (fp64 copy)
+    double* d_ztaps;
+    double* d_yc[3];          // CN ux/uz operator: a, vecm, arr (ny+2 each)
+    double* d_yp[3];          // uy operator
+  } adi[3];
+  void* d_tabs = nullptr;
+
+  // state and scratch
+  void* state[C3D_NSTATE] = {};
+  void* tmpA = nullptr;       // field-shaped scratch (ADI ping-pong)
+  void* spec = nullptr;       // complex spectral array
+  void* sarr = nullptr;       // real spectral-shaped scratch (Thomas factors)
+  double* d_partials = nullptr;
+  int max_partials = 0;
+  c3d::DevScalars* d_sc = nullptr;
+  c3d::DevScalars h_sc{};
+  void* d_tw = nullptr;       // FFT twiddles
+  const void* tw_x = nullptr; // nk entries
+  const void* tw_xr = nullptr;// nk+1 entries (real-FFT post-processing)
+  const void* tw_z = nullptr; // nz entries
+  std::vector<void*> user_fields;
+  size_t dev_bytes = 0;
+
+  // diagnostics bookkeeping: uddxmax slot holding the CFL of the current step
+  bool cfl_valid = false;
+  int cfl_slot = 0;
+  int last_cfl_slot = 0;
+  void* staging = nullptr;    // pinned host staging buffer for upload/download
+  size_t staging_bytes = 0;
+
+  // profiling
+  bool prof = false;
+  struct Pending { int stage; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_free;
+  long prof_calls[c3d::ST_COUNT] = {};
+  double prof_ms[c3d::ST_COUNT] = {};
+  double prof_bytes[c3d::ST_COUNT] = {};
+};
+
+namespace c3d {
+
+// error helpers (c3d_api.hip)
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+#define C3D_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return c3d::hip_fail(e_, #call); \
+  } while (0)
+
+// profiling brackets (c3d_api.hip)
+int prof_begin(c3d_ctx* c, int stage, hipEvent_t* ev);
+int prof_end(c3d_ctx* c, int stage, hipEvent_t ev, double bytes);
+
+// ---- launchers (k_stencil.hip) ----
+int launch_rhs(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
+               const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
+               int want_bulk);
+int launch_bulk_finalize(c3d_ctx* c, int nblocks, int reset_aver);
+int launch_add_dpmdxns(c3d_ctx* c, void* rx);
+int launch_prsrc(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* prsrc,
+                 void* pr, double alpha_override);
+int launch_max_finalize(c3d_ctx* c, int nblocks, int slot);
+int launch_bc_vel(c3d_ctx* c, void* ux, void* uy, void* uz, const void* uy_ym, int full);
+int launch_bc_pr(c3d_ctx* c, void* pr);
+int launch_update_uy_ym(c3d_ctx* c, void* uy_ym, void* duy_ym);
+int launch_proj(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, void* pr,
+                int do_vel, int do_pr, int ghost_from_interior);
+// what: bit0 CFL, bit1 divergence, bit2 vmax, bit3 ux average; cfl_slot selects
+// the uddxmax slot written
+int launch_diag(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int what,
+                int cfl_slot);
+int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot);
+
+// ---- launchers (k_adi.hip) ----
+// comp: 0 ux (CN metrics), 1 uy (node metrics + wall row), 2 uz (CN metrics)
+int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy_ym,
+               int add_dpmdxns);
+
+// ---- launchers (k_fft.hip / k_ppe.hip) ----
+bool fft_size_supported(int n);
+int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec);
+int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale);
+int launch_fft_z(c3d_ctx* c, void* spec, int inverse);
+int launch_tridiag(c3d_ctx* c, void* spec);
+int run_ppe(c3d_ctx* c, void* prsrc, void* phi);
+
+}  // namespace c3d

@@ -1,0 +1,430 @@
+// LDS-staged Stockham FFTs for the pressure Poisson solver
+// (the FFTW R2HC/HC2R transforms of m_Poisson.f90:241-333, 567-688).
+//
+//  - x: real-to-complex of each contiguous row (length nx) through a complex
+//    transform of length nx/2 plus the even/odd split; writes the half
+//    spectrum kx = 0..nx/2 of the row into the spectral array.
+//  - z: complex transform in place along kz for a tile of TK consecutive kx
+//    of one y plane, so every global access is a TK*16-byte contiguous run.
+//  - x inverse: complex-to-real, scaled by normfft, into a field.
+//
+// Transform lengths are compile-time: the radix sequence (8,4,2 then 3,5)
+// is unrolled, each pass stages its butterflies in registers between two
+// barriers, so one LDS buffer per line suffices.  Twiddles come from tables
+// rounded from long-double values on the host.
+#include "c3d_internal.h"
+
+namespace c3d {
+
+template <typename T>
+struct CT;
+template <>
+struct CT<double> { using t = double2; };
+template <>
+struct CT<float> { using t = float2; };
+
+template <typename T2>
+__device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
+  T2 r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
+template <typename T2>
+__device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }
+template <typename T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return mk<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by i*S
+template <int S, typename T2>
+__device__ __forceinline__ T2 mulis(T2 a) { return S > 0 ? mk<T2>(-a.y, a.x) : mk<T2>(a.y, -a.x); }
+
+// ---- small DFTs, y_k = sum_n x_n exp(S 2 pi i n k / R) --------------------
+template <int R, int S, typename T2>
+struct Dft;
+
+template <int S, typename T2>
+struct Dft<2, S, T2> {
+  __device__ __forceinline__ static void run(T2* v) {
+    T2 a = cadd(v[0], v[1]), b = csub(v[0], v[1]);
+    v[0] = a;
+    v[1] = b;
+  }
+};
+
+template <int S, typename T2>
+struct Dft<3, S, T2> {
+  __device__ __forceinline__ static void run(T2* v) {
+    using R_ = decltype(T2::x);
+    const R_ s3 = (R_)0.86602540378443864676372317075293618;
+    T2 t1 = cadd(v[1], v[2]);
+    T2 t2 = mk<T2>(v[0].x - (R_)0.5 * t1.x, v[0].y - (R_)0.5 * t1.y);
+    T2 d = csub(v[1], v[2]);
+    d = mk<T2>(s3 * d.x, s3 * d.y);
+    T2 id = mulis<S>(d);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, id);
+    v[2] = csub(t2, id);
+  }
+};
+
+template <int S, typename T2>
+struct Dft<4, S, T2> {
+  __device__ __forceinline__ static void run(T2* v) {
+    T2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    T2 a2 = cadd(v[1], v[3]), a3 = mulis<S>(csub(v[1], v[3]));
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+  }
+};
+
+template <int S, typename T2>
+struct Dft<5, S, T2> {
+  __device__ __forceinline__ static void run(T2* v) {
+    using R_ = decltype(T2::x);
+    const R_ c1 = (R_)0.30901699437494742410229341718281906;
+    const R_ c2 = (R_)-0.80901699437494742410229341718281906;
+    const R_ s1 = (R_)0.95105651629515357211643933337938214;
+    const R_ s2 = (R_)0.58778525229247312916870595463907277;
+    T2 a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+    T2 b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+    T2 t1 = mk<T2>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    T2 t2 = mk<T2>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    T2 u1 = mulis<S>(mk<T2>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    T2 u2 = mulis<S>(mk<T2>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(t1, u1);
+    v[4] = csub(t1, u1);
+    v[2] = cadd(t2, u2);
+    v[3] = csub(t2, u2);
+  }
+};
+
+template <int S, typename T2>
+struct Dft<8, S, T2> {
+  __device__ __forceinline__ static void run(T2* v) {
+    using R_ = decltype(T2::x);
+    const R_ r2 = (R_)0.70710678118654752440084436210484904;
+    T2 a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      a[n] = cadd(v[n], v[n + 4]);
+      b[n] = csub(v[n], v[n + 4]);
+    }
+    // b_n *= w8^n, w8 = exp(S i pi/4)
+    {
+      T2 x = b[1];
+      b[1] = mk<T2>(r2 * (x.x - (R_)S * x.y), r2 * (x.y + (R_)S * x.x));
+      b[2] = mulis<S>(b[2]);
+      x = b[3];
+      b[3] = mk<T2>(r2 * (-x.x - (R_)S * x.y), r2 * (-x.y + (R_)S * x.x));
+    }
+    Dft<4, S, T2>::run(a);
+    Dft<4, S, T2>::run(b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[2 * q] = a[q];
+      v[2 * q + 1] = b[q];
+    }
+  }
+};
+
+__host__ __device__ constexpr int pick_radix(int m) {
+  return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m % 5 == 0 ? 5 : m;
+}
+
+// One Stockham pass of radix R with current sub-transform size NS over
+// LINES lines of length N held in LDS (line stride LDN).
+template <typename T2, int N, int LINES, int LDN, int NT, int S, int R, int NS>
+__device__ __forceinline__ void stockham_pass(T2* sm, const T2* __restrict__ tw) {
+  constexpr int NR = N / R;
+  constexpr int NB = LINES * NR;
+  constexpr int PER = (NB + NT - 1) / NT;
+  constexpr int TWS = N / (NS * R);
+  const int tid = threadIdx.x;
+  T2 v[PER][R];
+  int od[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int gb = tid + p * NT;
+    od[p] = -1;
+    if (NB % NT == 0 || gb < NB) {
+      const int line = gb / NR;
+      const int j = gb - line * NR;
+      const int base = line * LDN;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[p][r] = sm[base + j + r * NR];
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          T2 w = tw[k * r * TWS];
+          if (S > 0) w.y = -w.y;
+          v[p][r] = cmul(v[p][r], w);
+        }
+      }
+      Dft<R, S, T2>::run(v[p]);
+      od[p] = base + (j / NS) * NS * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    if (od[p] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) sm[od[p] + r * NS] = v[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T2, int N, int LINES, int LDN, int NT, int S, int NS>
+__device__ __forceinline__ void stockham(T2* sm, const T2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = pick_radix(N / NS);
+    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported FFT length");
+    stockham_pass<T2, N, LINES, LDN, NT, S, R, NS>(sm, tw);
+    stockham<T2, N, LINES, LDN, NT, S, NS * R>(sm, tw);
+  }
+}
+
+constexpr int FNT = 256;
+
+__host__ __device__ constexpr int xlines(int N) { return N <= 128 ? 16 : N <= 256 ? 8 : N <= 512 ? 8 : 4; }
+constexpr int ZTK = 8;
+
+// ---------------------------------------------------------------------------
+// x forward: field rows (jj = 1..ny) -> spectral rows, kx = 0..N (N = nx/2)
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const T* __restrict__ in,
+                                                   typename CT<T>::t* __restrict__ spec,
+                                                   const typename CT<T>::t* __restrict__ tw,
+                                                   const typename CT<T>::t* __restrict__ twr) {
+  using T2 = typename CT<T>::t;
+  constexpr int LINES = xlines(N);
+  constexpr int LDN = N + 1;
+  __shared__ T2 sm[LINES * LDN];
+  const int nrows = g.ny * g.nz;
+  const int row0 = blockIdx.x * LINES;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < LINES * N; e += FNT) {
+    const int l = e / N, m = e - l * N, row = row0 + l;
+    T2 v = mk<T2>(0, 0);
+    if (row < nrows) {
+      const int j0 = row % g.ny, k = row / g.ny;
+      const T2* rp = (const T2*)(in + (long long)g.nx * (j0 + 1) + g.plane * k);
+      v = rp[m];
+    }
+    sm[l * LDN + m] = v;
+  }
+  __syncthreads();
+  stockham<T2, N, LINES, LDN, FNT, -1, 1>(sm, tw);
+  for (int e = tid; e < LINES * (N + 1); e += FNT) {
+    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
+    if (row >= nrows) continue;
+    const T2 zk = sm[l * LDN + (k == N ? 0 : k)];
+    const T2 zn = sm[l * LDN + (k == 0 ? 0 : N - k)];
+    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
+    const T dr = zk.x - zn.x, di = zk.y + zn.y;
+    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
+    const T2 wo = cmul(twr[k], o);
+    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x inverse: spectral rows -> field rows (jj = 1..ny), times scale
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>::t* __restrict__ spec,
+                                                   T* __restrict__ out,
+                                                   const typename CT<T>::t* __restrict__ tw,
+                                                   const typename CT<T>::t* __restrict__ twr,
+                                                   T scale) {
+  using T2 = typename CT<T>::t;
+  constexpr int LINES = xlines(N);
+  constexpr int LDN = N + 1;
+  __shared__ T2 sm[LINES * LDN];
+  const int nrows = g.ny * g.nz;
+  const int row0 = blockIdx.x * LINES;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < LINES * N; e += FNT) {
+    const int l = e / N, k = e - l * N, row = row0 + l;
+    T2 z = mk<T2>(0, 0);
+    if (row < nrows) {
+      const T2* sp = spec + (long long)g.P * row;
+      T2 xk = sp[k], xn = sp[N - k];
+      if (k == 0) {
+        xk.y = 0;
+        xn.y = 0;
+      }
+      const T2 a = mk<T2>(xk.x + xn.x, xk.y - xn.y);
+      const T2 b = mk<T2>(xk.x - xn.x, xk.y + xn.y);
+      T2 w = twr[k];
+      w.y = -w.y;
+      const T2 wb = cmul(w, b);
+      z = mk<T2>(a.x - wb.y, a.y + wb.x);
+    }
+    sm[l * LDN + k] = z;
+  }
+  __syncthreads();
+  stockham<T2, N, LINES, LDN, FNT, 1, 1>(sm, tw);
+  for (int e = tid; e < LINES * N; e += FNT) {
+    const int l = e / N, m = e - l * N, row = row0 + l;
+    if (row >= nrows) continue;
+    const int j0 = row % g.ny, k = row / g.ny;
+    T2* rp = (T2*)(out + (long long)g.nx * (j0 + 1) + g.plane * k);
+    const T2 z = sm[l * LDN + m];
+    rp[m] = mk<T2>(z.x * scale, z.y * scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z, in place: TK consecutive kx of one y plane, all kz
+// ---------------------------------------------------------------------------
+template <typename T, int N, int S>
+__global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __restrict__ spec,
+                                               const typename CT<T>::t* __restrict__ tw) {
+  using T2 = typename CT<T>::t;
+  constexpr int TK = ZTK;
+  constexpr int LDN = N + 1;
+  __shared__ T2 sm[TK * LDN];
+  const int kx0 = blockIdx.x * TK;
+  const int j0 = blockIdx.y;
+  const int nkx = g.nk + 1;
+  const long long sk = (long long)g.P * g.ny;
+  T2* base = spec + (long long)g.P * j0;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TK * N; e += FNT) {
+    const int kz = e / TK, l = e - kz * TK, kx = kx0 + l;
+    T2 v = mk<T2>(0, 0);
+    if (kx < nkx) v = base[sk * kz + kx];
+    sm[l * LDN + kz] = v;
+  }
+  __syncthreads();
+  stockham<T2, N, TK, LDN, FNT, S, 1>(sm, tw);
+  for (int e = tid; e < TK * N; e += FNT) {
+    const int kz = e / TK, l = e - kz * TK, kx = kx0 + l;
+    if (kx < nkx) base[sk * kz + kx] = sm[l * LDN + kz];
+  }
+}
+
+// ===========================================================================
+// size dispatch
+// ===========================================================================
+#define C3D_FFT_SIZES(X)                                                                       \
+  X(8) X(12) X(16) X(18) X(20) X(24) X(32) X(36) X(40) X(48) X(64) X(72) X(80) X(96) X(128)    \
+  X(144) X(160) X(192) X(256) X(288) X(320) X(384) X(512) X(576) X(640) X(768) X(1024)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define X(n_) case n_:
+    C3D_FFT_SIZES(X)
+#undef X
+    return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const int nrows = g.ny * g.nz;
+  switch (g.nk) {
+#define X(n_)                                                                                    \
+  case n_:                                                                                       \
+    hipLaunchKernelGGL((k_fft_x_fwd<T, n_>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
+                       dim3(FNT), 0, c->stream, g, (const T*)in, (T2*)spec, (const T2*)c->tw_x, \
+                       (const T2*)c->tw_xr);                                                     \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <typename T>
+static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const int nrows = g.ny * g.nz;
+  switch (g.nk) {
+#define X(n_)                                                                                    \
+  case n_:                                                                                       \
+    hipLaunchKernelGGL((k_fft_x_inv<T, n_>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
+                       dim3(FNT), 0, c->stream, g, (const T2*)spec, (T*)out, (const T2*)c->tw_x, \
+                       (const T2*)c->tw_xr, (T)scale);                                           \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <typename T>
+static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny);
+  switch (g.nz) {
+#define X(n_)                                                                                  \
+  case n_:                                                                                     \
+    if (inverse)                                                                               \
+      hipLaunchKernelGGL((k_fft_z<T, n_, 1>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,    \
+                         (const T2*)c->tw_z);                                                  \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_fft_z<T, n_, -1>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,   \
+                         (const T2*)c->tw_z);                                                  \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_z: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec) {
+  hipEvent_t ev;
+  prof_begin(c, ST_FFT_X_FWD, &ev);
+  int rc = c->prec == C3D_FP64 ? fft_x_fwd_t<double>(c, in, spec) : fft_x_fwd_t<float>(c, in, spec);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_FFT_X_FWD, ev, 2.0 * N * c->esize);
+  return C3D_OK;
+}
+
+int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale) {
+  hipEvent_t ev;
+  prof_begin(c, ST_FFT_X_INV, &ev);
+  int rc = c->prec == C3D_FP64 ? fft_x_inv_t<double>(c, spec, out, scale)
+                               : fft_x_inv_t<float>(c, spec, out, scale);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_FFT_X_INV, ev, 2.0 * N * c->esize);
+  return C3D_OK;
+}
+
+int launch_fft_z(c3d_ctx* c, void* spec, int inverse) {
+  hipEvent_t ev;
+  const int st = inverse ? ST_FFT_Z_INV : ST_FFT_Z_FWD;
+  prof_begin(c, st, &ev);
+  int rc = c->prec == C3D_FP64 ? fft_z_t<double>(c, spec, inverse) : fft_z_t<float>(c, spec, inverse);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, st, ev, 2.0 * N * c->esize);
+  return C3D_OK;
+}
+
+}  // namespace c3d

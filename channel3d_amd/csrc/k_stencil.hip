@@ -1,0 +1,737 @@
+// Stencil kernels of the channel step: convection/diffusion right-hand sides,
+// pressure source, projection, wall ghosts and the per-step reductions.
+//
+// One thread per cell (i fastest across a 64-wide wavefront, 4 rows of j per
+// workgroup), each thread marching a short run of z planes.  x and z
+// neighbours wrap periodically by index; y neighbours read the stored wall
+// ghost planes exactly as the reference reads its halo arrays.
+#include "c3d_internal.h"
+
+namespace c3d {
+
+constexpr int BX = 64, BY = 4, KCH = 8;
+
+template <typename T>
+struct Fld {
+  T* p;
+};
+
+__device__ __forceinline__ int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+  // wavefront reduction by shuffles, then across the 4 wavefronts
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, w = tid >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (tid == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    for (int q = 0; q < nw; ++q) r += sh[q];
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, w = tid >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (tid == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    for (int q = 0; q < nw; ++q) r = fmax(r, sh[q]);
+  }
+  return r;
+}
+
+// block reduction of reproducible-sum limbs; result valid in thread 0
+__device__ __forceinline__ void block_sum_fix(long long* v, long long (*sh)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, w = tid >> 6;
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q) sh[w][q] = v[q];
+  __syncthreads();
+  if (tid == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    for (int q = 0; q < 4; ++q) {
+      long long a = 0;
+      for (int u = 0; u < nw; ++u) a += sh[u][q];
+      v[q] = a;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand sides of the three momentum equations, m_TSchemeFIMP_inc.f90:
+// clcRhsX_FIMP :7-75, clcRhsY_FIMP :80-129, clcRhsZ_FIMP :134-182.
+// MASK bit 0/1/2 selects the x/y/z component; the x component also writes
+// the per-block partial of the bulk-velocity integral s1tot (:46).
+// ---------------------------------------------------------------------------
+template <typename T, int MASK>
+__global__ __launch_bounds__(BX* BY) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
+                                                const T* __restrict__ ux, const T* __restrict__ uy,
+                                                const T* __restrict__ uz, const T* __restrict__ pr,
+                                                T* __restrict__ rx, T* __restrict__ ry,
+                                                T* __restrict__ rz, T* __restrict__ hx,
+                                                T* __restrict__ hy, T* __restrict__ hz,
+                                                double* __restrict__ partials) {
+  __shared__ long long shf[8][4];
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int jc = blockIdx.y * BY + threadIdx.y + 1;  // reference j
+  const int k0 = blockIdx.z * KCH;
+  long long s1[4] = {0, 0, 0, 0};
+  if (i < g.nx && jc <= g.ny) {
+    const int nx = g.nx, nz = g.nz;
+    const long long sj = nx, sk = g.plane;
+    const int im = wrapm(i, nx), ip = wrapp(i, nx);
+    const int jm = jc - 1, jp = jc + 1;
+    const T I1 = m.yint[jc], I2 = T(1) - I1, I3 = m.yint[jp], I4 = T(1) - I3;
+    const T qdx1 = T(0.25) * cf.rdx, qdx3 = T(0.25) * cf.rdz;
+    const T hdx1 = T(0.5) * cf.rdx, hdx3 = T(0.5) * cf.rdz;
+    const T sucaj = T(0.5) * m.rdyp[jc];
+    const T sucac = m.rdyc[jc], qsucac = T(0.25) * sucac;
+    const T am1c = cf.rdx2, ap1c = cf.rdx2, ac1c = -(am1c + ap1c);
+    const T am3c = cf.rdz2, ap3c = cf.rdz2, ac3c = -(am3c + ap3c);
+    const T s1coe = m.s1coe[jc];
+    const int k1 = min(nz, k0 + KCH);
+    for (int k = k0; k < k1; ++k) {
+      const int km = wrapm(k, nz), kp = wrapp(k, nz);
+      const long long rowc = sj * jc + sk * k;  // (jc, k)
+      const long long rowm = rowc - sj, rowp = rowc + sj;
+      const long long rowkm = sj * jc + sk * km, rowkp = sj * jc + sk * kp;
+      const long long c = rowc + i;
+      if (MASK & 1) {
+        const T u_c = ux[c], u_xp = ux[rowc + ip], u_xm = ux[rowc + im];
+        const T u_yp = ux[rowp + i], u_ym = ux[rowm + i];
+        const T u_zp = ux[rowkp + i], u_zm = ux[rowkm + i];
+        const T h11 = ((u_xp + u_c) * (u_xp + u_c) - (u_c + u_xm) * (u_c + u_xm)) * qdx1;
+        const T h12 = ((uy[rowp + i] + uy[rowp + im]) * (I3 * u_c + I4 * u_yp) -
+                       (uy[c] + uy[rowc + im]) * (I1 * u_ym + I2 * u_c)) *
+                      sucaj;
+        const T h13 = ((uz[rowkp + i] + uz[rowkp + im]) * (u_zp + u_c) -
+                       (uz[c] + uz[rowc + im]) * (u_c + u_zm)) *
+                      qdx3;
+        const T d11 = (u_xp - T(2) * u_c + u_xm) * cf.rdx2;
+        const T d22 = m.ap2c[jc] * u_yp + m.ac2c[jc] * u_c + m.am2c[jc] * u_ym;
+        const T d33 = ap3c * u_zp + ac3c * u_c + am3c * u_zm;
+        const T dcq = d11 + d22 + d33;
+        const T gradp = (pr[c] - pr[rowc + im]) * cf.rdx;
+        fix_add(s1, (double)(u_c * s1coe));
+        const T conv = -h11 - h12 - h13 + cf.gx;
+        rx[c] = cf.pmGamma * conv + cf.pmTheta * hx[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        hx[c] = conv;
+      }
+      if (MASK & 2) {
+        const T v_c = uy[c], v_xp = uy[rowc + ip], v_xm = uy[rowc + im];
+        const T v_yp = uy[rowp + i], v_ym = uy[rowm + i];
+        const T v_zp = uy[rowkp + i], v_zm = uy[rowkm + i];
+        const T h21 = ((I1 * ux[rowm + ip] + I2 * ux[rowc + ip]) * (v_xp + v_c) -
+                       (I1 * ux[rowm + i] + I2 * ux[c]) * (v_c + v_xm)) *
+                      hdx1;
+        const T h22 = ((v_yp + v_c) * (v_yp + v_c) - (v_c + v_ym) * (v_c + v_ym)) * qsucac;
+        const T h23 = ((I1 * uz[rowkp - sj + i] + I2 * uz[rowkp + i]) * (v_zp + v_c) -
+                       (I1 * uz[rowm + i] + I2 * uz[c]) * (v_c + v_zm)) *
+                      hdx3;
+        const T d11 = ap1c * v_xp + ac1c * v_c + am1c * v_xm;
+        const T d22 = m.ap2p[jc] * v_yp + m.ac2p[jc] * v_c + m.am2p[jc] * v_ym;
+        const T d33 = ap3c * v_zp + ac3c * v_c + am3c * v_zm;
+        const T dcq = d11 + d22 + d33;
+        const T gradp = (pr[c] - pr[rowm + i]) * sucac;
+        const T conv = -h21 - h22 - h23 + cf.gy;
+        ry[c] = cf.pmGamma * conv + cf.pmTheta * hy[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        hy[c] = conv;
+      }
+      if (MASK & 4) {
+        const T w_c = uz[c], w_xp = uz[rowc + ip], w_xm = uz[rowc + im];
+        const T w_yp = uz[rowp + i], w_ym = uz[rowm + i];
+        const T w_zp = uz[rowkp + i], w_zm = uz[rowkm + i];
+        const T h31 = ((ux[rowc + ip] + ux[rowkm + ip]) * (w_xp + w_c) -
+                       (ux[c] + ux[rowkm + i]) * (w_c + w_xm)) *
+                      qdx1;
+        const T h32 = ((uy[rowp + i] + uy[rowkm + sj + i]) * (I3 * w_c + I4 * w_yp) -
+                       (uy[c] + uy[rowkm + i]) * (I1 * w_ym + I2 * w_c)) *
+                      sucaj;
+        const T h33 = ((w_zp + w_c) * (w_zp + w_c) - (w_c + w_zm) * (w_c + w_zm)) * qdx3;
+        const T d11 = ap1c * w_xp + ac1c * w_c + am1c * w_xm;
+        const T d22 = m.ap2c[jc] * w_yp + m.ac2c[jc] * w_c + m.am2c[jc] * w_ym;
+        const T d33 = (w_zp - T(2) * w_c + w_zm) * cf.rdz2;
+        const T dcq = d11 + d22 + d33;
+        const T gradp = (pr[c] - pr[rowkm + i]) * cf.rdz;
+        const T conv = -h31 - h32 - h33 + cf.gz;
+        rz[c] = cf.pmGamma * conv + cf.pmTheta * hz[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        hz[c] = conv;
+      }
+    }
+  }
+  if (MASK & 1) {
+    block_sum_fix(s1, shf);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      long long* pf = (long long*)partials +
+                      4LL * (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+      for (int q = 0; q < 4; ++q) pf[q] = s1[q];
+    }
+  }
+}
+
+// Constant flow rate: s1tot -> dpmdxns, PrGradNow, PrGradAver
+// (m_TSchemeFIMP_inc.f90:61-74).  One workgroup, fixed summation order.
+__global__ void k_bulk_finalize(const double* __restrict__ partials, int n, DevScalars* sc,
+                                double nxnz, double ybulk_len, double ubulk, double alphaC,
+                                double dt, int reset_aver) {
+  __shared__ long long sh[4][1024];
+  const int t = threadIdx.x;
+  const long long* pf = (const long long*)partials;
+  long long a[4] = {0, 0, 0, 0};
+  for (int q = t; q < n; q += blockDim.x)
+    for (int l = 0; l < 4; ++l) a[l] += pf[4LL * q + l];
+  for (int l = 0; l < 4; ++l) sh[l][t] = a[l];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int l = 0; l < 4; ++l) sh[l][t] += sh[l][t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const long long tot[4] = {sh[0][0], sh[1][0], sh[2][0], sh[3][0]};
+    const double s1tot1 = fix_value(tot) / nxnz / ybulk_len;
+    sc->s1tot1 = s1tot1;
+    sc->dpmdxns = alphaC * (ubulk - s1tot1);
+    sc->PrGradNow = (ubulk - s1tot1) / dt;
+    const double aver = reset_aver ? 0.0 : sc->PrGradAver;
+    sc->PrGradAver = aver + alphaC * sc->PrGradNow;
+  }
+}
+
+template <typename T>
+__global__ void k_add_scalar(Geom g, T* __restrict__ f, const DevScalars* sc) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  if (i >= g.nx || jc > g.ny) return;
+  const T d = (T)sc->dpmdxns;
+  const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
+  for (int k = blockIdx.z * KCH; k < k1; ++k) f[fidx(g, i, jc, k)] += d;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure source: clcPrSrc_FIMP (m_TScheme.f90:246-276).  alpha is the
+// time factor pmAlpha; the pressure is corrected by -nu/2 * div.
+// ---------------------------------------------------------------------------
+template <typename T, bool PCORR>
+__global__ __launch_bounds__(BX* BY) void k_prsrc(Geom g, Metrics<T> m, T rdx, T rdz, T sudtal,
+                                                  T xnuhm, const T* __restrict__ ux,
+                                                  const T* __restrict__ uy,
+                                                  const T* __restrict__ uz, T* __restrict__ prsrc,
+                                                  T* __restrict__ pr, double* __restrict__ partials) {
+  __shared__ double sh[8];
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  double dm = 0.0;
+  if (i < g.nx && jc <= g.ny) {
+    const int ip = wrapp(i, g.nx);
+    const T sucaj = m.rdyp[jc];
+    const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
+    for (int k = blockIdx.z * KCH; k < k1; ++k) {
+      const int kp = wrapp(k, g.nz);
+      const long long rowc = (long long)g.nx * jc + g.plane * k;
+      const long long c = rowc + i;
+      const T rdiv = (ux[rowc + ip] - ux[c]) * rdx + (uy[c + g.nx] - uy[c]) * sucaj +
+                     (uz[(long long)g.nx * jc + g.plane * kp + i] - uz[c]) * rdz;
+      dm = fmax(dm, (double)fabs(rdiv));
+      prsrc[c] = sudtal * rdiv;
+      if (PCORR) pr[c] = pr[c] + xnuhm * rdiv;
+    }
+  }
+  const double r = block_max(dm, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = r;
+}
+
+__global__ void k_max_finalize(const double* __restrict__ partials, int n, double* out) {
+  __shared__ double sh[1024];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int q = t; q < n; q += blockDim.x) s = fmax(s, partials[q]);
+  sh[t] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] = fmax(sh[t], sh[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) *out = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// Wall ghosts, m_BC_and_Halo.f90.  full=1: SetBC_and_UpdateHalo (:232-344);
+// full=0: SetBC_and_UpdateHaloForPrSrc (:349-405, only the uy wall nodes).
+// One thread per (i,k) wall column.
+// ---------------------------------------------------------------------------
+struct WallBC {
+  int bcym, bcyp;
+  double uxm, uym_unused, uzm, uxp, uyp, uzp;  // wall values
+  double dyp1, dypn;                           // dyp(1), dyp(nyp)
+};
+
+template <typename T>
+__global__ void k_bc_vel(Geom g, WallBC b, T* __restrict__ ux, T* __restrict__ uy,
+                         T* __restrict__ uz, const T* __restrict__ uy_ym, int full) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= g.nx || k >= g.nz) return;
+  const int ny = g.ny, nyp = ny + 1;
+  const T v_ym = uy_ym[i + (long long)g.nx * k];
+  if (!full) {
+    uy[fidx(g, i, nyp, k)] = (T)b.uyp;
+    uy[fidx(g, i, 1, k)] = v_ym;
+    return;
+  }
+  // yp
+  if (b.bcyp == C3D_BC_NOSLIP) {
+    ux[fidx(g, i, nyp, k)] = (T)b.uxp * T(2) - ux[fidx(g, i, ny, k)];
+    uy[fidx(g, i, nyp, k)] = (T)b.uyp;
+    uy[fidx(g, i, nyp + 1, k)] = (T)b.uyp * T(2) - uy[fidx(g, i, ny, k)];
+    uz[fidx(g, i, nyp, k)] = (T)b.uzp * T(2) - uz[fidx(g, i, ny, k)];
+  } else {
+    ux[fidx(g, i, nyp, k)] = (T)b.uxp * (T)b.dypn + ux[fidx(g, i, ny, k)];
+    uy[fidx(g, i, nyp, k)] = (T)b.uyp;
+    uy[fidx(g, i, nyp + 1, k)] = (T)b.uyp * T(2) - uy[fidx(g, i, ny, k)];
+    uz[fidx(g, i, nyp, k)] = (T)b.uzp * (T)b.dypn + uz[fidx(g, i, ny, k)];
+  }
+  // ym
+  if (b.bcym == C3D_BC_NOSLIP) {
+    ux[fidx(g, i, 0, k)] = (T)b.uxm * T(2) - ux[fidx(g, i, 1, k)];
+    uy[fidx(g, i, 1, k)] = v_ym;
+    uy[fidx(g, i, 0, k)] = v_ym * T(2) - uy[fidx(g, i, 2, k)];
+    uz[fidx(g, i, 0, k)] = (T)b.uzm * T(2) - uz[fidx(g, i, 1, k)];
+  } else {
+    ux[fidx(g, i, 0, k)] = ux[fidx(g, i, 1, k)] - (T)b.uxm * (T)b.dyp1;
+    uy[fidx(g, i, 1, k)] = v_ym;
+    uy[fidx(g, i, 0, k)] = v_ym * T(2) - uy[fidx(g, i, 2, k)];
+    uz[fidx(g, i, 0, k)] = uz[fidx(g, i, 1, k)] - (T)b.uzm * (T)b.dyp1;
+  }
+}
+
+// SetBC_and_UpdateHalo_pr (m_BC_and_Halo.f90:410-456): Neumann ghosts
+template <typename T>
+__global__ void k_bc_pr(Geom g, T* __restrict__ pr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= g.nx || k >= g.nz) return;
+  pr[fidx(g, i, g.ny + 1, k)] = pr[fidx(g, i, g.ny, k)];
+  pr[fidx(g, i, 0, k)] = pr[fidx(g, i, 1, k)];
+}
+
+// Update_uy_ym_CH (m_FlowType_Channel.f90:120-130)
+template <typename T>
+__global__ void k_update_uy_ym(int n, T uybc, T* __restrict__ uy_ym, T* __restrict__ duy_ym) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  T d = uy_ym[q];
+  uy_ym[q] = uybc;
+  duy_ym[q] = uybc - d;
+}
+
+// ---------------------------------------------------------------------------
+// Projection: FluidVelUpdate (m_TScheme.f90:281-306) and PressureUpdate_FIMP
+// (m_TSchemeFIMP_inc.f90:1371-1386).  With ghost_from_interior the Neumann
+// ghost phi(0)=phi(1) of SetBC_and_UpdateHalo_pr is applied on the fly.
+// ---------------------------------------------------------------------------
+template <typename T, bool VEL, bool PRS, bool GHOST>
+__global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2, T rdzEta2,
+                                                 T alpha, const T* __restrict__ phi,
+                                                 T* __restrict__ ux, T* __restrict__ uy,
+                                                 T* __restrict__ uz, T* __restrict__ pr) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  if (i >= g.nx || jc > g.ny) return;
+  const int im = wrapm(i, g.nx);
+  const T sucacEta2 = m.rdyc[jc] * alpha;
+  const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
+  for (int k = blockIdx.z * KCH; k < k1; ++k) {
+    const int km = wrapm(k, g.nz);
+    const long long rowc = (long long)g.nx * jc + g.plane * k;
+    const long long c = rowc + i;
+    const T locphi = phi[c];
+    if (VEL) {
+      const T phx = phi[rowc + im];
+      const T phy = (GHOST && jc == 1) ? locphi : phi[c - g.nx];
+      const T phz = phi[(long long)g.nx * jc + g.plane * km + i];
+      ux[c] = ux[c] - (locphi - phx) * rdxEta2;
+      uy[c] = uy[c] - (locphi - phy) * sucacEta2;
+      uz[c] = uz[c] - (locphi - phz) * rdzEta2;
+    }
+    if (PRS) pr[c] = pr[c] + locphi;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (m_Tools.f90): CalcMaxCFL :21-46, CheckDivergence :51-75,
+// CalcVmax :80-104, CalcUxAver :250-270.  One pass; WHAT selects outputs.
+// partials layout: [6][nblocks]: cfl, div, vx, vy, vz, uxsum
+// ---------------------------------------------------------------------------
+template <typename T, int WHAT>
+__global__ __launch_bounds__(BX* BY) void k_diag(Geom g, Metrics<T> m, T rdx, T rdz,
+                                                 const T* __restrict__ ux,
+                                                 const T* __restrict__ uy,
+                                                 const T* __restrict__ uz,
+                                                 const double* __restrict__ dyp,
+                                                 double* __restrict__ partials, int nblocks) {
+  __shared__ double sh[8];
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  double cfl = 0, dv = 0, vx = 0, vy = 0, vz = 0;
+  long long us[4] = {0, 0, 0, 0};
+  if (i < g.nx && jc <= g.ny) {
+    const int ip = wrapp(i, g.nx);
+    const T rdyp = m.rdyp[jc];
+    const double dy = dyp[jc];
+    const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
+    for (int k = blockIdx.z * KCH; k < k1; ++k) {
+      const int kp = wrapp(k, g.nz);
+      const long long rowc = (long long)g.nx * jc + g.plane * k;
+      const long long c = rowc + i;
+      const T u = ux[c], v = uy[c], w = uz[c];
+      const T uxp = ux[rowc + ip], vyp = uy[c + g.nx];
+      const T wzp = uz[(long long)g.nx * jc + g.plane * kp + i];
+      if (WHAT & 1) {
+        const double vcf = (double)(fabs(u + uxp) * rdx + fabs(v + vyp) * rdyp + fabs(w + wzp) * rdz);
+        cfl = fmax(cfl, vcf);
+      }
+      if (WHAT & 2) {
+        const double d = (double)fabs((uxp - u) * rdx + (vyp - v) * rdyp + (wzp - w) * rdz);
+        dv = fmax(dv, d);
+      }
+      if (WHAT & 4) {
+        vx = fmax(vx, (double)fabs(u));
+        vy = fmax(vy, (double)fabs(v));
+        vz = fmax(vz, (double)fabs(w));
+      }
+      if (WHAT & 8) fix_add(us, (double)u * dy);
+    }
+  }
+  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  double r;
+  if (WHAT & 1) { r = block_max(cfl, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[0 * (long long)nblocks + b] = r; __syncthreads(); }
+  if (WHAT & 2) { r = block_max(dv, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[1 * (long long)nblocks + b] = r; __syncthreads(); }
+  if (WHAT & 4) {
+    r = block_max(vx, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[2 * (long long)nblocks + b] = r; __syncthreads();
+    r = block_max(vy, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[3 * (long long)nblocks + b] = r; __syncthreads();
+    r = block_max(vz, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[4 * (long long)nblocks + b] = r; __syncthreads();
+  }
+  if (WHAT & 8) {
+    __shared__ long long shf[8][4];
+    block_sum_fix(us, shf);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      long long* pf = (long long*)(partials + 5LL * nblocks) + 4LL * b;
+      for (int q = 0; q < 4; ++q) pf[q] = us[q];
+    }
+  }
+}
+
+__global__ void k_diag_finalize(const double* __restrict__ partials, int n, int what,
+                                DevScalars* sc, double uxaver_div, int cfl_slot) {
+  __shared__ double sh[5][256];
+  __shared__ long long shl[4][256];
+  const int t = threadIdx.x;
+  double a[5] = {0, 0, 0, 0, 0};
+  long long lm[4] = {0, 0, 0, 0};
+  const long long* pf = (const long long*)(partials + 5LL * n);
+  for (int q = t; q < n; q += blockDim.x) {
+    for (int s = 0; s < 5; ++s)
+      if (what & (s == 0 ? 1 : s == 1 ? 2 : 4)) a[s] = fmax(a[s], partials[(long long)s * n + q]);
+    if (what & 8)
+      for (int l = 0; l < 4; ++l) lm[l] += pf[4LL * q + l];
+  }
+  for (int s = 0; s < 5; ++s) sh[s][t] = a[s];
+  for (int l = 0; l < 4; ++l) shl[l][t] = lm[l];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      for (int s = 0; s < 5; ++s) sh[s][t] = fmax(sh[s][t], sh[s][t + w]);
+      for (int l = 0; l < 4; ++l) shl[l][t] += shl[l][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (what & 1) sc->uddxmax[cfl_slot] = sh[0][0] * 0.5;
+    if (what & 2) sc->divmax2 = sh[1][0];
+    if (what & 4) { sc->vmax[0] = sh[2][0]; sc->vmax[1] = sh[3][0]; sc->vmax[2] = sh[4][0]; }
+    if (what & 8) {
+      const long long tot[4] = {shl[0][0], shl[1][0], shl[2][0], shl[3][0]};
+      sc->uxaver = fix_value(tot) / uxaver_div;
+    }
+  }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static inline dim3 cell_grid(const Geom& g) {
+  return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + KCH - 1) / KCH);
+}
+static inline int ncell_blocks(const Geom& g) {
+  dim3 d = cell_grid(g);
+  return (int)(d.x * d.y * d.z);
+}
+
+template <typename T>
+static Coef<T> make_coef(c3d_ctx* c) {
+  Coef<T> cf;
+  cf.pmGamma = (T)c->pmGamma; cf.pmTheta = (T)c->pmTheta;
+  cf.pmAlpha = (T)c->pmAlpha; cf.pmBeta = (T)c->pmBeta;
+  cf.rdx = (T)c->rdx; cf.rdz = (T)c->rdz; cf.rdx2 = (T)c->rdx2; cf.rdz2 = (T)c->rdz2;
+  cf.gx = (T)c->p.gravity[0]; cf.gy = (T)c->p.gravity[1]; cf.gz = (T)c->p.gravity[2];
+  cf.xnu = (T)c->p.xnu;
+  return cf;
+}
+
+template <typename T>
+static const Metrics<T>& mets(c3d_ctx* c);
+template <>
+const Metrics<double>& mets<double>(c3d_ctx* c) { return c->md; }
+template <>
+const Metrics<float>& mets<float>(c3d_ctx* c) { return c->mf; }
+
+template <typename T>
+static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
+                 const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz) {
+  const Geom& g = c->g;
+  dim3 grid = cell_grid(g), blk(BX, BY);
+  Coef<T> cf = make_coef<T>(c);
+  const Metrics<T>& m = mets<T>(c);
+  auto U = [](const void* p) { return (const T*)p; };
+  auto W = [](void* p) { return (T*)p; };
+#define RHS_LAUNCH(M)                                                                         \
+  hipLaunchKernelGGL((k_rhs<T, M>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz),  \
+                     U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials)
+  switch (mask) {
+    case 1: RHS_LAUNCH(1); break;
+    case 2: RHS_LAUNCH(2); break;
+    case 4: RHS_LAUNCH(4); break;
+    case 7: RHS_LAUNCH(7); break;
+    default: set_error("rhs: bad component mask"); return C3D_ERR_ARG;
+  }
+#undef RHS_LAUNCH
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_rhs(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
+               const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
+               int want_bulk) {
+  hipEvent_t ev;
+  const int st = mask == 7 ? ST_RHS : ST_RHS1;
+  prof_begin(c, st, &ev);
+  int rc = c->prec == C3D_FP64 ? rhs_t<double>(c, mask, ux, uy, uz, pr, rx, ry, rz, hx, hy, hz)
+                               : rhs_t<float>(c, mask, ux, uy, uz, pr, rx, ry, rz, hx, hy, hz);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const int ncomp = (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1);
+  prof_end(c, st, ev, (ncomp == 3 ? 13.0 : 6.0) * N * c->esize);
+  if (want_bulk && (mask & 1) && c->p.is_ux_const)
+    return launch_bulk_finalize(c, ncell_blocks(c->g), c->cur_ns == 1);
+  return C3D_OK;
+}
+
+int launch_bulk_finalize(c3d_ctx* c, int nblocks, int reset_aver) {
+  hipEvent_t ev;
+  prof_begin(c, ST_BULK, &ev);
+  hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(1024), 0, c->stream, c->d_partials, nblocks,
+                     c->d_sc, (double)c->g.nx * (double)c->g.nz, c->p.ybulk2 - c->p.ybulk1,
+                     c->p.ubulk, c->pmAlphaC, c->dt, reset_aver);
+  C3D_HIP(hipGetLastError());
+  prof_end(c, ST_BULK, ev, 0.0);
+  return C3D_OK;
+}
+
+int launch_add_dpmdxns(c3d_ctx* c, void* rx) {
+  const Geom& g = c->g;
+  if (c->prec == C3D_FP64)
+    hipLaunchKernelGGL(k_add_scalar<double>, cell_grid(g), dim3(BX, BY), 0, c->stream, g,
+                       (double*)rx, (const DevScalars*)c->d_sc);
+  else
+    hipLaunchKernelGGL(k_add_scalar<float>, cell_grid(g), dim3(BX, BY), 0, c->stream, g,
+                       (float*)rx, (const DevScalars*)c->d_sc);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <typename T>
+static int prsrc_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* prsrc,
+                   void* pr, double alpha) {
+  const Geom& g = c->g;
+  const T sudtal = (T)(1.0 / alpha);
+  const T xnuhm = (T)(-0.5 * c->p.xnu);
+  if (pr)
+    hipLaunchKernelGGL((k_prsrc<T, true>), cell_grid(g), dim3(BX, BY), 0, c->stream, g,
+                       mets<T>(c), (T)c->rdx, (T)c->rdz, sudtal, xnuhm, (const T*)ux,
+                       (const T*)uy, (const T*)uz, (T*)prsrc, (T*)pr, c->d_partials);
+  else
+    hipLaunchKernelGGL((k_prsrc<T, false>), cell_grid(g), dim3(BX, BY), 0, c->stream, g,
+                       mets<T>(c), (T)c->rdx, (T)c->rdz, sudtal, xnuhm, (const T*)ux,
+                       (const T*)uy, (const T*)uz, (T*)prsrc, (T*)nullptr, c->d_partials);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_prsrc(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* prsrc,
+                 void* pr, double alpha_override) {
+  hipEvent_t ev;
+  prof_begin(c, ST_PRSRC, &ev);
+  const double alpha = alpha_override > 0 ? alpha_override : c->pmAlpha;
+  int rc = c->prec == C3D_FP64 ? prsrc_t<double>(c, ux, uy, uz, prsrc, pr, alpha)
+                               : prsrc_t<float>(c, ux, uy, uz, prsrc, pr, alpha);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_PRSRC, ev, (pr ? 6.0 : 4.0) * N * c->esize);
+  return launch_max_finalize(c, ncell_blocks(c->g), 0);
+}
+
+int launch_max_finalize(c3d_ctx* c, int nblocks, int slot) {
+  double* out = slot == 0 ? &c->d_sc->divmax1 : &c->d_sc->divmax2;
+  hipLaunchKernelGGL(k_max_finalize, dim3(1), dim3(1024), 0, c->stream, c->d_partials, nblocks,
+                     out);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+static WallBC make_wallbc(c3d_ctx* c) {
+  WallBC b;
+  b.bcym = c->p.bc[2];
+  b.bcyp = c->p.bc[3];
+  b.uxm = c->p.uxbc[2]; b.uym_unused = c->p.uybc[2]; b.uzm = c->p.uzbc[2];
+  b.uxp = c->p.uxbc[3]; b.uyp = c->p.uybc[3]; b.uzp = c->p.uzbc[3];
+  b.dyp1 = c->met[C3D_M_DYP][1];
+  b.dypn = c->met[C3D_M_DYP][c->g.ny + 1];
+  return b;
+}
+
+int launch_bc_vel(c3d_ctx* c, void* ux, void* uy, void* uz, const void* uy_ym, int full) {
+  const Geom& g = c->g;
+  hipEvent_t ev;
+  prof_begin(c, ST_BC, &ev);
+  dim3 grid((g.nx + 255) / 256, g.nz), blk(256);
+  WallBC b = make_wallbc(c);
+  if (c->prec == C3D_FP64)
+    hipLaunchKernelGGL(k_bc_vel<double>, grid, blk, 0, c->stream, g, b, (double*)ux, (double*)uy,
+                       (double*)uz, (const double*)uy_ym, full);
+  else
+    hipLaunchKernelGGL(k_bc_vel<float>, grid, blk, 0, c->stream, g, b, (float*)ux, (float*)uy,
+                       (float*)uz, (const float*)uy_ym, full);
+  C3D_HIP(hipGetLastError());
+  prof_end(c, ST_BC, ev, 0.0);
+  return C3D_OK;
+}
+
+int launch_bc_pr(c3d_ctx* c, void* pr) {
+  const Geom& g = c->g;
+  hipEvent_t ev;
+  prof_begin(c, ST_BC, &ev);
+  dim3 grid((g.nx + 255) / 256, g.nz), blk(256);
+  if (c->prec == C3D_FP64)
+    hipLaunchKernelGGL(k_bc_pr<double>, grid, blk, 0, c->stream, g, (double*)pr);
+  else
+    hipLaunchKernelGGL(k_bc_pr<float>, grid, blk, 0, c->stream, g, (float*)pr);
+  C3D_HIP(hipGetLastError());
+  prof_end(c, ST_BC, ev, 0.0);
+  return C3D_OK;
+}
+
+int launch_update_uy_ym(c3d_ctx* c, void* uy_ym, void* duy_ym) {
+  const int n = c->g.nx * c->g.nz;
+  if (c->prec == C3D_FP64)
+    hipLaunchKernelGGL(k_update_uy_ym<double>, dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
+                       (double)c->p.uybc[2], (double*)uy_ym, (double*)duy_ym);
+  else
+    hipLaunchKernelGGL(k_update_uy_ym<float>, dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
+                       (float)c->p.uybc[2], (float*)uy_ym, (float*)duy_ym);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <typename T>
+static int proj_t(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, void* pr, int vel,
+                  int prs, int ghost) {
+  const Geom& g = c->g;
+  const T a = (T)c->pmAlpha;
+  const T rdxE = (T)(c->rdx * c->pmAlpha), rdzE = (T)(c->rdz * c->pmAlpha);
+#define PROJ_LAUNCH(V, P, G)                                                                     \
+  hipLaunchKernelGGL((k_proj<T, V, P, G>), cell_grid(g), dim3(BX, BY), 0, c->stream, g,         \
+                     mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz, (T*)pr)
+  if (vel && prs && ghost) PROJ_LAUNCH(true, true, true);
+  else if (vel && prs) PROJ_LAUNCH(true, true, false);
+  else if (vel && ghost) PROJ_LAUNCH(true, false, true);
+  else if (vel) PROJ_LAUNCH(true, false, false);
+  else PROJ_LAUNCH(false, true, false);
+#undef PROJ_LAUNCH
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_proj(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, void* pr, int do_vel,
+                int do_pr, int ghost_from_interior) {
+  hipEvent_t ev;
+  const int st = (do_vel && do_pr) ? ST_PROJ : do_vel ? ST_VELUPD : ST_PRUPD;
+  prof_begin(c, st, &ev);
+  int rc = c->prec == C3D_FP64
+               ? proj_t<double>(c, phi, ux, uy, uz, pr, do_vel, do_pr, ghost_from_interior)
+               : proj_t<float>(c, phi, ux, uy, uz, pr, do_vel, do_pr, ghost_from_interior);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double words = (do_vel && do_pr) ? 9.0 : do_vel ? 7.0 : 3.0;
+  prof_end(c, st, ev, words * N * c->esize);
+  return C3D_OK;
+}
+
+template <typename T>
+static int diag_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int what,
+                  int cfl_slot) {
+  const Geom& g = c->g;
+  const int nb = ncell_blocks(g);
+  const double* dyp = c->d_dyp;
+#define DIAG_LAUNCH(W)                                                                           \
+  hipLaunchKernelGGL((k_diag<T, W>), cell_grid(g), dim3(BX, BY), 0, c->stream, g, mets<T>(c),    \
+                     (T)c->rdx, (T)c->rdz, (const T*)ux, (const T*)uy, (const T*)uz, dyp,        \
+                     c->d_partials, nb)
+  switch (what) {
+    case 1: DIAG_LAUNCH(1); break;
+    case 2: DIAG_LAUNCH(2); break;
+    case 4: DIAG_LAUNCH(4); break;
+    case 8: DIAG_LAUNCH(8); break;
+    case 7: DIAG_LAUNCH(7); break;
+    case 6: DIAG_LAUNCH(6); break;
+    default: set_error("diag: unsupported selection"); return C3D_ERR_ARG;
+  }
+#undef DIAG_LAUNCH
+  C3D_HIP(hipGetLastError());
+  return launch_diag_finalize(c, nb, what, cfl_slot);
+}
+
+int launch_diag(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int what,
+                int cfl_slot) {
+  hipEvent_t ev;
+  prof_begin(c, ST_DIAG, &ev);
+  int rc = c->prec == C3D_FP64 ? diag_t<double>(c, ux, uy, uz, what, cfl_slot)
+                               : diag_t<float>(c, ux, uy, uz, what, cfl_slot);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_DIAG, ev, (what == 8 ? 1.0 : 3.0) * N * c->esize);
+  return C3D_OK;
+}
+
+int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot) {
+  hipLaunchKernelGGL(k_diag_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblocks,
+                     what, c->d_sc, (double)c->g.nx * (double)c->g.nz * c->p.yly, cfl_slot);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+}  // namespace c3d

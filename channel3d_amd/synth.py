@@ -1,0 +1,85 @@
+"""Seeded synthetic inputs for the channel workloads.
+
+Every random value is a pure function of (seed, field id, linear cell index)
+through the SplitMix64 finaliser, so any process (oracle, GPU run, bench)
+regenerates bit-identical fields without sharing state.  Arrays come out in
+the reference's Fortran layouts (see channel.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_K1 = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + _K1
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, field: int, n: int, start: int = 0) -> np.ndarray:
+    """n values in [0, 1) for linear indices start .. start+n-1"""
+    key = _mix(np.array([np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(field)],
+                        dtype=np.uint64))[0]
+    idx = np.arange(start, start + n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = _mix(idx * np.uint64(0xD1B54A32D192ED03) ^ key)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def uniform_pm1(seed, field, shape):
+    n = int(np.prod(shape))
+    return (2.0 * uniform01(seed, field, n) - 1.0).reshape(shape, order="F")
+
+
+def normal(seed, field, shape):
+    n = int(np.prod(shape))
+    u1 = uniform01(seed, field, n)
+    u2 = uniform01(seed, field + 1000003, n)
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    return (r * np.cos(2.0 * np.pi * u2)).reshape(shape, order="F")
+
+
+def yc_from_yp(yp: np.ndarray) -> np.ndarray:
+    """cell-centre y, yc(j) for j = 1..nyc (m_MeshAndMetries.f90:172-174)"""
+    nyc = yp.size - 2
+    return 0.5 * (yp[1:nyc + 1] + yp[2:nyc + 2])
+
+
+def laminar_noise(nxc, nyc, nzc, yp, yly=2.0, ubulk=1.0, amp=0.1, seed=0):
+    """u = 1.5 ub (1 - (y/h-1)^2) + amp ub U(-1,1), v = amp ub U(-1,1) off the
+    wall node, w = amp ub U(-1,1); returns halo arrays (halo layers zero)."""
+    shp = (nxc + 3, nyc + 3, nzc + 3)
+    ux = np.zeros(shp, order="F")
+    uy = np.zeros(shp, order="F")
+    uz = np.zeros(shp, order="F")
+    h = 0.5 * yly
+    yc = yc_from_yp(yp)
+    prof = 1.5 * ubulk * (1.0 - (yc / h - 1.0) ** 2)
+    ish = (nxc, nyc, nzc)
+    ux[1:nxc + 1, 1:nyc + 1, 1:nzc + 1] = prof[None, :, None] + amp * ubulk * uniform_pm1(seed, 1, ish)
+    noise_v = amp * ubulk * uniform_pm1(seed, 2, ish)
+    noise_v[:, 0, :] = 0.0
+    uy[1:nxc + 1, 1:nyc + 1, 1:nzc + 1] = noise_v
+    uz[1:nxc + 1, 1:nyc + 1, 1:nzc + 1] = amp * ubulk * uniform_pm1(seed, 3, ish)
+    return ux, uy, uz
+
+
+def ppe_rhs(nxc, nyc, nzc, seed=0):
+    """N(0,1) interior field with its mean removed (a compatible PPE source)"""
+    f = normal(seed, 7, (nxc, nyc, nzc))
+    return np.asfortranarray(f - f.mean())
+
+
+def random_halo(nxc, nyc, nzc, seed, field, amp=1.0):
+    """uniform(-amp, amp) everywhere in the halo array, halo layers included"""
+    return amp * uniform_pm1(seed, field, (nxc + 3, nyc + 3, nzc + 3))
+
+
+def random_interior(nxc, nyc, nzc, seed, field, amp=1.0):
+    return amp * uniform_pm1(seed, field, (nxc, nyc, nzc))

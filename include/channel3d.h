@@ -1,0 +1,190 @@
+/*
+ * channel3d.h -- C ABI of the MI355X-native Channel3d time-step core.
+ *
+ * Drop-in replacement for the per-time-step core of Channel3d's 2nd-order
+ * full-implicit channel solver: the routines that the reference binds through
+ * Fortran procedure pointers in InitTimeScheme (src/CFD_2nd/m_TScheme.f90:28-30,
+ * 147-173) and through clcPPE (src/CFD_2nd/m_Poisson.f90:16), plus the plain
+ * routines called from ChannelIterate (src/CFD_2nd/m_ChannelSystem.f90:91-130).
+ *
+ * Conventions
+ *  - Every entry returns int: C3D_OK (0) on success, a negative C3D_ERR_*
+ *    otherwise; c3d_last_error() gives the message.  The reference has no
+ *    status returns (it aborts); the Fortran binding turns a non-zero status
+ *    into ErrT_Abort (see the Fortran module shipped with the library).
+ *  - Field arguments are DEVICE pointers obtained from c3d_field_alloc() or
+ *    c3d_state_field().  Their in-memory layout is private to the library
+ *    (x fastest, y wall-ghost planes stored, x/z periodicity by index wrap);
+ *    move data in and out with c3d_upload()/c3d_download(), which take host
+ *    arrays in the reference's own Fortran layouts:
+ *      C3D_HALO      real(RK) (0:nxc+2, 0:nyc+2, 0:nzc+2)  m_Variables.f90:58-61
+ *      C3D_INTERIOR  real(RK) (1:nxc, 1:nyc, 1:nzc)         m_Variables.f90:91-95
+ *      C3D_WALL      real(RK) (1:nxc, 1:nzc)                m_Variables.f90:96-97
+ *  - One HIP stream per context; calls are asynchronous on it unless stated.
+ *    Contexts are single-host-thread and not reentrant (as the reference).
+ */
+#ifndef CHANNEL3D_H
+#define CHANNEL3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define C3D_OK 0
+#define C3D_ERR_ARG (-1)         /* invalid argument / unsupported configuration */
+#define C3D_ERR_HIP (-2)         /* HIP runtime error                            */
+#define C3D_ERR_NOMEM (-3)       /* device or host allocation failed             */
+#define C3D_ERR_UNSUPPORTED (-4) /* size not supported by the FFT plans          */
+
+/* boundary codes, m_Parameters.f90:69-76 */
+#define C3D_BC_PERIOD 0
+#define C3D_BC_NOSLIP (-1)
+#define C3D_BC_FREESLIP (-2)
+
+/* precision of a context's fields and kernels */
+#define C3D_FP64 0
+#define C3D_FP32 1
+
+/* layouts for upload/download */
+#define C3D_HALO 0
+#define C3D_INTERIOR 1
+#define C3D_WALL 2
+
+typedef struct c3d_ctx c3d_ctx;
+
+/* Inputs the reference keeps as module globals (SURVEY of m_Parameters.f90:
+ * 21-63, m_TScheme.f90:16-17, m_MeshAndMetries.f90:10-56). */
+typedef struct c3d_params {
+  int nxc, nyc, nzc;           /* cells (nxc/2 and nzc must have FFT plans)   */
+  double xlx, yly, zlz;        /* domain lengths                              */
+  double xnu;                  /* kinematic viscosity                         */
+  double dt;                   /* time step (iCFL=2: fixed)                   */
+  int bc[6];                   /* BcOption xm,xp,ym,yp,zm,zp: x,z periodic    */
+  double uxbc[6], uybc[6], uzbc[6];
+  double gravity[3];
+  int is_ux_const;             /* IsUxConst: constant flow rate               */
+  double ubulk, ybulk1, ybulk2;
+  int ischeme;                 /* 1 AB2, 2 RK2, 3 RK3                         */
+  int precision;               /* C3D_FP64 or C3D_FP32                        */
+} c3d_params;
+
+/* ---------------------------------------------------------------- setup -- */
+/* yp: node coordinates yp(1..nyc+1) of the wall-normal grid, passed as an
+ * array of nyc+2 doubles indexed like the reference (yp[0] unused).
+ * Replaces InitMeshAndMetries + InitPoissonSolver + InitTimeScheme
+ * (m_MeshAndMetries.f90:64-287, m_Poisson.f90:28-66, m_TScheme.f90:40-99). */
+int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out);
+int c3d_destroy(c3d_ctx* ctx);
+const char* c3d_last_error(void);
+int c3d_sync(c3d_ctx* ctx);
+
+/* y node coordinates for nsection sections (m_MeshAndMetries.f90:122-160,
+ * clcCoord :292-360); yp_out has nyc+2 entries, yp_out[1..nyc+1] written. */
+int c3d_mesh_coords(int nyc, double yly, int nsection, const double* section_length,
+                    const int* nyc_section, const int* stret_type, const int* stret_option,
+                    const double* cstret, double* yp_out);
+
+/* a host copy of a 1-D metric vector, indexed like the reference, written
+ * for j in [lo, hi] into out[0 .. hi-lo]; ids below. */
+int c3d_get_metric(c3d_ctx* ctx, int id, int lo, int hi, double* out);
+enum {
+  C3D_M_YP = 0, C3D_M_YC, C3D_M_DYP, C3D_M_DYC, C3D_M_RDYP, C3D_M_RDYC, C3D_M_YINTERP,
+  C3D_M_AM2PR, C3D_M_AC2PR, C3D_M_AP2PR, C3D_M_AM2C, C3D_M_AC2C, C3D_M_AP2C,
+  C3D_M_AM2P, C3D_M_AC2P, C3D_M_AP2P, C3D_M_AM2CCN, C3D_M_AP2CCN,
+  C3D_M_WAVEX, C3D_M_WAVEZ, C3D_NMETRICS
+};
+
+/* change dt (recomputes the implicit-operator tables), m_ChannelSystem.f90:81-88 */
+int c3d_set_dt(c3d_ctx* ctx, double dt);
+
+/* --------------------------------------------------------------- fields -- */
+/* a device field of the given layout in the context's precision */
+int c3d_field_alloc(c3d_ctx* ctx, int layout, void** dptr);
+int c3d_field_free(c3d_ctx* ctx, void* dptr);
+int c3d_upload(c3d_ctx* ctx, int layout, void* dptr, const double* host);
+int c3d_download(c3d_ctx* ctx, int layout, const void* dptr, double* host);
+
+/* the context's own channel state (m_Variables.f90:10-24) */
+enum {
+  C3D_UX = 0, C3D_UY, C3D_UZ, C3D_PR,       /* halo      */
+  C3D_HISTX, C3D_HISTY, C3D_HISTZ,          /* interior  */
+  C3D_ARR1, C3D_ARR2,                       /* interior scratch (RealArr1/2) */
+  C3D_HALOARR,                              /* halo scratch (RealHalo) */
+  C3D_UY_YM, C3D_DUY_YM,                    /* wall planes */
+  C3D_NSTATE
+};
+int c3d_state_field(c3d_ctx* ctx, int id, void** dptr);
+
+/* ---------------------------------------- reference routines (per call) -- */
+/* m_Parameters.f90:173-186 */
+int c3d_PMcoeUpdate(c3d_ctx* ctx, int ns);
+/* m_FlowType_Channel.f90:120-130 */
+int c3d_Update_uy_ym(c3d_ctx* ctx, void* uy_ym, void* duy_ym);
+/* m_TSchemeFIMP_inc.f90:7-75 / 80-129 / 134-182 */
+int c3d_clcRhsX(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* RhsX,
+                void* HistXOld, const void* pressure);
+int c3d_clcRhsY(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* RhsY,
+                void* HistYOld, const void* pressure);
+int c3d_clcRhsZ(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* RhsZ,
+                void* HistZOld, const void* pressure);
+/* m_TSchemeFIMP_inc.f90:263-347 / 636-729 / 1035-1128 (the _010 variants) */
+int c3d_clcU1Hat(c3d_ctx* ctx, void* ux, void* RhsX);
+int c3d_clcU2Hat(c3d_ctx* ctx, void* uy, void* RhsY, const void* duy_ym);
+int c3d_clcU3Hat(c3d_ctx* ctx, void* uz, void* RhsZ);
+/* m_BC_and_Halo.f90:349-405 / 410-456 / 232-344 */
+int c3d_SetBC_and_UpdateHaloForPrSrc(c3d_ctx* ctx, void* ux, void* uy, void* uz,
+                                     const void* uy_ym);
+int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* ctx, void* pressure);
+int c3d_SetBC_and_UpdateHalo(c3d_ctx* ctx, void* ux, void* uy, void* uz, const void* uy_ym);
+/* m_TScheme.f90:246-276; divmax is a HOST pointer, written after a sync */
+int c3d_clcPrSrc(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* prsrc,
+                 void* pressure, double* divmax);
+/* m_Poisson.f90:567-688 (clcPPE_x1 semantics: gauge pin, normfft) */
+int c3d_clcPPE(c3d_ctx* ctx, void* prsrc, void* prphiHalo);
+/* m_TScheme.f90:281-306 */
+int c3d_FluidVelUpdate(c3d_ctx* ctx, const void* prphiHalo, void* ux, void* uy, void* uz);
+/* m_TSchemeFIMP_inc.f90:1371-1386 */
+int c3d_PressureUpdate(c3d_ctx* ctx, void* pressure, const void* prphiHalo);
+/* m_Tools.f90:21-46, 51-75, 80-104, 250-270 (results on the host, synchronous) */
+int c3d_CalcMaxCFL(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
+                   double* uddxmax);
+int c3d_CheckDivergence(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
+                        double* divmax);
+int c3d_CalcVmax(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, double* vmax3);
+int c3d_CalcUxAver(c3d_ctx* ctx, const void* ux, double* uxaver);
+
+/* ------------------------------------------------ fused, device-resident -- */
+/* one RK substep ns (1..iadvance) on the context state, in ChannelIterate's
+ * order (m_ChannelSystem.f90:91-125), fused kernels, no host sync */
+int c3d_substep(c3d_ctx* ctx, int ns);
+/* one time step: CFL, all substeps, divergence / vmax checks
+ * (m_ChannelSystem.f90:73-130); no host sync */
+int c3d_step(c3d_ctx* ctx);
+/* diagnostics of the last step, synchronous:
+ * out[0]=cfl*dt, [1]=divmax before projection (last substep), [2]=divmax
+ * after, [3..5]=vmax, [6]=PrGradNow, [7]=PrGradAver, [8]=SimTime */
+int c3d_step_diagnostics(c3d_ctx* ctx, double* out9);
+/* project the state's velocity onto the divergence-free space with a unit
+ * time factor (builds projected initial fields) */
+int c3d_project(c3d_ctx* ctx);
+
+/* one PPE solve (the fused path's clcPPE) from prsrc to prphiHalo, both
+ * device fields of the context; exposed for the solves/s benchmark */
+int c3d_ppe(c3d_ctx* ctx, void* prsrc, void* prphiHalo);
+
+/* ---------------------------------------------------------- profiling -- */
+/* per-stage HIP-event timing on the context stream */
+int c3d_profile_enable(c3d_ctx* ctx, int on);
+int c3d_profile_reset(c3d_ctx* ctx);
+/* stage names and accumulated (count, total ms, algorithmic bytes per call) */
+int c3d_profile_count(c3d_ctx* ctx);
+int c3d_profile_get(c3d_ctx* ctx, int idx, const char** name, long* calls, double* total_ms,
+                    double* bytes_per_call);
+size_t c3d_device_bytes(c3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,23 @@
+"""Workload definitions shared by tests and bench (reference coordinates:
+y in [0, yly], walls at y=0 and y=yly)."""
+import math
+
+# config 1: 2nd-order channel, Re_bulk=5600 (Re_tau~180), FIMP, RK3, dt=0.005
+C1 = dict(nxc=64, nyc=64, nzc=64, xlx=4 * math.pi, yly=2.0, zlz=2 * math.pi,
+          xnu=2.0 / 5600.0, dt=0.005, gamma=1.5, ubulk=1.0)
+
+# production step: Re_tau=550, 768x384x576, Lx=2pi, Lz=pi, dt=0.002
+C5 = dict(nxc=768, nyc=384, nzc=576, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
+          xnu=9.9369e-5, dt=0.002, gamma=1.5, ubulk=1.0)
+
+# small shapes exercising other FFT radices and non-square planes
+SMALL = dict(nxc=32, nyc=24, nzc=40, xlx=4 * math.pi, yly=2.0, zlz=2 * math.pi,
+             xnu=2.0 / 5600.0, dt=0.005, gamma=1.5, ubulk=1.0)
+ODD = dict(nxc=48, nyc=37, nzc=36, xlx=3.0, yly=2.0, zlz=1.7, xnu=1.0 / 1000.0,
+           dt=0.004, gamma=1.2, ubulk=1.0)
+# large implicit factor: beta*rdx^2 of order one, exercises the full-circulant path
+STIFF = dict(nxc=16, nyc=20, nzc=20, xlx=1.0, yly=2.0, zlz=1.0, xnu=0.05, dt=0.05,
+             gamma=1.5, ubulk=1.0)
+# free-slip top wall (half-channel style boundary)
+FREESLIP = dict(nxc=32, nyc=24, nzc=24, xlx=4.0, yly=1.0, zlz=2.0, xnu=1.0 / 2000.0,
+                dt=0.005, gamma=1.5, ubulk=1.0, bc=(0, 0, -1, -2, 0, 0))
