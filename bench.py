@@ -75,6 +75,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="64^3 workload (quick check)")
+    ap.add_argument("--probe", choices=["copy"], default=None,
+                    help="only run plain field copies (HBM reference for traffic counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs "
@@ -94,6 +96,28 @@ def main(argv=None):
     g = Channel(case)
     log(f"context created, {g.device_bytes() / 2**30:.1f} GiB device memory "
         f"({time.perf_counter() - t0:.1f}s)")
+
+    def copy_probe(reps):
+        """plain device copy of one halo field: measured HBM copy rate"""
+        dst = g.alloc(L.HALO)
+        fbytes = (nx * (ny + 3) * nz) * 8
+        g.copy(dst, L.UX)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.copy(dst, L.UX)
+        g.sync()
+        dt = time.perf_counter() - t0
+        g.free(dst)
+        return 2.0 * fbytes * reps / dt / 1e9, fbytes
+
+    if a.probe == "copy":
+        gbs, fbytes = copy_probe(max(a.steps, 1))
+        log(f"copy probe: {a.steps} copies of {fbytes / 2**20:.1f} MiB, {gbs:.1f} GB/s")
+        print(json.dumps({"probe": "copy", "bytes_per_copy": fbytes, "copies": a.steps,
+                          "gbs": round(gbs, 1)}), flush=True)
+        g.close()
+        return 0
     t0 = time.perf_counter()
     ux, uy, uz = synth.laminar_noise(nx, ny, nz, g.yp, yly=case["yly"], ubulk=1.0, seed=0)
     g.upload(L.UX, ux)
@@ -137,6 +161,8 @@ def main(argv=None):
     t_ppe = time.perf_counter() - t0
     ppe_per_s = a.ppe_solves / t_ppe
     log(f"{a.ppe_solves} PPE solves: {1e3 * t_ppe / a.ppe_solves:.2f} ms each")
+    copy_gbs, _ = copy_probe(10)
+    log(f"copy probe: {copy_gbs:.1f} GB/s")
 
     # dominant kernel roofline (algorithmic bytes / average launch time)
     stage_rows = {}
@@ -189,6 +215,7 @@ def main(argv=None):
         "poisson_solves_per_s": round(ppe_per_s, 3),
         "grid_points_steps_per_s": round(ncell * steps_per_s, 1),
         "roofline": roof,
+        "hbm_copy_gbs_measured": round(copy_gbs, 1),
         "cpu_baseline": cpu,
         "stages": stage_rows,
         "divmax_after_step": d1["divmax2"],

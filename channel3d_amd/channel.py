@@ -141,6 +141,11 @@ class Channel:
         L.check(self.lib.c3d_download(self._h, layout, self.ptr(f), L.dptr(a)), "c3d_download")
         return a
 
+    def copy(self, dst, src, layout=L.HALO):
+        """device-to-device copy of a whole field"""
+        L.check(self.lib.c3d_field_copy(self._h, layout, self.ptr(dst), self.ptr(src)),
+                "c3d_field_copy")
+
     def metric(self, mid, lo, hi):
         out = np.zeros(hi - lo + 1)
         L.check(self.lib.c3d_get_metric(self._h, mid, lo, hi, out.ctypes.data_as(L._DP)),

@@ -504,7 +504,9 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   g.plane = (long long)g.nx * g.nyh;
   g.nfield = g.plane * g.nz;
   g.nk = g.nx / 2;
-  g.P = ((g.nk + 1) + 3) / 4 * 4;
+  // spectral rows padded to a multiple of 128 bytes (8 complex FP64), so the
+  // 8-wide kx tiles of the z transforms are whole cache lines
+  g.P = ((g.nk + 1) + 7) / 8 * 8;
   g.nspec = (long long)g.P * g.ny * g.nz;
   compute_metrics(c, yp);
   // m_Parameters.f90:135-150
@@ -644,6 +646,41 @@ extern "C" int c3d_field_free(c3d_ctx* c, void* dptr) {
     }
   set_error("c3d_field_free: not a field of this context");
   return C3D_ERR_ARG;
+}
+
+// streaming copy, four 16-byte loads in flight per thread before the stores
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src,
+                                                u32x4* __restrict__ dst, long long n16) {
+  const long long base = ((long long)blockIdx.x * blockDim.x) * 4 + threadIdx.x;
+  u32x4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long long q = base + (long long)u * blockDim.x;
+    if (q < n16) v[u] = __builtin_nontemporal_load(src + q);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long long q = base + (long long)u * blockDim.x;
+    if (q < n16) __builtin_nontemporal_store(v[u], dst + q);
+  }
+}
+
+extern "C" int c3d_field_copy(c3d_ctx* c, int layout, void* dst, const void* src) {
+  if (!c || !dst || !src || layout < C3D_HALO || layout > C3D_WALL) {
+    set_error("c3d_field_copy: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  const size_t bytes = layout_elems(c, layout) * c->esize;
+  const long long n16 = (long long)(bytes / 16);
+  if (n16 > 0)
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, c->stream,
+                       (const u32x4*)src, (u32x4*)dst, n16);
+  C3D_HIP(hipGetLastError());
+  if (bytes % 16)
+    C3D_HIP(hipMemcpyAsync((char*)dst + n16 * 16, (const char*)src + n16 * 16, bytes % 16,
+                           hipMemcpyDeviceToDevice, c->stream));
+  return C3D_OK;
 }
 
 extern "C" int c3d_state_field(c3d_ctx* c, int id, void** dptr) {
@@ -1009,7 +1046,8 @@ extern "C" int c3d_project(c3d_ctx* c) {
 // profiling
 // ===========================================================================
 static const char* kStageNames[ST_COUNT] = {
-    "rhs_fused", "rhs_single", "bulk_finalize", "adi_x", "adi_z", "adi_y_update", "bc_ghosts",
+    "rhs_fused", "rhs_single", "bulk_finalize", "adi_x", "adi_z", "adi_xz", "adi_y_update",
+    "bc_ghosts",
     "prsrc", "fft_x_fwd", "fft_z_fwd", "tridiag_y", "fft_z_inv", "fft_x_inv", "projection",
     "vel_update", "pr_update", "diagnostics", "misc"};
 

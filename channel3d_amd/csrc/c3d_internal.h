@@ -37,6 +37,16 @@ __host__ __device__ inline long long fidx(const Geom& g, int i, int jj, int k) {
   return (long long)i + (long long)g.nx * ((long long)jj + (long long)g.nyh * (long long)k);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_remap turns the physical block id into a logical one such that
+// every XCD receives one contiguous range of logical blocks, so tiles that
+// share halo rows run on the same XCD and hit the same L2.
+__device__ inline int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
 // Reproducible sums.  Each double term is truncated to a multiple of 2^-80
 // and split into four signed 32-bit limbs accumulated in int64: integer
 // addition is associative, so the total is independent of the order and of
@@ -113,7 +123,7 @@ struct DevScalars {
 };
 
 enum Stage {
-  ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_Y, ST_BC, ST_PRSRC,
+  ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_XZ, ST_ADI_Y, ST_BC, ST_PRSRC,
   ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
   ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_COUNT
 };

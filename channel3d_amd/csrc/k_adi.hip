@@ -71,6 +71,73 @@ __global__ __launch_bounds__(ABX* ABY) void k_conv_z(Geom g, const double* __res
   }
 }
 
+// Fused x and z convolutions.  A workgroup owns a 64(i) x 4(j) tile and
+// marches z through a chunk of planes: each new plane's rows are staged in
+// LDS (with the x halo) and convolved along x, the results enter a register
+// window of 2*MZ+1 planes, and one z-convolved plane is written per step.
+// Every input element is read from HBM once per chunk (plus the 2*MZ-plane
+// lead-in), every output element written once.
+constexpr int MAXMX = 32;
+
+template <typename T, int MZ, bool ADD>
+__global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restrict__ xt, int mx,
+                                                 const double* __restrict__ zt,
+                                                 const T* __restrict__ f, T* __restrict__ out,
+                                                 const DevScalars* __restrict__ sc, int kchunk,
+                                                 int gxb, int gyb) {
+  // the convolutions replace the reference's periodic Thomas solves, so their
+  // rounding differs from it regardless: fused multiply-adds are allowed here
+#pragma clang fp contract(fast)
+  constexpr int NZT = 2 * MZ + 1;
+  __shared__ T sm[4][64 + 2 * MAXMX];
+  const int nbl = gridDim.x;
+  const int lb = xcd_remap(blockIdx.x, nbl);
+  const int bx = lb % gxb, by = (lb / gxb) % gyb, bz = lb / (gxb * gyb);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int nx = g.nx, nz = g.nz;
+  const int i0 = bx * 64, i = i0 + tx;
+  const int jc = by * 4 + ty + 1;
+  const bool rowok = jc <= g.ny;
+  const bool active = rowok && i < nx;
+  const int k0 = bz * kchunk, k1 = min(nz, k0 + kchunk);
+  const T add = ADD ? (T)sc->dpmdxns : T(0);
+  const int w = 64 + 2 * mx;
+  auto xconv = [&](int kk) -> T {
+    const long long row = (long long)nx * jc + g.plane * kk;
+    for (int t = tx; t < w; t += 64) {
+      int ii = (i0 - mx + t) % nx;
+      if (ii < 0) ii += nx;
+      T v = T(0);
+      if (rowok) v = ADD ? f[row + ii] + add : f[row + ii];
+      sm[ty][t] = v;
+    }
+    __syncthreads();
+    T acc = 0;
+    for (int t = 0; t <= 2 * mx; ++t) acc += (T)xt[t] * sm[ty][tx + t];
+    __syncthreads();
+    return acc;
+  };
+  auto wrapk = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
+  // ring of x-convolved planes: plane p sits in slot (p - k0 + MZ) mod NZT;
+  // unrolling by NZT makes every slot index a compile-time constant
+  T win[NZT];
+#pragma unroll
+  for (int q = 0; q < NZT - 1; ++q) win[q] = xconv(wrapk(k0 - MZ + q));
+  for (int kb = k0; kb < k1; kb += NZT) {
+#pragma unroll
+    for (int s = 0; s < NZT; ++s) {
+      const int k = kb + s;
+      if (k < k1) {
+        win[(s + NZT - 1) % NZT] = xconv(wrapk(k + MZ));
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < NZT; ++q) acc += (T)zt[q] * win[(s + q) % NZT];
+        if (active) out[(long long)nx * jc + g.plane * k + i] = acc;
+      }
+    }
+  }
+}
+
 // Thomas sweeps along y with shared factors: a(j), vecm(j), arrmn(j) for
 // j = 1..ny (reference indexing).  WALLROW: row 1 is the wall condition
 // c=1, f=duy_ym (m_TSchemeFIMP_inc.f90:701-716).
@@ -86,24 +153,113 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
   const int ny = g.ny;
   const long long base = (long long)i + g.plane * k;
   const long long sj = g.nx;
+  // The recurrences are sequential in j; the loads are not.  Each sweep
+  // keeps the next PD rows in flight in a register ring.  The main loops
+  // carry no conditionals (every prefetch address is in range), so loads
+  // stay outstanding across iterations; short guarded tails finish the line.
+  constexpr int PD = 8;
+  T buf[PD];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) buf[q] = r[base + sj * min(2 + q, ny)];
   T f = WALLROW ? duy[i + (long long)g.nx * k] : r[base + sj];
   f = f / (T)tvecm[1];
   r[base + sj] = f;
-  for (int jc = 2; jc <= ny; ++jc) {
-    const long long q = base + sj * jc;
-    f = (r[q] - (T)ta[jc] * f) / (T)tvecm[jc];
-    if (jc < ny) r[q] = f;
+  int jb = 2;
+  for (; jb + 2 * PD <= ny; jb += PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      const int jc = jb + q;
+      const T rv = buf[q];
+      buf[q] = r[base + sj * (jc + PD)];
+      f = (rv - (T)ta[jc] * f) / (T)tvecm[jc];
+      r[base + sj * jc] = f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int jc = jb + q;
+    if (jc <= ny) {
+      f = (buf[q] - (T)ta[jc] * f) / (T)tvecm[jc];
+      if (jc < ny) r[base + sj * jc] = f;
+    }
+  }
+  for (int jc = jb + PD; jc <= ny; ++jc) {
+    f = (r[base + sj * jc] - (T)ta[jc] * f) / (T)tvecm[jc];
+    if (jc < ny) r[base + sj * jc] = f;
   }
   u[base + sj * ny] += f;
-  for (int jc = ny - 1; jc >= 1; --jc) {
-    const long long q = base + sj * jc;
-    f = r[q] - (T)tarr[jc + 1] * f;
-    u[q] += f;
+  T rb[PD], ub[PD];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int jn = max(ny - 1 - q, 1);
+    rb[q] = r[base + sj * jn];
+    ub[q] = u[base + sj * jn];
+  }
+  jb = ny - 1;
+  for (; jb - 2 * PD + 1 >= 1; jb -= PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      const int jc = jb - q;
+      const T rv = rb[q], uv = ub[q];
+      rb[q] = r[base + sj * (jc - PD)];
+      ub[q] = u[base + sj * (jc - PD)];
+      f = rv - (T)tarr[jc + 1] * f;
+      u[base + sj * jc] = uv + f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int jc = jb - q;
+    if (jc >= 1) {
+      f = rb[q] - (T)tarr[jc + 1] * f;
+      u[base + sj * jc] = ub[q] + f;
+    }
+  }
+  for (int jc = jb - PD; jc >= 1; --jc) {
+    f = r[base + sj * jc] - (T)tarr[jc + 1] * f;
+    u[base + sj * jc] = u[base + sj * jc] + f;
   }
 }
 
 static inline dim3 cgrid(const Geom& g) {
   return dim3((g.nx + ABX - 1) / ABX, (g.ny + ABY - 1) / ABY, (g.nz + AKCH - 1) / AKCH);
+}
+
+constexpr int MAXMZ = 12;
+
+// the fused kernel needs banded x and z kernels within its LDS halo and
+// register window
+static bool xz_fusable(const Geom& g, const c3d_ctx::AdiTab& tb) {
+  const int mx = -tb.xlo, mz = -tb.zlo;
+  return tb.xlo < 0 && tb.zlo < 0 && mx <= MAXMX && mz <= MAXMZ && 2 * mz + 1 < g.nz &&
+         2 * mx + 1 < g.nx;
+}
+
+template <typename T, bool ADD>
+static bool launch_xz_fused(c3d_ctx* c, const c3d_ctx::AdiTab& tb, const void* rhs) {
+  const Geom& g = c->g;
+  const int mx = -tb.xlo, mz = -tb.zlo;
+  if (!xz_fusable(g, tb)) return false;
+  const int gxb = (g.nx + 63) / 64, gyb = (g.ny + 3) / 4;
+  int nch = (4096 + gxb * gyb - 1) / (gxb * gyb);
+  nch = nch < 1 ? 1 : nch;
+  int kchunk = (g.nz + nch - 1) / nch;
+  if (kchunk < 4 * mz + 8) kchunk = 4 * mz + 8;
+  if (kchunk > g.nz) kchunk = g.nz;
+  const int gzb = (g.nz + kchunk - 1) / kchunk;
+  const dim3 grid(gxb * gyb * gzb), blk(64, 4);
+  switch (mz) {
+#define XZ(M)                                                                                  \
+  case M:                                                                                      \
+    hipLaunchKernelGGL((k_conv_xz<T, M, ADD>), grid, blk, 0, c->stream, g, tb.d_xtaps, mx,     \
+                       tb.d_ztaps, (const T*)rhs, (T*)c->tmpA, (const DevScalars*)c->d_sc,     \
+                       kchunk, gxb, gyb);                                                      \
+    break;
+    XZ(0) XZ(1) XZ(2) XZ(3) XZ(4) XZ(5) XZ(6) XZ(7) XZ(8) XZ(9) XZ(10) XZ(11) XZ(12)
+#undef XZ
+    default: return false;
+  }
+  return true;
 }
 
 template <typename T>
@@ -112,6 +268,27 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
   const auto& tb = c->adi[ns - 1];
   const double N = (double)g.nx * g.ny * g.nz;
   hipEvent_t ev;
+  // fused x and z sweeps: rhs -> tmpA, then y sweep + update from tmpA
+  if (xz_fusable(g, tb)) {
+    prof_begin(c, ST_ADI_XZ, &ev);
+    if (add) launch_xz_fused<T, true>(c, tb, rhs);
+    else launch_xz_fused<T, false>(c, tb, rhs);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_ADI_XZ, ev, 2.0 * N * c->esize);
+    prof_begin(c, ST_ADI_Y, &ev);
+    dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
+    double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
+    if (comp == 1)
+      hipLaunchKernelGGL((k_adi_y<T, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)c->tmpA, (T*)u, (const T*)duy_ym);
+    else
+      hipLaunchKernelGGL((k_adi_y<T, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)c->tmpA, (T*)u, (const T*)nullptr);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
+    return C3D_OK;
+  }
+  // general path (wide kernels, e.g. full circulant): separate x and z sweeps
   // x sweep: rhs -> tmpA
   prof_begin(c, ST_ADI_X, &ev);
   if (add)

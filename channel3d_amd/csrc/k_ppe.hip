@@ -36,6 +36,12 @@ __global__ __launch_bounds__(256) void k_tridiag(Geom g, Metrics<T> m,
   // row j = 1
   T vecm = pin ? T(1) : (m.ac2Pr[1] + lx) + lz;
   T cprev = pin ? T(0) : m.ap2Pr[1];
+  // register rings keep the next PD rows of each sweep in flight; the main
+  // loops carry no conditionals so the loads stay outstanding across steps
+  constexpr int PD = 8;
+  T2 buf[PD];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) buf[q] = spec[base + P * min(1 + q, ny - 1)];
   T2 f = spec[base];
   if (pin) {
     f.x = 0;
@@ -44,27 +50,76 @@ __global__ __launch_bounds__(256) void k_tridiag(Geom g, Metrics<T> m,
   f.x = f.x / vecm;
   f.y = f.y / vecm;
   spec[base] = f;
-  for (int j0 = 1; j0 < ny; ++j0) {
+  auto fwd = [&](int j0, T2 fj) {
     const int jr = j0 + 1;
     const T arr = cprev / vecm;
     const T a = m.am2Pr[jr];
     vecm = ((m.ac2Pr[jr] + lx) + lz) - a * arr;
-    const long long q = base + P * j0;
-    const T2 fj = spec[q];
     f.x = (fj.x - a * f.x) / vecm;
     f.y = (fj.y - a * f.y) / vecm;
-    if (j0 < ny - 1) spec[q] = f;
-    sarr[q] = arr;
+    sarr[base + P * j0] = arr;
     cprev = m.ap2Pr[jr];
+  };
+  int jb = 1;
+  for (; jb + 2 * PD <= ny - 1; jb += PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      const int j0 = jb + q;
+      const T2 fj = buf[q];
+      buf[q] = spec[base + P * (j0 + PD)];
+      fwd(j0, fj);
+      spec[base + P * j0] = f;
+    }
   }
-  spec[base + P * (ny - 1)] = f;
-  for (int j0 = ny - 2; j0 >= 0; --j0) {
-    const long long q = base + P * j0;
-    const T arr1 = sarr[q + P];
-    const T2 fj = spec[q];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int j0 = jb + q;
+    if (j0 < ny) {
+      fwd(j0, buf[q]);
+      spec[base + P * j0] = f;
+    }
+  }
+  for (int j0 = jb + PD; j0 < ny; ++j0) {
+    fwd(j0, spec[base + P * j0]);
+    spec[base + P * j0] = f;
+  }
+  T2 fb[PD];
+  T ab[PD];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int jn = max(ny - 2 - q, 0);
+    fb[q] = spec[base + P * jn];
+    ab[q] = sarr[base + P * (jn + 1)];
+  }
+  jb = ny - 2;
+  for (; jb - 2 * PD + 1 >= 0; jb -= PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      const int j0 = jb - q;
+      const T2 fj = fb[q];
+      const T arr1 = ab[q];
+      fb[q] = spec[base + P * (j0 - PD)];
+      ab[q] = sarr[base + P * (j0 - PD + 1)];
+      f.x = fj.x - arr1 * f.x;
+      f.y = fj.y - arr1 * f.y;
+      spec[base + P * j0] = f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    const int j0 = jb - q;
+    if (j0 >= 0) {
+      f.x = fb[q].x - ab[q] * f.x;
+      f.y = fb[q].y - ab[q] * f.y;
+      spec[base + P * j0] = f;
+    }
+  }
+  for (int j0 = jb - PD; j0 >= 0; --j0) {
+    const T arr1 = sarr[base + P * (j0 + 1)];
+    const T2 fj = spec[base + P * j0];
     f.x = fj.x - arr1 * f.x;
     f.y = fj.y - arr1 * f.y;
-    spec[q] = f;
+    spec[base + P * j0] = f;
   }
 }
 

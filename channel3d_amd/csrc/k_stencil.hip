@@ -10,11 +10,20 @@
 namespace c3d {
 
 constexpr int BX = 64, BY = 4, KCH = 8;
+constexpr int KCHD = 64;  // z run of the diagnostics (fewer partials to combine)
 
 template <typename T>
 struct Fld {
   T* p;
 };
+
+// Cell kernels run on a 1-D grid of 64 x 4 (i, j) tiles times z runs; the
+// tile of a workgroup comes from its XCD-remapped id, x fastest, so the
+// tiles sharing halo rows are neighbours on the same XCD.
+#define CELL_BLOCK()                                                         \
+  const int cgx_ = (g.nx + BX - 1) / BX, cgy_ = (g.ny + BY - 1) / BY;        \
+  const int clb_ = xcd_remap(blockIdx.x, gridDim.x);                         \
+  const int cbx = clb_ % cgx_, cby = (clb_ / cgx_) % cgy_, cbz = clb_ / (cgx_ * cgy_)
 
 __device__ __forceinline__ int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ __forceinline__ int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
@@ -75,6 +84,13 @@ __device__ __forceinline__ void block_sum_fix(long long* v, long long (*sh)[4]) 
 // clcRhsX_FIMP :7-75, clcRhsY_FIMP :80-129, clcRhsZ_FIMP :134-182.
 // MASK bit 0/1/2 selects the x/y/z component; the x component also writes
 // the per-block partial of the bulk-velocity integral s1tot (:46).
+//
+// A thread owns one (i, j) column and marches a run of z planes.  The
+// values its own column needs at k-1, k and k+1 (and the few off-column
+// ones the conservative fluxes need one plane up or down) roll through
+// registers, so every plane of every field is fetched once per run; x and
+// y neighbours of the current plane come from the tile's other threads
+// through the caches.  Expressions keep the reference's operation order.
 // ---------------------------------------------------------------------------
 template <typename T, int MASK>
 __global__ __launch_bounds__(BX* BY) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
@@ -83,11 +99,14 @@ __global__ __launch_bounds__(BX* BY) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf
                                                 T* __restrict__ rx, T* __restrict__ ry,
                                                 T* __restrict__ rz, T* __restrict__ hx,
                                                 T* __restrict__ hy, T* __restrict__ hz,
-                                                double* __restrict__ partials) {
+                                                double* __restrict__ partials, int kchunk) {
   __shared__ long long shf[8][4];
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int jc = blockIdx.y * BY + threadIdx.y + 1;  // reference j
-  const int k0 = blockIdx.z * KCH;
+  const int cgx = (g.nx + BX - 1) / BX, cgy = (g.ny + BY - 1) / BY;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int cbx = lb % cgx, cby = (lb / cgx) % cgy, cbz = lb / (cgx * cgy);
+  const int i = cbx * BX + threadIdx.x;
+  const int jc = cby * BY + threadIdx.y + 1;  // reference j
+  const int k0 = cbz * kchunk;
   long long s1[4] = {0, 0, 0, 0};
   if (i < g.nx && jc <= g.ny) {
     const int nx = g.nx, nz = g.nz;
@@ -101,82 +120,90 @@ __global__ __launch_bounds__(BX* BY) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf
     const T sucac = m.rdyc[jc], qsucac = T(0.25) * sucac;
     const T am1c = cf.rdx2, ap1c = cf.rdx2, ac1c = -(am1c + ap1c);
     const T am3c = cf.rdz2, ap3c = cf.rdz2, ac3c = -(am3c + ap3c);
+    const T ap2c = m.ap2c[jc], ac2c = m.ac2c[jc], am2c = m.am2c[jc];
+    const T ap2p = m.ap2p[jc], ac2p = m.ac2p[jc], am2p = m.am2p[jc];
     const T s1coe = m.s1coe[jc];
-    const int k1 = min(nz, k0 + KCH);
+    const int k1 = min(nz, k0 + kchunk);
+    // rolling registers: *_m at k-1, *_c at k, *_p at k+1 of this column
+    int kprev = wrapm(k0, nz);
+    long long rq = sj * jc + sk * kprev;
+    T u_m = ux[rq + i], v_m = uy[rq + i], w_m = uz[rq + i], p_m = pr[rq + i];
+    T uxp_m = ux[rq + ip];                 // ux(ip, k-1)
+    T vyp_m = uy[rq + sj + i];             // uy(jp, k-1)
+    rq = sj * jc + sk * k0;
+    T u_c = ux[rq + i], v_c = uy[rq + i], w_c = uz[rq + i], p_c = pr[rq + i];
+    T w_xm = uz[rq + im], w_ym = uz[rq - sj + i];   // uz(im, k), uz(jm, k)
     for (int k = k0; k < k1; ++k) {
-      const int km = wrapm(k, nz), kp = wrapp(k, nz);
-      const long long rowc = sj * jc + sk * k;  // (jc, k)
+      const int kp = wrapp(k, nz);
+      const long long rowc = sj * jc + sk * k;
       const long long rowm = rowc - sj, rowp = rowc + sj;
-      const long long rowkm = sj * jc + sk * km, rowkp = sj * jc + sk * kp;
+      const long long rowkp = sj * jc + sk * kp;
       const long long c = rowc + i;
+      // plane k+1 of this column (and uz one step left / down)
+      const T u_p = ux[rowkp + i], v_p = uy[rowkp + i], w_p = uz[rowkp + i];
+      const T w_xm_p = uz[rowkp + im], w_ym_p = uz[rowkp - sj + i];
+      const T p_p = pr[rowkp + i];
+      // plane k neighbours
+      const T u_xp = ux[rowc + ip], u_xm = ux[rowc + im], u_yp = ux[rowp + i], u_ym = ux[rowm + i];
+      const T v_xp = uy[rowc + ip], v_xm = uy[rowc + im], v_yp = uy[rowp + i], v_ym = uy[rowm + i];
+      const T w_xp = uz[rowc + ip], w_yp = uz[rowp + i];
       if (MASK & 1) {
-        const T u_c = ux[c], u_xp = ux[rowc + ip], u_xm = ux[rowc + im];
-        const T u_yp = ux[rowp + i], u_ym = ux[rowm + i];
-        const T u_zp = ux[rowkp + i], u_zm = ux[rowkm + i];
         const T h11 = ((u_xp + u_c) * (u_xp + u_c) - (u_c + u_xm) * (u_c + u_xm)) * qdx1;
-        const T h12 = ((uy[rowp + i] + uy[rowp + im]) * (I3 * u_c + I4 * u_yp) -
-                       (uy[c] + uy[rowc + im]) * (I1 * u_ym + I2 * u_c)) *
+        const T h12 = ((v_yp + uy[rowp + im]) * (I3 * u_c + I4 * u_yp) -
+                       (v_c + v_xm) * (I1 * u_ym + I2 * u_c)) *
                       sucaj;
-        const T h13 = ((uz[rowkp + i] + uz[rowkp + im]) * (u_zp + u_c) -
-                       (uz[c] + uz[rowc + im]) * (u_c + u_zm)) *
-                      qdx3;
+        const T h13 = ((w_p + w_xm_p) * (u_p + u_c) - (w_c + w_xm) * (u_c + u_m)) * qdx3;
         const T d11 = (u_xp - T(2) * u_c + u_xm) * cf.rdx2;
-        const T d22 = m.ap2c[jc] * u_yp + m.ac2c[jc] * u_c + m.am2c[jc] * u_ym;
-        const T d33 = ap3c * u_zp + ac3c * u_c + am3c * u_zm;
+        const T d22 = ap2c * u_yp + ac2c * u_c + am2c * u_ym;
+        const T d33 = ap3c * u_p + ac3c * u_c + am3c * u_m;
         const T dcq = d11 + d22 + d33;
-        const T gradp = (pr[c] - pr[rowc + im]) * cf.rdx;
+        const T gradp = (p_c - pr[rowc + im]) * cf.rdx;
         fix_add(s1, (double)(u_c * s1coe));
         const T conv = -h11 - h12 - h13 + cf.gx;
         rx[c] = cf.pmGamma * conv + cf.pmTheta * hx[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
         hx[c] = conv;
       }
       if (MASK & 2) {
-        const T v_c = uy[c], v_xp = uy[rowc + ip], v_xm = uy[rowc + im];
-        const T v_yp = uy[rowp + i], v_ym = uy[rowm + i];
-        const T v_zp = uy[rowkp + i], v_zm = uy[rowkm + i];
-        const T h21 = ((I1 * ux[rowm + ip] + I2 * ux[rowc + ip]) * (v_xp + v_c) -
-                       (I1 * ux[rowm + i] + I2 * ux[c]) * (v_c + v_xm)) *
+        const T h21 = ((I1 * ux[rowm + ip] + I2 * u_xp) * (v_xp + v_c) -
+                       (I1 * u_ym + I2 * u_c) * (v_c + v_xm)) *
                       hdx1;
         const T h22 = ((v_yp + v_c) * (v_yp + v_c) - (v_c + v_ym) * (v_c + v_ym)) * qsucac;
-        const T h23 = ((I1 * uz[rowkp - sj + i] + I2 * uz[rowkp + i]) * (v_zp + v_c) -
-                       (I1 * uz[rowm + i] + I2 * uz[c]) * (v_c + v_zm)) *
+        const T h23 = ((I1 * w_ym_p + I2 * w_p) * (v_p + v_c) - (I1 * w_ym + I2 * w_c) * (v_c + v_m)) *
                       hdx3;
         const T d11 = ap1c * v_xp + ac1c * v_c + am1c * v_xm;
-        const T d22 = m.ap2p[jc] * v_yp + m.ac2p[jc] * v_c + m.am2p[jc] * v_ym;
-        const T d33 = ap3c * v_zp + ac3c * v_c + am3c * v_zm;
+        const T d22 = ap2p * v_yp + ac2p * v_c + am2p * v_ym;
+        const T d33 = ap3c * v_p + ac3c * v_c + am3c * v_m;
         const T dcq = d11 + d22 + d33;
-        const T gradp = (pr[c] - pr[rowm + i]) * sucac;
+        const T gradp = (p_c - pr[rowm + i]) * sucac;
         const T conv = -h21 - h22 - h23 + cf.gy;
         ry[c] = cf.pmGamma * conv + cf.pmTheta * hy[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
         hy[c] = conv;
       }
       if (MASK & 4) {
-        const T w_c = uz[c], w_xp = uz[rowc + ip], w_xm = uz[rowc + im];
-        const T w_yp = uz[rowp + i], w_ym = uz[rowm + i];
-        const T w_zp = uz[rowkp + i], w_zm = uz[rowkm + i];
-        const T h31 = ((ux[rowc + ip] + ux[rowkm + ip]) * (w_xp + w_c) -
-                       (ux[c] + ux[rowkm + i]) * (w_c + w_xm)) *
-                      qdx1;
-        const T h32 = ((uy[rowp + i] + uy[rowkm + sj + i]) * (I3 * w_c + I4 * w_yp) -
-                       (uy[c] + uy[rowkm + i]) * (I1 * w_ym + I2 * w_c)) *
+        const T h31 = ((u_xp + uxp_m) * (w_xp + w_c) - (u_c + u_m) * (w_c + w_xm)) * qdx1;
+        const T h32 = ((v_yp + vyp_m) * (I3 * w_c + I4 * w_yp) - (v_c + v_m) * (I1 * w_ym + I2 * w_c)) *
                       sucaj;
-        const T h33 = ((w_zp + w_c) * (w_zp + w_c) - (w_c + w_zm) * (w_c + w_zm)) * qdx3;
+        const T h33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * qdx3;
         const T d11 = ap1c * w_xp + ac1c * w_c + am1c * w_xm;
-        const T d22 = m.ap2c[jc] * w_yp + m.ac2c[jc] * w_c + m.am2c[jc] * w_ym;
-        const T d33 = (w_zp - T(2) * w_c + w_zm) * cf.rdz2;
+        const T d22 = ap2c * w_yp + ac2c * w_c + am2c * w_ym;
+        const T d33 = (w_p - T(2) * w_c + w_m) * cf.rdz2;
         const T dcq = d11 + d22 + d33;
-        const T gradp = (pr[c] - pr[rowkm + i]) * cf.rdz;
+        const T gradp = (p_c - p_m) * cf.rdz;
         const T conv = -h31 - h32 - h33 + cf.gz;
         rz[c] = cf.pmGamma * conv + cf.pmTheta * hz[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
         hz[c] = conv;
       }
+      // roll
+      u_m = u_c; v_m = v_c; w_m = w_c; p_m = p_c;
+      u_c = u_p; v_c = v_p; w_c = w_p; p_c = p_p;
+      uxp_m = u_xp; vyp_m = v_yp;
+      w_xm = w_xm_p; w_ym = w_ym_p;
     }
   }
   if (MASK & 1) {
     block_sum_fix(s1, shf);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-      long long* pf = (long long*)partials +
-                      4LL * (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+      long long* pf = (long long*)partials + 4LL * blockIdx.x;
       for (int q = 0; q < 4; ++q) pf[q] = s1[q];
     }
   }
@@ -213,12 +240,13 @@ __global__ void k_bulk_finalize(const double* __restrict__ partials, int n, DevS
 
 template <typename T>
 __global__ void k_add_scalar(Geom g, T* __restrict__ f, const DevScalars* sc) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  CELL_BLOCK();
+  const int i = cbx * BX + threadIdx.x;
+  const int jc = cby * BY + threadIdx.y + 1;
   if (i >= g.nx || jc > g.ny) return;
   const T d = (T)sc->dpmdxns;
-  const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
-  for (int k = blockIdx.z * KCH; k < k1; ++k) f[fidx(g, i, jc, k)] += d;
+  const int k1 = min(g.nz, (cbz + 1) * KCH);
+  for (int k = cbz * KCH; k < k1; ++k) f[fidx(g, i, jc, k)] += d;
 }
 
 // ---------------------------------------------------------------------------
@@ -232,14 +260,15 @@ __global__ __launch_bounds__(BX* BY) void k_prsrc(Geom g, Metrics<T> m, T rdx, T
                                                   const T* __restrict__ uz, T* __restrict__ prsrc,
                                                   T* __restrict__ pr, double* __restrict__ partials) {
   __shared__ double sh[8];
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  CELL_BLOCK();
+  const int i = cbx * BX + threadIdx.x;
+  const int jc = cby * BY + threadIdx.y + 1;
   double dm = 0.0;
   if (i < g.nx && jc <= g.ny) {
     const int ip = wrapp(i, g.nx);
     const T sucaj = m.rdyp[jc];
-    const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
-    for (int k = blockIdx.z * KCH; k < k1; ++k) {
+    const int k1 = min(g.nz, (cbz + 1) * KCH);
+    for (int k = cbz * KCH; k < k1; ++k) {
       const int kp = wrapp(k, g.nz);
       const long long rowc = (long long)g.nx * jc + g.plane * k;
       const long long c = rowc + i;
@@ -252,7 +281,7 @@ __global__ __launch_bounds__(BX* BY) void k_prsrc(Geom g, Metrics<T> m, T rdx, T
   }
   const double r = block_max(dm, sh);
   if (threadIdx.x == 0 && threadIdx.y == 0)
-    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = r;
+    partials[blockIdx.x] = r;
 }
 
 __global__ void k_max_finalize(const double* __restrict__ partials, int n, double* out) {
@@ -349,13 +378,14 @@ __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2
                                                  T alpha, const T* __restrict__ phi,
                                                  T* __restrict__ ux, T* __restrict__ uy,
                                                  T* __restrict__ uz, T* __restrict__ pr) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  CELL_BLOCK();
+  const int i = cbx * BX + threadIdx.x;
+  const int jc = cby * BY + threadIdx.y + 1;
   if (i >= g.nx || jc > g.ny) return;
   const int im = wrapm(i, g.nx);
   const T sucacEta2 = m.rdyc[jc] * alpha;
-  const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
-  for (int k = blockIdx.z * KCH; k < k1; ++k) {
+  const int k1 = min(g.nz, (cbz + 1) * KCH);
+  for (int k = cbz * KCH; k < k1; ++k) {
     const int km = wrapm(k, g.nz);
     const long long rowc = (long long)g.nx * jc + g.plane * k;
     const long long c = rowc + i;
@@ -385,16 +415,17 @@ __global__ __launch_bounds__(BX* BY) void k_diag(Geom g, Metrics<T> m, T rdx, T 
                                                  const double* __restrict__ dyp,
                                                  double* __restrict__ partials, int nblocks) {
   __shared__ double sh[8];
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int jc = blockIdx.y * BY + threadIdx.y + 1;
+  CELL_BLOCK();
+  const int i = cbx * BX + threadIdx.x;
+  const int jc = cby * BY + threadIdx.y + 1;
   double cfl = 0, dv = 0, vx = 0, vy = 0, vz = 0;
   long long us[4] = {0, 0, 0, 0};
   if (i < g.nx && jc <= g.ny) {
     const int ip = wrapp(i, g.nx);
     const T rdyp = m.rdyp[jc];
     const double dy = dyp[jc];
-    const int k1 = min(g.nz, (int)(blockIdx.z + 1) * KCH);
-    for (int k = blockIdx.z * KCH; k < k1; ++k) {
+    const int k1 = min(g.nz, (cbz + 1) * KCHD);
+    for (int k = cbz * KCHD; k < k1; ++k) {
       const int kp = wrapp(k, g.nz);
       const long long rowc = (long long)g.nx * jc + g.plane * k;
       const long long c = rowc + i;
@@ -417,7 +448,7 @@ __global__ __launch_bounds__(BX* BY) void k_diag(Geom g, Metrics<T> m, T rdx, T 
       if (WHAT & 8) fix_add(us, (double)u * dy);
     }
   }
-  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int b = blockIdx.x;
   double r;
   if (WHAT & 1) { r = block_max(cfl, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[0 * (long long)nblocks + b] = r; __syncthreads(); }
   if (WHAT & 2) { r = block_max(dv, sh); if (threadIdx.x == 0 && threadIdx.y == 0) partials[1 * (long long)nblocks + b] = r; __syncthreads(); }
@@ -474,13 +505,10 @@ __global__ void k_diag_finalize(const double* __restrict__ partials, int n, int 
 // ===========================================================================
 // launchers
 // ===========================================================================
-static inline dim3 cell_grid(const Geom& g) {
-  return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + KCH - 1) / KCH);
+static inline int ncell_blocks(const Geom& g, int kch = KCH) {
+  return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + kch - 1) / kch);
 }
-static inline int ncell_blocks(const Geom& g) {
-  dim3 d = cell_grid(g);
-  return (int)(d.x * d.y * d.z);
-}
+static inline dim3 cell_grid(const Geom& g) { return dim3(ncell_blocks(g)); }
 
 template <typename T>
 static Coef<T> make_coef(c3d_ctx* c) {
@@ -500,18 +528,34 @@ const Metrics<double>& mets<double>(c3d_ctx* c) { return c->md; }
 template <>
 const Metrics<float>& mets<float>(c3d_ctx* c) { return c->mf; }
 
+// z run of the RHS kernel: long enough to amortise the two lead-in planes,
+// short enough to give the device a few thousand workgroups
+static int rhs_kchunk(const Geom& g) {
+  const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
+  int nch = (4096 + tiles - 1) / tiles;
+  if (nch < 1) nch = 1;
+  int kc = (g.nz + nch - 1) / nch;
+  if (kc < 16) kc = 16;
+  return kc > g.nz ? g.nz : kc;
+}
+static int rhs_nblocks(const Geom& g) {
+  const int kc = rhs_kchunk(g);
+  return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + kc - 1) / kc);
+}
+
 template <typename T>
 static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
                  const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz) {
   const Geom& g = c->g;
-  dim3 grid = cell_grid(g), blk(BX, BY);
+  const int kc = rhs_kchunk(g);
+  dim3 grid(rhs_nblocks(g)), blk(BX, BY);
   Coef<T> cf = make_coef<T>(c);
   const Metrics<T>& m = mets<T>(c);
   auto U = [](const void* p) { return (const T*)p; };
   auto W = [](void* p) { return (T*)p; };
 #define RHS_LAUNCH(M)                                                                         \
   hipLaunchKernelGGL((k_rhs<T, M>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz),  \
-                     U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials)
+                     U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials, kc)
   switch (mask) {
     case 1: RHS_LAUNCH(1); break;
     case 2: RHS_LAUNCH(2); break;
@@ -537,7 +581,7 @@ int launch_rhs(c3d_ctx* c, int mask, const void* ux, const void* uy, const void*
   const int ncomp = (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1);
   prof_end(c, st, ev, (ncomp == 3 ? 13.0 : 6.0) * N * c->esize);
   if (want_bulk && (mask & 1) && c->p.is_ux_const)
-    return launch_bulk_finalize(c, ncell_blocks(c->g), c->cur_ns == 1);
+    return launch_bulk_finalize(c, rhs_nblocks(c->g), c->cur_ns == 1);
   return C3D_OK;
 }
 
@@ -695,10 +739,11 @@ template <typename T>
 static int diag_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int what,
                   int cfl_slot) {
   const Geom& g = c->g;
-  const int nb = ncell_blocks(g);
+  const int nb = ncell_blocks(g, KCHD);
+  const dim3 dgrid(nb);
   const double* dyp = c->d_dyp;
 #define DIAG_LAUNCH(W)                                                                           \
-  hipLaunchKernelGGL((k_diag<T, W>), cell_grid(g), dim3(BX, BY), 0, c->stream, g, mets<T>(c),    \
+  hipLaunchKernelGGL((k_diag<T, W>), dgrid, dim3(BX, BY), 0, c->stream, g, mets<T>(c),           \
                      (T)c->rdx, (T)c->rdz, (const T*)ux, (const T*)uy, (const T*)uz, dyp,        \
                      c->d_partials, nb)
   switch (what) {

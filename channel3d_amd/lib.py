@@ -30,7 +30,7 @@ STATE_LAYOUT = {UX: HALO, UY: HALO, UZ: HALO, PR: HALO, HISTX: INTERIOR, HISTY: 
 EXPORTS = (
     "c3d_create", "c3d_destroy", "c3d_last_error", "c3d_sync", "c3d_mesh_coords",
     "c3d_get_metric", "c3d_set_dt", "c3d_field_alloc", "c3d_field_free", "c3d_upload",
-    "c3d_download", "c3d_state_field", "c3d_PMcoeUpdate", "c3d_Update_uy_ym", "c3d_clcRhsX",
+    "c3d_download", "c3d_field_copy", "c3d_state_field", "c3d_PMcoeUpdate", "c3d_Update_uy_ym", "c3d_clcRhsX",
     "c3d_clcRhsY", "c3d_clcRhsZ", "c3d_clcU1Hat", "c3d_clcU2Hat", "c3d_clcU3Hat",
     "c3d_SetBC_and_UpdateHaloForPrSrc", "c3d_SetBC_and_UpdateHalo_pr",
     "c3d_SetBC_and_UpdateHalo", "c3d_clcPrSrc", "c3d_clcPPE", "c3d_FluidVelUpdate",
@@ -90,6 +90,7 @@ def load(path: str | None = None):
         "c3d_field_free": ([_VP, _VP], I),
         "c3d_upload": ([_VP, I, _VP, _DP], I),
         "c3d_download": ([_VP, I, _VP, _DP], I),
+        "c3d_field_copy": ([_VP, I, _VP, _VP], I),
         "c3d_state_field": ([_VP, I, ctypes.POINTER(_VP)], I),
         "c3d_PMcoeUpdate": ([_VP, I], I),
         "c3d_Update_uy_ym": ([_VP, _VP, _VP], I),

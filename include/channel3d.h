@@ -105,6 +105,9 @@ int c3d_field_alloc(c3d_ctx* ctx, int layout, void** dptr);
 int c3d_field_free(c3d_ctx* ctx, void* dptr);
 int c3d_upload(c3d_ctx* ctx, int layout, void* dptr, const double* host);
 int c3d_download(c3d_ctx* ctx, int layout, const void* dptr, double* host);
+/* device-to-device copy of a whole field of the given layout (asynchronous);
+ * a plain streaming kernel, also the known-size reference for traffic counters */
+int c3d_field_copy(c3d_ctx* ctx, int layout, void* dst, const void* src);
 
 /* the context's own channel state (m_Variables.f90:10-24) */
 enum {
