@@ -22,6 +22,8 @@ sys.path.insert(0, ROOT)
 METRIC = ("RK3 steps/s & Poisson solves/s at 768×384×576; FP32+refine vs FP64 at "
           "equal error")
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (HBM bytes per launch)
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "fp64_step_768x384x576.md")
 
 PROD = dict(nxc=768, nyc=384, nzc=576, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
             xnu=9.9369e-5, dt=0.002, gamma=1.5, ubulk=1.0)
@@ -29,6 +31,23 @@ PROD = dict(nxc=768, nyc=384, nzc=576, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
 
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel, read from the committed
+    rocprofv3 counter summary of the full-size workload; None if absent"""
+    try:
+        lines = open(TRAFFIC_TABLE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) >= 7 and cols[0] == stage:
+            try:
+                return float(cols[5])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_threads(requested):
@@ -180,8 +199,9 @@ def main(argv=None):
         s = stages[dom]
         avg = s["total_ms"] / s["calls"]
         ach = s["bytes_per_call"] / (avg * 1e-3) / 1e9
+        traffic = None if a.small else measured_traffic(dom)
         roof = dict(kernel=dom, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic,
                     bytes_per_launch=s["bytes_per_call"], avg_ms=round(avg, 4))
 
     cpu = None
