@@ -20,9 +20,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchannel3d.so")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "_build")
+FCHECK = os.path.join(FDIR, "c3d_fortran_check")
 
 ARCH = os.environ.get("C3D_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FC_AMD", shutil.which("amdflang") or "/opt/rocm/bin/amdflang")
 
 CXXFLAGS = [
     f"--offload-arch={ARCH}",
@@ -76,6 +80,25 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the ISO_C_BINDING modules (channel3d_c, m_channel3d_gpu) and
+    the Fortran driver check program against libchannel3d.so.  Returns the
+    program path, or None when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    srcs = [os.path.join(FDIR, f) for f in
+            ("channel3d_c.f90", "m_channel3d_gpu.f90", "c3d_fortran_check.f90")]
+    cmd = [FLANG, "-O2", "-module-dir", FBUILD, "-o", FCHECK, *srcs,
+           f"-L{HERE}", "-lchannel3d", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=FBUILD)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed\n{r.stdout}\n{r.stderr}")
+    return FCHECK
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
@@ -83,6 +106,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     path = build(a.jobs, a.verbose)
     print(path)
+    f = build_fortran(a.verbose)
+    print(f or "amdflang not found: Fortran binding not built")
 
 
 if __name__ == "__main__":

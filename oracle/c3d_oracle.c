@@ -13,6 +13,7 @@
  * depend on the thread count.
  */
 #include "c3d_oracle.h"
+#include "c3d_oracle_int.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -26,12 +27,15 @@
 /* FFTW's R2HC/HC2R (FFTW 3.3.9 manual, "The Halfcomplex-format DFT") are     */
 /* restated on top of it.                                                    */
 /* ------------------------------------------------------------------------- */
-typedef struct fftplan {
-  int n, nf, fac[40];
-  double *twr, *twi; /* exp(-2 pi i k / n), k=0..n-1, rounded from long double */
-} fftplan;
+#define fft_plan_init orci_fft_plan_init
+#define fft_plan_free orci_fft_plan_free
+#define r2hc_plan orci_r2hc
+#define hc2r_plan orci_hc2r
+#define xsum_add orci_xsum_add
+#define xsum_value orci_xsum_value
+#define thomas_line orci_thomas_line
 
-static void fft_plan_init(fftplan* p, int n) {
+void fft_plan_init(fftplan* p, int n) {
   p->n = n;
   p->nf = 0;
   int m = n;
@@ -50,7 +54,7 @@ static void fft_plan_init(fftplan* p, int n) {
   }
 }
 
-static void fft_plan_free(fftplan* p) {
+void fft_plan_free(fftplan* p) {
   free(p->twr);
   free(p->twi);
 }
@@ -95,7 +99,7 @@ static void fft_rec(const fftplan* p, const double* xr, const double* xi, int xs
 }
 
 /* FFTW_R2HC (unnormalised): out[k]=Re X_k (k<=n/2), out[n-k]=Im X_k (0<k<(n+1)/2) */
-static void r2hc_plan(const fftplan* p, const double* in, double* out, double* wk) {
+void r2hc_plan(const fftplan* p, const double* in, double* out, double* wk) {
   const int n = p->n;
   double *xr = wk, *xi = wk + n, *yr = wk + 2 * n, *yi = wk + 3 * n;
   for (int j = 0; j < n; ++j) { xr[j] = in[j]; xi[j] = 0.0; }
@@ -105,7 +109,7 @@ static void r2hc_plan(const fftplan* p, const double* in, double* out, double* w
 }
 
 /* FFTW_HC2R (unnormalised inverse of R2HC) */
-static void hc2r_plan(const fftplan* p, const double* in, double* out, double* wk) {
+void hc2r_plan(const fftplan* p, const double* in, double* out, double* wk) {
   const int n = p->n;
   double *xr = wk, *xi = wk + n, *yr = wk + 2 * n, *yi = wk + 3 * n;
   xr[0] = in[0];
@@ -147,7 +151,7 @@ void orc_hc2r(int n, const double* in, double* out) {
 /* 32-bit limbs in int64.  The result is the same for any thread count or    */
 /* summation order and within N*2^-80 of the exact sum of the terms.         */
 /* ------------------------------------------------------------------------- */
-static void xsum_add(long long* acc, double t) {
+void xsum_add(long long* acc, double t) {
   if (t == 0.0) return;
   int e2;
   double m = frexp(fabs(t), &e2); /* |t| = m 2^e2, m in [0.5, 1) */
@@ -169,7 +173,7 @@ static void xsum_add(long long* acc, double t) {
   for (int q = 0; q < 4; ++q) acc[q] += (t < 0) ? -l[q] : l[q];
 }
 
-static double xsum_value(const long long* acc) {
+double xsum_value(const long long* acc) {
   return (((double)acc[3] * ldexp(1.0, 16) + (double)acc[2] * ldexp(1.0, -16)) +
           (double)acc[1] * ldexp(1.0, -48)) + (double)acc[0] * ldexp(1.0, -80);
 }
@@ -711,7 +715,7 @@ void orc_InverseTridiagonal(const double* aj, const double* bj, const double* cj
 }
 
 /* single-line form of InverseTridiagonal with the same operation order */
-static void thomas_line(const double* a, const double* b, const double* cc, double* f, double* arr,
+void thomas_line(const double* a, const double* b, const double* cc, double* f, double* arr,
                         int n, size_t fs) {
   double vecm = b[0];
   f[0] = f[0] / vecm;

@@ -1,0 +1,922 @@
+/*
+ * c3d_oracle4.c -- CPU FP64 restatement of Channel3d's 4th-order channel
+ * step, src/CFD_4th (TEST INFRASTRUCTURE: correctness oracle).
+ *
+ * Every routine follows the reference routine cited above it, in the
+ * reference's array layout and with its expressions evaluated in the same
+ * order.  On one rank the 2DECOMP transposes are copies and the periodic
+ * halo exchange is a wrap copy of the interior (src/CFD_4th/m_Decomp2d.f90:
+ * 1588-1743); both are restated as such.  Sums that the reference
+ * accumulates sequentially (the wall-shear integral of the constant-flow-rate
+ * forcing and the bulk velocity) use the order-independent fixed-point sum of
+ * the 2nd-order oracle, so results do not depend on the thread count.
+ */
+#include "c3d_oracle4.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "c3d_oracle_int.h"
+
+#ifdef _OPENMP
+#define OMP_NT num_threads(orc_get_threads())
+#endif
+
+struct orc4_ctx {
+  orc4_params p;
+  int nxc, nyc, nzc, nyp;
+  size_t hx, hy, hz, nhalo, nint, nwall;
+  double dx, dz, rdx, rdz, rdx2, rdz2;
+  /* m_MeshAndMetries.f90:87-91 */
+  double dxCoe1, dxCoe2, dxCoe3, dxCoe4, dzCoe1, dzCoe2, dzCoe3, dzCoe4;
+  double dxxCoe1, dxxCoe2, dxxCoe3, dxxCoe4, dxxCoe5;
+  double dzzCoe1, dzzCoe2, dzzCoe3, dzzCoe4, dzzCoe5;
+  double InterpCoe1, InterpCoe2, InterpCoe3, InterpCoe4;
+  double* mbuf[ORC4_NMETRICS];
+  double* met[ORC4_NMETRICS];
+  double normfft;
+  double gconst[3], tconst[3], aconst[3];
+  int iadvance;
+  double pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, SimTime, dt;
+  double dpmdxns, PrGradAver;
+  double divmax1, divmax2, cfl, uxaver, vmax[3];
+  double* f[ORC_NFIELDS];
+  fftplan px, pz;
+};
+
+/* halo (-1:nxc+2, 0:nyc+1, -1:nzc+2), interior (1:nxc,1:nyc,1:nzc), wall (1:nxc,1:nzc) */
+#define H4(c, i, j, k) ((size_t)((i) + 2) + (c)->hx * ((size_t)(j) + (c)->hy * (size_t)((k) + 2)))
+#define I4(c, i, j, k) \
+  ((size_t)((i)-1) + (size_t)(c)->nxc * ((size_t)((j)-1) + (size_t)(c)->nyc * (size_t)((k)-1)))
+#define W4(c, i, k) ((size_t)((i)-1) + (size_t)(c)->nxc * (size_t)((k)-1))
+
+double* orc4_field(orc4_ctx* c, int id) { return (id >= 0 && id < ORC_NFIELDS) ? c->f[id] : NULL; }
+double* orc4_metric(orc4_ctx* c, int id) { return (id >= 0 && id < ORC4_NMETRICS) ? c->met[id] : NULL; }
+
+double orc4_scalar(orc4_ctx* c, int id) {
+  switch (id) {
+    case ORC_S_PMGAMMA: return c->pmGamma;
+    case ORC_S_PMTHETA: return c->pmTheta;
+    case ORC_S_PMALPHA: return c->pmAlpha;
+    case ORC_S_PMALPHAC: return c->pmAlphaC;
+    case ORC_S_PMBETA: return c->pmBeta;
+    case ORC_S_SIMTIME: return c->SimTime;
+    case ORC_S_PRGRADNOW: return c->dpmdxns;
+    case ORC_S_PRGRADAVER: return c->PrGradAver;
+    case ORC_S_NORMFFT: return c->normfft;
+    case ORC_S_DIVMAX1: return c->divmax1;
+    case ORC_S_DIVMAX2: return c->divmax2;
+    case ORC_S_CFL: return c->cfl;
+    case ORC_S_UXAVER: return c->uxaver;
+    case ORC_S_VMAX1: return c->vmax[0];
+    case ORC_S_VMAX2: return c->vmax[1];
+    case ORC_S_VMAX3: return c->vmax[2];
+    case ORC_S_DT: return c->dt;
+  }
+  return 0.0;
+}
+
+void orc4_set_scalar(orc4_ctx* c, int id, double v) {
+  if (id == ORC_S_SIMTIME) c->SimTime = v;
+  else if (id == ORC_S_PRGRADAVER) c->PrGradAver = v;
+  else if (id == ORC_S_DT) c->dt = v;
+}
+
+static double* met4_alloc(orc4_ctx* c, int id, int lo, int hi) {
+  c->mbuf[id] = (double*)calloc((size_t)(hi - lo + 1), sizeof(double));
+  c->met[id] = c->mbuf[id] - lo;
+  return c->met[id];
+}
+
+orc4_ctx* orc4_create(const orc4_params* p, const double* yp_in) {
+  if (p->nxc < 4 || p->nyc < 3 || p->nzc < 4) return NULL;
+  if (p->flow_type != ORC4_FT_CH && p->flow_type != ORC4_FT_HC) return NULL;
+  orc4_ctx* c = (orc4_ctx*)calloc(1, sizeof(orc4_ctx));
+  c->p = *p;
+  const int nxc = p->nxc, nyc = p->nyc, nzc = p->nzc, nyp = nyc + 1;
+  c->nxc = nxc; c->nyc = nyc; c->nzc = nzc; c->nyp = nyp;
+  c->hx = (size_t)nxc + 4; c->hy = (size_t)nyc + 2; c->hz = (size_t)nzc + 4;
+  c->nhalo = c->hx * c->hy * c->hz;
+  c->nint = (size_t)nxc * nyc * nzc;
+  c->nwall = (size_t)nxc * nzc;
+
+  /* m_MeshAndMetries.f90:76-91 */
+  c->dx = p->xlx / (double)nxc;
+  c->dz = p->zlz / (double)nzc;
+  c->rdx = (double)nxc / p->xlx;
+  c->rdz = (double)nzc / p->zlz;
+  c->rdx2 = c->rdx * c->rdx;
+  c->rdz2 = c->rdz * c->rdz;
+  const double rdx = c->rdx, rdz = c->rdz, rdx2 = c->rdx2, rdz2 = c->rdz2;
+  c->dxCoe1 = rdx / 24.0; c->dxCoe2 = -rdx * 1.125; c->dxCoe3 = rdx * 1.125; c->dxCoe4 = -rdx / 24.0;
+  c->dzCoe1 = rdz / 24.0; c->dzCoe2 = -rdz * 1.125; c->dzCoe3 = rdz * 1.125; c->dzCoe4 = -rdz / 24.0;
+  c->dxxCoe1 = -rdx2 / 12.0; c->dxxCoe2 = 4.0 / 3.0 * rdx2; c->dxxCoe3 = -2.5 * rdx2;
+  c->dxxCoe4 = 4.0 / 3.0 * rdx2; c->dxxCoe5 = -rdx2 / 12.0;
+  c->dzzCoe1 = -rdz2 / 12.0; c->dzzCoe2 = 4.0 / 3.0 * rdz2; c->dzzCoe3 = -2.5 * rdz2;
+  c->dzzCoe4 = 4.0 / 3.0 * rdz2; c->dzzCoe5 = -rdz2 / 12.0;
+  c->InterpCoe1 = -1.0 / 16.0; c->InterpCoe2 = 9.0 / 16.0;
+  c->InterpCoe3 = 9.0 / 16.0; c->InterpCoe4 = -1.0 / 16.0;
+
+  double* yp = met4_alloc(c, ORC_YP, 0, nyp);
+  double* yc = met4_alloc(c, ORC_YC, 0, nyp);
+  double* dyp = met4_alloc(c, ORC_DYP, 0, nyp);
+  double* dyc = met4_alloc(c, ORC_DYC, 0, nyp);
+  double* rdyp = met4_alloc(c, ORC_RDYP, 0, nyp);
+  double* rdyc = met4_alloc(c, ORC_RDYC, 0, nyp);
+  double* yint = met4_alloc(c, ORC_YINTERP, 0, nyp);
+  double* am2Pr = met4_alloc(c, ORC_AM2PR, 0, nyc);
+  double* ac2Pr = met4_alloc(c, ORC_AC2PR, 0, nyc);
+  double* ap2Pr = met4_alloc(c, ORC_AP2PR, 0, nyc);
+  double* am2c = met4_alloc(c, ORC_AM2C, 0, nyc);
+  double* ac2c = met4_alloc(c, ORC_AC2C, 0, nyc);
+  double* ap2c = met4_alloc(c, ORC_AP2C, 0, nyc);
+  double* am2p = met4_alloc(c, ORC_AM2P, 0, nyc);
+  double* ac2p = met4_alloc(c, ORC_AC2P, 0, nyc);
+  double* ap2p = met4_alloc(c, ORC_AP2P, 0, nyc);
+  double* am2cCN = met4_alloc(c, ORC_AM2CCN, 0, nyc);
+  double* ap2cCN = met4_alloc(c, ORC_AP2CCN, 0, nyc);
+  double* ac2cCN = met4_alloc(c, ORC4_AC2CCN, 0, nyc);
+  double* wx = met4_alloc(c, ORC_WAVEX, 0, nxc);
+  double* wz = met4_alloc(c, ORC_WAVEZ, 0, nzc);
+
+  for (int j = 1; j <= nyp; ++j) yp[j] = yp_in[j];
+  /* m_MeshAndMetries.f90:171-211 */
+  for (int j = 1; j <= nyc; ++j) yc[j] = 0.5 * (yp[j] + yp[j + 1]);
+  yc[0] = 2.0 * yp[1] - yc[1];
+  yc[nyp] = 2.0 * yp[nyp] - yc[nyc];
+  for (int j = 1; j <= nyc; ++j) dyp[j] = yp[j + 1] - yp[j];
+  dyp[0] = dyp[1];
+  dyp[nyp] = dyp[nyc];
+  for (int j = 2; j <= nyc; ++j) dyc[j] = yc[j] - yc[j - 1];
+  dyc[1] = dyp[1];
+  dyc[nyp] = dyp[nyc];
+  for (int j = 0; j <= nyp; ++j) rdyp[j] = 1.0 / dyp[j];
+  for (int j = 1; j <= nyp; ++j) rdyc[j] = 1.0 / dyc[j];
+  /* :214-216 */
+  for (int j = 1; j <= nyp; ++j) yint[j] = dyp[j] / (dyp[j] + dyp[j - 1]);
+  /* :219-228 */
+  for (int j = 1; j <= nyc; ++j) {
+    am2Pr[j] = rdyp[j] * rdyc[j];
+    ap2Pr[j] = rdyp[j] * rdyc[j + 1];
+  }
+  am2Pr[1] = 0.0;
+  ap2Pr[1] = rdyp[1] * rdyc[2];
+  am2Pr[nyc] = rdyp[nyc] * rdyc[nyc];
+  ap2Pr[nyc] = 0.0;
+  for (int j = 1; j <= nyc; ++j) ac2Pr[j] = -(am2Pr[j] + ap2Pr[j]);
+  /* :230-243 */
+  for (int j = 1; j <= nyc; ++j) {
+    am2c[j] = rdyp[j] * rdyc[j];
+    ap2c[j] = rdyp[j] * rdyc[j + 1];
+  }
+  am2c[1] = 4.0 * rdyc[1] / (dyc[1] + 2.0 * dyc[2]);
+  ap2c[1] = 4.0 * rdyc[2] / (dyc[1] + 2.0 * dyc[2]);
+  if (p->flow_type == ORC4_FT_CH) {
+    am2c[nyc] = 4.0 * rdyc[nyc] / (dyc[nyp] + 2.0 * dyc[nyc]);
+    ap2c[nyc] = 4.0 * rdyc[nyp] / (dyc[nyp] + 2.0 * dyc[nyc]);
+  } else {
+    am2c[nyc] = rdyp[nyc] * rdyc[nyc];
+    ap2c[nyc] = 0.0;
+  }
+  for (int j = 1; j <= nyc; ++j) ac2c[j] = -(am2c[j] + ap2c[j]);
+  /* :245-250 */
+  for (int j = 1; j <= nyc; ++j) {
+    am2p[j] = rdyc[j] * rdyp[j - 1];
+    ap2p[j] = rdyc[j] * rdyp[j];
+  }
+  for (int j = 1; j <= nyc; ++j) ac2p[j] = -(am2p[j] + ap2p[j]);
+  /* m_TScheme.f90:41-51 */
+  for (int j = 1; j <= nyc; ++j) { ap2cCN[j] = ap2c[j]; ac2cCN[j] = ac2c[j]; am2cCN[j] = am2c[j]; }
+  ap2cCN[1] = ap2c[1];
+  ac2cCN[1] = ac2c[1] - am2c[1];
+  am2cCN[1] = 2.0 * am2c[1];
+  if (p->flow_type == ORC4_FT_CH) {
+    am2cCN[nyc] = am2c[nyc];
+    ac2cCN[nyc] = ac2c[nyc] - ap2c[nyc];
+    ap2cCN[nyc] = 2.0 * ap2c[nyc];
+  }
+  /* m_Poisson.f90:40-48 */
+  const double twoPi = 6.283185307179586476925286766559005768;
+  for (int i = 1; i <= nxc; ++i) {
+    double wa1 = twoPi * (double)(i - 1) / (double)nxc;
+    wx[i] = rdx2 / 288.0 * (cos(3.0 * wa1) - 54.0 * cos(2.0 * wa1) + 783.0 * cos(wa1) - 730.0);
+  }
+  for (int k = 1; k <= nzc; ++k) {
+    double wa3 = twoPi * (double)(k - 1) / (double)nzc;
+    wz[k] = rdz2 / 288.0 * (cos(3.0 * wa3) - 54.0 * cos(2.0 * wa3) + 783.0 * cos(wa3) - 730.0);
+  }
+  c->normfft = 1.0 / ((double)nxc * (double)nzc);
+
+  /* m_Parameters.f90:158-173 */
+  if (p->ischeme == 1) {
+    c->iadvance = 1;
+    c->gconst[0] = 1.5; c->gconst[1] = 0.0; c->gconst[2] = 0.0;
+    c->tconst[0] = -0.5; c->tconst[1] = 0.0; c->tconst[2] = 0.0;
+  } else if (p->ischeme == 2) {
+    c->iadvance = 2;
+    c->gconst[0] = 0.5; c->gconst[1] = 1.0; c->gconst[2] = 0.0;
+    c->tconst[0] = 0.0; c->tconst[1] = -0.5; c->tconst[2] = 0.0;
+  } else {
+    c->iadvance = 3;
+    c->gconst[0] = 8.0 / 15.0; c->gconst[1] = 5.0 / 12.0; c->gconst[2] = 3.0 / 4.0;
+    c->tconst[0] = 0.0; c->tconst[1] = -17.0 / 60.0; c->tconst[2] = -5.0 / 12.0;
+  }
+  for (int s = 0; s < 3; ++s) c->aconst[s] = c->gconst[s] + c->tconst[s];
+  c->dt = p->dt;
+
+  for (int id = 0; id < ORC_NFIELDS; ++id) {
+    size_t n = (id <= ORC_PR || id == ORC_HALO) ? c->nhalo
+             : (id == ORC_UY_YM || id == ORC_DUY_YM) ? c->nwall : c->nint;
+    c->f[id] = (double*)calloc(n, sizeof(double));
+  }
+  orci_fft_plan_init(&c->px, nxc);
+  orci_fft_plan_init(&c->pz, nzc);
+  return c;
+}
+
+void orc4_destroy(orc4_ctx* c) {
+  if (!c) return;
+  for (int id = 0; id < ORC4_NMETRICS; ++id) free(c->mbuf[id]);
+  for (int id = 0; id < ORC_NFIELDS; ++id) free(c->f[id]);
+  orci_fft_plan_free(&c->px);
+  orci_fft_plan_free(&c->pz);
+  free(c);
+}
+
+/* m_Parameters.f90:213-228 */
+void orc4_PMcoeUpdate(orc4_ctx* c, int ns) {
+  const int s = ns - 1;
+  c->pmGamma = c->gconst[s] * c->dt;
+  c->pmTheta = c->tconst[s] * c->dt;
+  c->pmAlpha = c->aconst[s] * c->dt;
+  c->pmBeta = 0.5 * c->aconst[s] * c->p.xnu * c->dt;
+  c->SimTime = c->SimTime + c->dt * c->aconst[s];
+  c->pmAlphaC = c->aconst[s];
+}
+
+/* m_FlowCase.f90:162-172 */
+void orc4_Update_uy_ym(orc4_ctx* c, double* uy_ym, double* duy_ym) {
+  for (size_t q = 0; q < c->nwall; ++q) {
+    duy_ym[q] = uy_ym[q];
+    uy_ym[q] = 0.0;
+    duy_ym[q] = uy_ym[q] - duy_ym[q];
+  }
+}
+
+/* myupdate_halo on one rank for x/z periodic halos of the given depths
+ * (m_Decomp2d.f90:1654-1738: x over all y and z, then z over all x and y) */
+static void halo4_xz(orc4_ctx* c, double* a, int xmh, int xph, int zmh, int zph) {
+  const int nxc = c->nxc, nzc = c->nzc, nyp = c->nyp;
+  for (int k = -1; k <= nzc + 2; ++k)
+    for (int j = 0; j <= nyp; ++j) {
+      for (int h = 1; h <= xmh; ++h) a[H4(c, 1 - h, j, k)] = a[H4(c, nxc + 1 - h, j, k)];
+      for (int h = 1; h <= xph; ++h) a[H4(c, nxc + h, j, k)] = a[H4(c, h, j, k)];
+    }
+  for (int k = 1; k <= zmh; ++k)
+    for (int j = 0; j <= nyp; ++j)
+      for (int i = -1; i <= nxc + 2; ++i) a[H4(c, i, j, 1 - k)] = a[H4(c, i, j, nzc + 1 - k)];
+  for (int k = 1; k <= zph; ++k)
+    for (int j = 0; j <= nyp; ++j)
+      for (int i = -1; i <= nxc + 2; ++i) a[H4(c, i, j, nzc + k)] = a[H4(c, i, j, k)];
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_TScheme.f90:189-438  clcRhsXYZ (the branch without scalar / particles)   */
+/* ------------------------------------------------------------------------- */
+void orc4_clcRhsXYZ(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                    double* rhsx, double* rhsy, double* rhsz, double* histx, double* histy,
+                    double* histz, const double* pr) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nyp = c->nyp;
+  const double I1c = c->InterpCoe1, I2c = c->InterpCoe2, I3c = c->InterpCoe3, I4c = c->InterpCoe4;
+  const double dxCoe1 = c->dxCoe1, dxCoe2 = c->dxCoe2, dxCoe3 = c->dxCoe3, dxCoe4 = c->dxCoe4;
+  const double dzCoe1 = c->dzCoe1, dzCoe2 = c->dzCoe2, dzCoe3 = c->dzCoe3, dzCoe4 = c->dzCoe4;
+  const double dxxCoe1 = c->dxxCoe1, dxxCoe2 = c->dxxCoe2, dxxCoe3 = c->dxxCoe3,
+               dxxCoe4 = c->dxxCoe4, dxxCoe5 = c->dxxCoe5;
+  const double dzzCoe1 = c->dzzCoe1, dzzCoe2 = c->dzzCoe2, dzzCoe3 = c->dzzCoe3,
+               dzzCoe4 = c->dzzCoe4, dzzCoe5 = c->dzzCoe5;
+  const double uCRF = c->p.u_crf, xnu = c->p.xnu;
+  const double pmGamma = c->pmGamma, pmTheta = c->pmTheta, pmAlpha = c->pmAlpha,
+               pmBeta = c->pmBeta;
+  const double* rdyp = c->met[ORC_RDYP];
+  const double* rdyc = c->met[ORC_RDYC];
+  const double* dyp = c->met[ORC_DYP];
+  const double* yint = c->met[ORC_YINTERP];
+  const double *am2c = c->met[ORC_AM2C], *ac2c = c->met[ORC_AC2C], *ap2c = c->met[ORC_AP2C];
+  const double *am2p = c->met[ORC_AM2P], *ac2p = c->met[ORC_AC2P], *ap2p = c->met[ORC_AP2P];
+#define UX(i, j, k) ux[H4(c, i, j, k)]
+#define UY(i, j, k) uy[H4(c, i, j, k)]
+#define UZ(i, j, k) uz[H4(c, i, j, k)]
+#define PR(i, j, k) pr[H4(c, i, j, k)]
+  double* uxhx = (double*)calloc(c->nhalo, sizeof(double));
+  double* uyhx = (double*)calloc(c->nhalo, sizeof(double));
+  double* uzhx = (double*)calloc(c->nhalo, sizeof(double));
+  double* uxhz = (double*)calloc(c->nhalo, sizeof(double));
+  double* uyhz = (double*)calloc(c->nhalo, sizeof(double));
+  double* uzhz = (double*)calloc(c->nhalo, sizeof(double));
+#define A(a, i, j, k) a[H4(c, i, j, k)]
+  /* Update_VelInterp_uvw, :203-233 */
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+    for (int jc = 1; jc <= nyc; ++jc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        A(uxhx, ic, jc, kc) = I1c * UX(im, jc, kc) + I2c * UX(ic, jc, kc) + I3c * UX(ip, jc, kc) + I4c * UX(iu, jc, kc);
+        A(uyhx, ic, jc, kc) = I1c * UY(is, jc, kc) + I2c * UY(im, jc, kc) + I3c * UY(ic, jc, kc) + I4c * UY(ip, jc, kc);
+        A(uzhx, ic, jc, kc) = I1c * UZ(is, jc, kc) + I2c * UZ(im, jc, kc) + I3c * UZ(ic, jc, kc) + I4c * UZ(ip, jc, kc);
+        A(uxhz, ic, jc, kc) = I1c * UX(ic, jc, ks) + I2c * UX(ic, jc, km) + I3c * UX(ic, jc, kc) + I4c * UX(ic, jc, kp);
+        A(uyhz, ic, jc, kc) = I1c * UY(ic, jc, ks) + I2c * UY(ic, jc, km) + I3c * UY(ic, jc, kc) + I4c * UY(ic, jc, kp);
+        A(uzhz, ic, jc, kc) = I1c * UZ(ic, jc, km) + I2c * UZ(ic, jc, kc) + I3c * UZ(ic, jc, kp) + I4c * UZ(ic, jc, ku);
+      }
+    for (int ic = 1; ic <= nxc; ++ic) {
+      A(uyhx, ic, nyp, kc) = 0.0;
+      A(uyhz, ic, nyp, kc) = 0.0;
+    }
+  }
+  /* m_TScheme.f90:85-90 halo depths */
+  halo4_xz(c, uxhx, 2, 1, 0, 0);
+  halo4_xz(c, uyhx, 1, 2, 0, 0);
+  halo4_xz(c, uzhx, 1, 2, 1, 2);
+  halo4_xz(c, uxhz, 1, 2, 1, 2);
+  halo4_xz(c, uyhz, 0, 0, 1, 2);
+  halo4_xz(c, uzhz, 0, 0, 2, 1);
+
+  /* RhsX, :243-290 */
+  long long* s3k = (long long*)calloc(4 * ((size_t)nzc + 1), sizeof(long long));
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+    long long* s3tot = s3k + 4 * (size_t)kc;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jm = jc - 1, jp = jc + 1;
+      double sucaj = rdyp[jc];
+      double InterpY1 = yint[jc], InterpY2 = 1.0 - InterpY1;
+      double InterpY3 = yint[jp], InterpY4 = 1.0 - InterpY3;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        double uxhyc = InterpY1 * UX(ic, jm, kc) + InterpY2 * UX(ic, jc, kc);
+        double uxhyp = InterpY3 * UX(ic, jc, kc) + InterpY4 * UX(ic, jp, kc);
+        double h11 = dxCoe1 * A(uxhx, is, jc, kc) * (A(uxhx, is, jc, kc) + uCRF) +
+                     dxCoe2 * A(uxhx, im, jc, kc) * (A(uxhx, im, jc, kc) + uCRF) +
+                     dxCoe3 * A(uxhx, ic, jc, kc) * (A(uxhx, ic, jc, kc) + uCRF) +
+                     dxCoe4 * A(uxhx, ip, jc, kc) * (A(uxhx, ip, jc, kc) + uCRF);
+        double h12 = (A(uyhx, ic, jp, kc) * uxhyp - A(uyhx, ic, jc, kc) * uxhyc) * sucaj;
+        double h13 = dzCoe1 * A(uxhz, ic, jc, km) * A(uzhx, ic, jc, km) +
+                     dzCoe2 * A(uxhz, ic, jc, kc) * A(uzhx, ic, jc, kc) +
+                     dzCoe3 * A(uxhz, ic, jc, kp) * A(uzhx, ic, jc, kp) +
+                     dzCoe4 * A(uxhz, ic, jc, ku) * A(uzhx, ic, jc, ku);
+        double d11q1 = dxxCoe1 * UX(is, jc, kc) + dxxCoe2 * UX(im, jc, kc) + dxxCoe3 * UX(ic, jc, kc) +
+                       dxxCoe4 * UX(ip, jc, kc) + dxxCoe5 * UX(iu, jc, kc);
+        double d22q1 = ap2c[jc] * UX(ic, jp, kc) + ac2c[jc] * UX(ic, jc, kc) + am2c[jc] * UX(ic, jm, kc);
+        double d33q1 = dzzCoe1 * UX(ic, jc, ks) + dzzCoe2 * UX(ic, jc, km) + dzzCoe3 * UX(ic, jc, kc) +
+                       dzzCoe4 * UX(ic, jc, kp) + dzzCoe5 * UX(ic, jc, ku);
+        double dcq13 = d11q1 + d33q1;
+        orci_xsum_add(s3tot, (dcq13 + d22q1) * dyp[jc]);
+        double convEd1 = -h11 - h12 - h13 + xnu * dcq13 + c->p.gravity[0];
+        double gradp1 = dxCoe1 * PR(is, jc, kc) + dxCoe2 * PR(im, jc, kc) + dxCoe3 * PR(ic, jc, kc) +
+                        dxCoe4 * PR(ip, jc, kc);
+        size_t q = I4(c, ic, jc, kc);
+        rhsx[q] = pmGamma * convEd1 + pmTheta * histx[q] - pmAlpha * gradp1 + 2.0 * pmBeta * d22q1;
+        histx[q] = convEd1;
+      }
+    }
+  }
+
+  /* RhsY, :293-330 */
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jm = jc - 1, jp = jc + 1;
+      double sucac = rdyc[jc];
+      double qsucac = 0.25 * sucac;
+      double InterpY1 = yint[jc], InterpY2 = 1.0 - InterpY1;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        double uxhym = InterpY1 * UX(im, jm, kc) + InterpY2 * UX(im, jc, kc) + uCRF;
+        double uzhym = InterpY1 * UZ(ic, jm, km) + InterpY2 * UZ(ic, jc, km);
+        double uxhyc = InterpY1 * UX(ic, jm, kc) + InterpY2 * UX(ic, jc, kc) + uCRF;
+        double uzhyc = InterpY1 * UZ(ic, jm, kc) + InterpY2 * UZ(ic, jc, kc);
+        double uxhyp = InterpY1 * UX(ip, jm, kc) + InterpY2 * UX(ip, jc, kc) + uCRF;
+        double uzhyp = InterpY1 * UZ(ic, jm, kp) + InterpY2 * UZ(ic, jc, kp);
+        double uxhyu = InterpY1 * UX(iu, jm, kc) + InterpY2 * UX(iu, jc, kc) + uCRF;
+        double uzhyu = InterpY1 * UZ(ic, jm, ku) + InterpY2 * UZ(ic, jc, ku);
+        double h21 = dxCoe1 * A(uyhx, im, jc, kc) * uxhym + dxCoe2 * A(uyhx, ic, jc, kc) * uxhyc +
+                     dxCoe3 * A(uyhx, ip, jc, kc) * uxhyp + dxCoe4 * A(uyhx, iu, jc, kc) * uxhyu;
+        double h22 = ((UY(ic, jp, kc) + UY(ic, jc, kc)) * (UY(ic, jp, kc) + UY(ic, jc, kc)) -
+                      (UY(ic, jc, kc) + UY(ic, jm, kc)) * (UY(ic, jc, kc) + UY(ic, jm, kc))) * qsucac;
+        double h23 = dzCoe1 * A(uyhz, ic, jc, km) * uzhym + dzCoe2 * A(uyhz, ic, jc, kc) * uzhyc +
+                     dzCoe3 * A(uyhz, ic, jc, kp) * uzhyp + dzCoe4 * A(uyhz, ic, jc, ku) * uzhyu;
+        double d11q2 = dxxCoe1 * UY(is, jc, kc) + dxxCoe2 * UY(im, jc, kc) + dxxCoe3 * UY(ic, jc, kc) +
+                       dxxCoe4 * UY(ip, jc, kc) + dxxCoe5 * UY(iu, jc, kc);
+        double d22q2 = ap2p[jc] * UY(ic, jp, kc) + ac2p[jc] * UY(ic, jc, kc) + am2p[jc] * UY(ic, jm, kc);
+        double d33q2 = dzzCoe1 * UY(ic, jc, ks) + dzzCoe2 * UY(ic, jc, km) + dzzCoe3 * UY(ic, jc, kc) +
+                       dzzCoe4 * UY(ic, jc, kp) + dzzCoe5 * UY(ic, jc, ku);
+        double dcq13 = d11q2 + d33q2;
+        double convEd2 = -h21 - h22 - h23 + xnu * dcq13 + c->p.gravity[1];
+        double gradp2 = (PR(ic, jc, kc) - PR(ic, jm, kc)) * sucac;
+        size_t q = I4(c, ic, jc, kc);
+        rhsy[q] = pmGamma * convEd2 + pmTheta * histy[q] - pmAlpha * gradp2 + 2.0 * pmBeta * d22q2;
+        histy[q] = convEd2;
+      }
+    }
+  }
+
+  /* RhsZ, :333-370 (RhsZ is a halo array) */
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jm = jc - 1, jp = jc + 1;
+      double sucaj = rdyp[jc];
+      double InterpY1 = yint[jc], InterpY2 = 1.0 - InterpY1;
+      double InterpY3 = yint[jp], InterpY4 = 1.0 - InterpY3;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        double uzhyc = InterpY1 * UZ(ic, jm, kc) + InterpY2 * UZ(ic, jc, kc);
+        double uzhyp = InterpY3 * UZ(ic, jc, kc) + InterpY4 * UZ(ic, jp, kc);
+        double h31 = dxCoe1 * A(uzhx, im, jc, kc) * (A(uxhz, im, jc, kc) + uCRF) +
+                     dxCoe2 * A(uzhx, ic, jc, kc) * (A(uxhz, ic, jc, kc) + uCRF) +
+                     dxCoe3 * A(uzhx, ip, jc, kc) * (A(uxhz, ip, jc, kc) + uCRF) +
+                     dxCoe4 * A(uzhx, iu, jc, kc) * (A(uxhz, iu, jc, kc) + uCRF);
+        double h32 = (A(uyhz, ic, jp, kc) * uzhyp - A(uyhz, ic, jc, kc) * uzhyc) * sucaj;
+        double h33 = dzCoe1 * A(uzhz, ic, jc, ks) * A(uzhz, ic, jc, ks) +
+                     dzCoe2 * A(uzhz, ic, jc, km) * A(uzhz, ic, jc, km) +
+                     dzCoe3 * A(uzhz, ic, jc, kc) * A(uzhz, ic, jc, kc) +
+                     dzCoe4 * A(uzhz, ic, jc, kp) * A(uzhz, ic, jc, kp);
+        double d11q3 = dxxCoe1 * UZ(is, jc, kc) + dxxCoe2 * UZ(im, jc, kc) + dxxCoe3 * UZ(ic, jc, kc) +
+                       dxxCoe4 * UZ(ip, jc, kc) + dxxCoe5 * UZ(iu, jc, kc);
+        double d22q3 = ap2c[jc] * UZ(ic, jp, kc) + ac2c[jc] * UZ(ic, jc, kc) + am2c[jc] * UZ(ic, jm, kc);
+        double d33q3 = dzzCoe1 * UZ(ic, jc, ks) + dzzCoe2 * UZ(ic, jc, km) + dzzCoe3 * UZ(ic, jc, kc) +
+                       dzzCoe4 * UZ(ic, jc, kp) + dzzCoe5 * UZ(ic, jc, ku);
+        double dcq13 = d11q3 + d33q3;
+        double convEd3 = -h31 - h32 - h33 + xnu * dcq13 + c->p.gravity[2];
+        double gradp3 = dzCoe1 * PR(ic, jc, ks) + dzCoe2 * PR(ic, jc, km) + dzCoe3 * PR(ic, jc, kc) +
+                        dzCoe4 * PR(ic, jc, kp);
+        size_t q = I4(c, ic, jc, kc);
+        rhsz[H4(c, ic, jc, kc)] = pmGamma * convEd3 + pmTheta * histz[q] - pmAlpha * gradp3 + 2.0 * pmBeta * d22q3;
+        histz[q] = convEd3;
+      }
+    }
+  }
+  /* constant flow rate, :392-407 */
+  if (c->p.is_ux_const) {
+    long long tot[4] = {0, 0, 0, 0};
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int q = 0; q < 4; ++q) tot[q] += s3k[4 * (size_t)kc + q];
+    double s3tot1 = orci_xsum_value(tot);
+    double dpmdxns = xnu * s3tot1 / (double)(nxc * nzc) / c->p.yly;
+    double dp1ns = pmAlpha * dpmdxns;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (size_t q = 0; q < c->nint; ++q) rhsx[q] = rhsx[q] - dp1ns;
+    c->dpmdxns = dpmdxns;
+    c->PrGradAver = c->PrGradAver + dpmdxns * c->pmAlphaC;
+  }
+  free(s3k);
+  free(uxhx); free(uyhx); free(uzhx); free(uxhz); free(uyhz); free(uzhz);
+#undef A
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_TScheme.f90:443-549 clcU1Hat / clcU2Hat / clcU3Hat: Crank-Nicolson in y */
+/* ------------------------------------------------------------------------- */
+static void y_solve_update(orc4_ctx* c, const double* f_in, int f_halo, double* u,
+                           const double* a, const double* b, const double* cc,
+                           const double* row1) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+  {
+    double* arr = (double*)malloc(sizeof(double) * (size_t)(nyc + 1));
+    double* f = (double*)malloc(sizeof(double) * (size_t)nyc);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        for (int jc = 1; jc <= nyc; ++jc)
+          f[jc - 1] = f_halo ? f_in[H4(c, ic, jc, kc)] : f_in[I4(c, ic, jc, kc)];
+        if (row1) f[0] = row1[W4(c, ic, kc)];
+        orci_thomas_line(a, b, cc, f, arr, nyc, 1);
+        for (int jc = 1; jc <= nyc; ++jc) u[H4(c, ic, jc, kc)] = u[H4(c, ic, jc, kc)] + f[jc - 1];
+      }
+    free(arr);
+    free(f);
+  }
+}
+
+static void cn_coefficients(orc4_ctx* c, double* a, double* b, double* cc) {
+  const double *am = c->met[ORC_AM2CCN], *ac = c->met[ORC4_AC2CCN], *ap = c->met[ORC_AP2CCN];
+  for (int jc = 1; jc <= c->nyc; ++jc) {
+    cc[jc - 1] = -c->pmBeta * ap[jc];
+    b[jc - 1] = -c->pmBeta * ac[jc] + 1.0;
+    a[jc - 1] = -c->pmBeta * am[jc];
+  }
+}
+
+void orc4_clcU1Hat(orc4_ctx* c, double* ux, double* rhsx) {
+  const int nyc = c->nyc;
+  double* a = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* b = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* cc = (double*)malloc(sizeof(double) * (size_t)nyc);
+  cn_coefficients(c, a, b, cc);
+  y_solve_update(c, rhsx, 0, ux, a, b, cc, NULL);
+  free(a); free(b); free(cc);
+}
+
+void orc4_clcU2Hat(orc4_ctx* c, double* uy, double* rhsy, const double* duy_ym) {
+  const int nyc = c->nyc;
+  double* a = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* b = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* cc = (double*)malloc(sizeof(double) * (size_t)nyc);
+  const double *am = c->met[ORC_AM2P], *ac = c->met[ORC_AC2P], *ap = c->met[ORC_AP2P];
+  cc[0] = 0.0; b[0] = 1.0; a[0] = 0.0;
+  for (int jc = 2; jc <= nyc; ++jc) {
+    cc[jc - 1] = -c->pmBeta * ap[jc];
+    b[jc - 1] = -c->pmBeta * ac[jc] + 1.0;
+    a[jc - 1] = -c->pmBeta * am[jc];
+  }
+  y_solve_update(c, rhsy, 0, uy, a, b, cc, duy_ym);
+  free(a); free(b); free(cc);
+}
+
+void orc4_clcU3Hat(orc4_ctx* c, double* uz, double* rhsz) {
+  const int nyc = c->nyc;
+  double* a = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* b = (double*)malloc(sizeof(double) * (size_t)nyc);
+  double* cc = (double*)malloc(sizeof(double) * (size_t)nyc);
+  cn_coefficients(c, a, b, cc);
+  y_solve_update(c, rhsz, 1, uz, a, b, cc, NULL);
+  free(a); free(b); free(cc);
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_BC_and_Halo.f90 (4th)                                                   */
+/* ------------------------------------------------------------------------- */
+/* :51-96 */
+void orc4_SetBC_and_UpdateHalo(orc4_ctx* c, double* ux, double* uy, double* uz,
+                               const double* uy_ym) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nyp = c->nyp;
+  const int ypd = 3, ymd = 2;
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int ic = 1; ic <= nxc; ++ic) {
+      if (c->p.flow_type == ORC4_FT_CH) {
+        UX(ic, nyp, kc) = c->p.uxbc[ypd] * 2.0 - UX(ic, nyc, kc);
+        UY(ic, nyp, kc) = c->p.uybc[ypd];
+        UZ(ic, nyp, kc) = c->p.uzbc[ypd] * 2.0 - UZ(ic, nyc, kc);
+      } else {
+        UX(ic, nyp, kc) = UX(ic, nyc, kc);
+        UY(ic, nyp, kc) = 0.0;
+        UZ(ic, nyp, kc) = UZ(ic, nyc, kc);
+      }
+    }
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int ic = 1; ic <= nxc; ++ic) {
+      UX(ic, 0, kc) = c->p.uxbc[ymd] * 2.0 - UX(ic, 1, kc);
+      UY(ic, 1, kc) = uy_ym[W4(c, ic, kc)];
+      UZ(ic, 0, kc) = c->p.uzbc[ymd] * 2.0 - UZ(ic, 1, kc);
+    }
+  halo4_xz(c, ux, 2, 2, 2, 2);
+  halo4_xz(c, uy, 2, 2, 2, 2);
+  halo4_xz(c, uz, 2, 2, 2, 2);
+}
+
+/* :101-134 */
+void orc4_SetBC_and_UpdateHaloForPrSrc(orc4_ctx* c, double* ux, double* uy, double* uz,
+                                       const double* uy_ym) {
+  const int nxc = c->nxc, nzc = c->nzc, nyp = c->nyp;
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int ic = 1; ic <= nxc; ++ic)
+      UY(ic, nyp, kc) = (c->p.flow_type == ORC4_FT_CH) ? c->p.uybc[3] : 0.0;
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int ic = 1; ic <= nxc; ++ic) UY(ic, 1, kc) = uy_ym[W4(c, ic, kc)];
+  halo4_xz(c, ux, 1, 2, 0, 0);
+  halo4_xz(c, uz, 0, 0, 1, 2);
+}
+
+/* :139-153 */
+void orc4_SetBC_and_UpdateHalo_pr(orc4_ctx* c, double* pr) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nyp = c->nyp;
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int ic = 1; ic <= nxc; ++ic) {
+      PR(ic, 0, kc) = PR(ic, 1, kc);
+      PR(ic, nyp, kc) = PR(ic, nyc, kc);
+    }
+  halo4_xz(c, pr, 2, 1, 2, 1);
+}
+
+/* m_TScheme.f90:114-143 */
+void orc4_clcPrSrc(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                   double* prsrc, double* divmax) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double* rdyp = c->met[ORC_RDYP];
+  const double sudtal = 1.0 / c->pmAlpha;
+  const double dxCoe1 = c->dxCoe1, dxCoe2 = c->dxCoe2, dxCoe3 = c->dxCoe3, dxCoe4 = c->dxCoe4;
+  const double dzCoe1 = c->dzCoe1, dzCoe2 = c->dzCoe2, dzCoe3 = c->dzCoe3, dzCoe4 = c->dzCoe4;
+  double* dk = (double*)calloc((size_t)nzc + 1, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int km = kc - 1, kp = kc + 1, ku = kc + 2;
+    double dm = 0.0;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jp = jc + 1;
+      double sucaj = rdyp[jc];
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int im = ic - 1, ip = ic + 1, iu = ic + 2;
+        double dudx = dxCoe1 * UX(im, jc, kc) + dxCoe2 * UX(ic, jc, kc) + dxCoe3 * UX(ip, jc, kc) + dxCoe4 * UX(iu, jc, kc);
+        double dvdy = (UY(ic, jp, kc) - UY(ic, jc, kc)) * sucaj;
+        double dwdz = dzCoe1 * UZ(ic, jc, km) + dzCoe2 * UZ(ic, jc, kc) + dzCoe3 * UZ(ic, jc, kp) + dzCoe4 * UZ(ic, jc, ku);
+        double rdiv = dudx + dvdy + dwdz;
+        dm = fmax(fabs(rdiv), dm);
+        prsrc[I4(c, ic, jc, kc)] = sudtal * rdiv;
+      }
+    }
+    dk[kc] = dm;
+  }
+  double d = 0.0;
+  for (int kc = 1; kc <= nzc; ++kc) d = fmax(d, dk[kc]);
+  free(dk);
+  *divmax = d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_Poisson.f90:371-489 clcPPE_x1 on one rank (4th-order wavenumbers)       */
+/* ------------------------------------------------------------------------- */
+void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const size_t sj = (size_t)nxc, sk = (size_t)nxc * (size_t)nyc;
+  double* arr = prsrc;
+  /* forward x */
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+  {
+    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)(nxc > nzc ? nxc : nzc));
+    double* in = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
+    double* out = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int jc = 1; jc <= nyc; ++jc) {
+        double* row = arr + (size_t)(jc - 1) * sj + (size_t)(kc - 1) * sk;
+        orci_r2hc(&c->px, row, out, wk);
+        memcpy(row, out, sizeof(double) * (size_t)nxc);
+      }
+    /* forward z */
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int jc = 1; jc <= nyc; ++jc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        double* col = arr + (size_t)(ic - 1) + (size_t)(jc - 1) * sj;
+        for (int k = 0; k < nzc; ++k) in[k] = col[(size_t)k * sk];
+        orci_r2hc(&c->pz, in, out, wk);
+        for (int k = 0; k < nzc; ++k) col[(size_t)k * sk] = out[k];
+      }
+    free(wk); free(in); free(out);
+  }
+  /* y solves, :411-464 */
+  const double *am2Pr = c->met[ORC_AM2PR], *ac2Pr = c->met[ORC_AC2PR], *ap2Pr = c->met[ORC_AP2PR];
+  const double *WaveNumX = c->met[ORC_WAVEX], *WaveNumZ = c->met[ORC_WAVEZ];
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+  {
+    double* a = (double*)malloc(sizeof(double) * (size_t)nyc);
+    double* b = (double*)malloc(sizeof(double) * (size_t)nyc);
+    double* cc = (double*)malloc(sizeof(double) * (size_t)nyc);
+    double* w = (double*)malloc(sizeof(double) * (size_t)(nyc + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        double* f = arr + (size_t)(ic - 1) + (size_t)(kc - 1) * sk;
+        for (int jc = 1; jc <= nyc; ++jc) {
+          a[jc - 1] = am2Pr[jc];
+          cc[jc - 1] = ap2Pr[jc];
+          b[jc - 1] = ac2Pr[jc] + WaveNumX[ic] + WaveNumZ[kc];
+        }
+        if (ic == 1 && kc == 1) {
+          cc[0] = 0.0; b[0] = 1.0; a[0] = 0.0; f[0] = 0.0;
+        }
+        orci_thomas_line(a, b, cc, f, w, nyc, sj);
+      }
+    free(a); free(b); free(cc); free(w);
+  }
+  /* inverse z, inverse x */
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+  {
+    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)(nxc > nzc ? nxc : nzc));
+    double* in = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
+    double* out = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int jc = 1; jc <= nyc; ++jc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        double* col = arr + (size_t)(ic - 1) + (size_t)(jc - 1) * sj;
+        for (int k = 0; k < nzc; ++k) in[k] = col[(size_t)k * sk];
+        orci_hc2r(&c->pz, in, out, wk);
+        for (int k = 0; k < nzc; ++k) col[(size_t)k * sk] = out[k];
+      }
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int jc = 1; jc <= nyc; ++jc) {
+        double* row = arr + (size_t)(jc - 1) * sj + (size_t)(kc - 1) * sk;
+        orci_hc2r(&c->px, row, out, wk);
+        memcpy(row, out, sizeof(double) * (size_t)nxc);
+      }
+    free(wk); free(in); free(out);
+  }
+  const double normfft = c->normfft;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int jc = 1; jc <= nyc; ++jc)
+      for (int ic = 1; ic <= nxc; ++ic)
+        prphi[H4(c, ic, jc, kc)] = arr[(size_t)(ic - 1) + (size_t)(jc - 1) * sj + (size_t)(kc - 1) * sk] * normfft;
+}
+
+/* m_TScheme.f90:148-173 */
+void orc4_FluidVelUpdate(orc4_ctx* c, const double* phi, double* ux, double* uy, double* uz) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double* rdyc = c->met[ORC_RDYC];
+  const double dxCoe1 = c->dxCoe1, dxCoe2 = c->dxCoe2, dxCoe3 = c->dxCoe3, dxCoe4 = c->dxCoe4;
+  const double dzCoe1 = c->dzCoe1, dzCoe2 = c->dzCoe2, dzCoe3 = c->dzCoe3, dzCoe4 = c->dzCoe4;
+  const double pmAlpha = c->pmAlpha;
+#define PH(i, j, k) phi[H4(c, i, j, k)]
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int ks = kc - 2, km = kc - 1, kp = kc + 1;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jm = jc - 1;
+      double sucac = rdyc[jc];
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int is = ic - 2, im = ic - 1, ip = ic + 1;
+        double dpdx = dxCoe1 * PH(is, jc, kc) + dxCoe2 * PH(im, jc, kc) + dxCoe3 * PH(ic, jc, kc) + dxCoe4 * PH(ip, jc, kc);
+        double dpdy = (PH(ic, jc, kc) - PH(ic, jm, kc)) * sucac;
+        double dpdz = dzCoe1 * PH(ic, jc, ks) + dzCoe2 * PH(ic, jc, km) + dzCoe3 * PH(ic, jc, kc) + dzCoe4 * PH(ic, jc, kp);
+        UX(ic, jc, kc) = UX(ic, jc, kc) - pmAlpha * dpdx;
+        UY(ic, jc, kc) = UY(ic, jc, kc) - pmAlpha * dpdy;
+        UZ(ic, jc, kc) = UZ(ic, jc, kc) - pmAlpha * dpdz;
+      }
+    }
+  }
+}
+
+/* m_TScheme.f90:590-610 */
+void orc4_PressureUpdate(orc4_ctx* c, double* pr, const double* phi) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double *am2Pr = c->met[ORC_AM2PR], *ac2Pr = c->met[ORC_AC2PR], *ap2Pr = c->met[ORC_AP2PR];
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jp = jc + 1, jm = jc - 1;
+      double pmBetap = -c->pmBeta * ap2Pr[jc];
+      double pmBetac = -c->pmBeta * ac2Pr[jc] + 1.0;
+      double pmBetam = -c->pmBeta * am2Pr[jc];
+      for (int ic = 1; ic <= nxc; ++ic)
+        PR(ic, jc, kc) = PR(ic, jc, kc) + pmBetap * PH(ic, jp, kc) + pmBetac * PH(ic, jc, kc) +
+                         pmBetam * PH(ic, jm, kc);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_Tools.f90 (4th) diagnostics                                             */
+/* ------------------------------------------------------------------------- */
+/* :24-49 */
+double orc4_CalcMaxCFL(orc4_ctx* c, const double* ux, const double* uy, const double* uz) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double* rdyp = c->met[ORC_RDYP];
+  double m = 0.0;
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int jc = 1; jc <= nyc; ++jc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        double vcf = fabs(UX(ic, jc, kc) + UX(ic + 1, jc, kc)) * c->rdx +
+                     fabs(UY(ic, jc, kc) + UY(ic, jc + 1, kc)) * rdyp[jc] +
+                     fabs(UZ(ic, jc, kc) + UZ(ic, jc, kc + 1)) * c->rdz;
+        if (vcf > m) m = vcf;
+      }
+  return m * 0.5;
+}
+
+/* :54-81 */
+double orc4_CheckDivergence(orc4_ctx* c, const double* ux, const double* uy, const double* uz) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double* rdyp = c->met[ORC_RDYP];
+  double m = 0.0;
+  for (int kc = 1; kc <= nzc; ++kc) {
+    int km = kc - 1, kp = kc + 1, ku = kc + 2;
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jp = jc + 1;
+      double sucaj = rdyp[jc];
+      for (int ic = 1; ic <= nxc; ++ic) {
+        int im = ic - 1, ip = ic + 1, iu = ic + 2;
+        double dudx = c->dxCoe1 * UX(im, jc, kc) + c->dxCoe2 * UX(ic, jc, kc) + c->dxCoe3 * UX(ip, jc, kc) + c->dxCoe4 * UX(iu, jc, kc);
+        double dvdy = (UY(ic, jp, kc) - UY(ic, jc, kc)) * sucaj;
+        double dwdz = c->dzCoe1 * UZ(ic, jc, km) + c->dzCoe2 * UZ(ic, jc, kc) + c->dzCoe3 * UZ(ic, jc, kp) + c->dzCoe4 * UZ(ic, jc, ku);
+        double udiv = fabs(dudx + dvdy + dwdz);
+        if (udiv > m) m = udiv;
+      }
+    }
+  }
+  return m;
+}
+
+/* :86-110 */
+void orc4_CalcVmax(orc4_ctx* c, const double* ux, const double* uy, const double* uz, double* vmax3) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int kc = 1; kc <= c->nzc; ++kc)
+    for (int jc = 1; jc <= c->nyc; ++jc)
+      for (int ic = 1; ic <= c->nxc; ++ic) {
+        double a = fabs(UX(ic, jc, kc)), b = fabs(UY(ic, jc, kc)), d = fabs(UZ(ic, jc, kc));
+        if (a > v[0]) v[0] = a;
+        if (b > v[1]) v[1] = b;
+        if (d > v[2]) v[2] = d;
+      }
+  vmax3[0] = v[0]; vmax3[1] = v[1]; vmax3[2] = v[2];
+}
+
+/* :256-275 (fixed-point sum in place of the sequential one) */
+double orc4_CalcUxAver(orc4_ctx* c, const double* ux) {
+  const double* dyp = c->met[ORC_DYP];
+  long long acc[4] = {0, 0, 0, 0};
+  for (int kc = 1; kc <= c->nzc; ++kc)
+    for (int jc = 1; jc <= c->nyc; ++jc)
+      for (int ic = 1; ic <= c->nxc; ++ic) orci_xsum_add(acc, UX(ic, jc, kc) * dyp[jc]);
+  return orci_xsum_value(acc) / (double)(c->nxc * c->nzc) / c->p.yly;
+}
+
+/* ------------------------------------------------------------------------- */
+/* m_ChannelSystem.f90:83-149 ChannelIterate                                 */
+/* ------------------------------------------------------------------------- */
+void orc4_substep(orc4_ctx* c, int ns) {
+  double *ux = c->f[ORC_UX], *uy = c->f[ORC_UY], *uz = c->f[ORC_UZ], *pr = c->f[ORC_PR];
+  double *arr1 = c->f[ORC_ARR1], *arr2 = c->f[ORC_ARR2], *halo = c->f[ORC_HALO];
+  orc4_PMcoeUpdate(c, ns);
+  orc4_Update_uy_ym(c, c->f[ORC_UY_YM], c->f[ORC_DUY_YM]);
+  orc4_clcRhsXYZ(c, ux, uy, uz, arr1, arr2, halo, c->f[ORC_HISTX], c->f[ORC_HISTY],
+                 c->f[ORC_HISTZ], pr);
+  orc4_clcU1Hat(c, ux, arr1);
+  orc4_clcU2Hat(c, uy, arr2, c->f[ORC_DUY_YM]);
+  orc4_clcU3Hat(c, uz, halo);
+  orc4_SetBC_and_UpdateHaloForPrSrc(c, ux, uy, uz, c->f[ORC_UY_YM]);
+  orc4_clcPrSrc(c, ux, uy, uz, arr1, &c->divmax1);
+  orc4_clcPPE(c, arr1, halo);
+  orc4_SetBC_and_UpdateHalo_pr(c, halo);
+  orc4_FluidVelUpdate(c, halo, ux, uy, uz);
+  orc4_PressureUpdate(c, pr, halo);
+  orc4_SetBC_and_UpdateHalo(c, ux, uy, uz, c->f[ORC_UY_YM]);
+  orc4_SetBC_and_UpdateHalo_pr(c, pr);
+}
+
+void orc4_iterate(orc4_ctx* c) {
+  double *ux = c->f[ORC_UX], *uy = c->f[ORC_UY], *uz = c->f[ORC_UZ];
+  c->cfl = orc4_CalcMaxCFL(c, ux, uy, uz) * c->dt;
+  for (int ns = 1; ns <= c->iadvance; ++ns) orc4_substep(c, ns);
+  c->divmax2 = orc4_CheckDivergence(c, ux, uy, uz);
+  orc4_CalcVmax(c, ux, uy, uz, c->vmax);
+  c->uxaver = orc4_CalcUxAver(c, ux);
+}
+
+void orc4_project(orc4_ctx* c) {
+  double *ux = c->f[ORC_UX], *uy = c->f[ORC_UY], *uz = c->f[ORC_UZ];
+  double save_alpha = c->pmAlpha;
+  orc4_SetBC_and_UpdateHalo(c, ux, uy, uz, c->f[ORC_UY_YM]);
+  c->pmAlpha = 1.0;
+  double dm;
+  orc4_clcPrSrc(c, ux, uy, uz, c->f[ORC_ARR1], &dm);
+  orc4_clcPPE(c, c->f[ORC_ARR1], c->f[ORC_HALO]);
+  orc4_SetBC_and_UpdateHalo_pr(c, c->f[ORC_HALO]);
+  orc4_FluidVelUpdate(c, c->f[ORC_HALO], ux, uy, uz);
+  orc4_SetBC_and_UpdateHalo(c, ux, uy, uz, c->f[ORC_UY_YM]);
+  c->pmAlpha = save_alpha;
+}

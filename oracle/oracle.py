@@ -38,6 +38,10 @@ WALL_FIELDS = (UY_YM, DUY_YM)
  S_PRGRADAVER, S_NORMFFT, S_DIVMAX1, S_DIVMAX2, S_CFL, S_UXAVER, S_VMAX1, S_VMAX2,
  S_VMAX3, S_DT) = range(17)
 
+# 4th-order oracle (c3d_oracle4.h)
+M4_AC2CCN = 20
+FT_CH, FT_HC = 1, 2
+
 
 class OrcParams(ctypes.Structure):
     _fields_ = [
@@ -50,6 +54,21 @@ class OrcParams(ctypes.Structure):
         ("gravity", ctypes.c_double * 3),
         ("is_ux_const", ctypes.c_int),
         ("ubulk", ctypes.c_double), ("ybulk1", ctypes.c_double), ("ybulk2", ctypes.c_double),
+        ("ischeme", ctypes.c_int),
+    ]
+
+
+class Orc4Params(ctypes.Structure):
+    _fields_ = [
+        ("nxc", ctypes.c_int), ("nyc", ctypes.c_int), ("nzc", ctypes.c_int),
+        ("xlx", ctypes.c_double), ("yly", ctypes.c_double), ("zlz", ctypes.c_double),
+        ("xnu", ctypes.c_double), ("dt", ctypes.c_double),
+        ("flow_type", ctypes.c_int),
+        ("uxbc", ctypes.c_double * 6), ("uybc", ctypes.c_double * 6),
+        ("uzbc", ctypes.c_double * 6),
+        ("gravity", ctypes.c_double * 3),
+        ("is_ux_const", ctypes.c_int),
+        ("u_crf", ctypes.c_double),
         ("ischeme", ctypes.c_int),
     ]
 
