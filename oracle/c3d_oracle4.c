@@ -49,7 +49,7 @@ struct orc4_ctx {
 };
 
 /* halo (-1:nxc+2, 0:nyc+1, -1:nzc+2), interior (1:nxc,1:nyc,1:nzc), wall (1:nxc,1:nzc) */
-#define H4(c, i, j, k) ((size_t)((i) + 2) + (c)->hx * ((size_t)(j) + (c)->hy * (size_t)((k) + 2)))
+#define H4(c, i, j, k) ((size_t)((i) + 1) + (c)->hx * ((size_t)(j) + (c)->hy * (size_t)((k) + 1)))
 #define I4(c, i, j, k) \
   ((size_t)((i)-1) + (size_t)(c)->nxc * ((size_t)((j)-1) + (size_t)(c)->nyc * (size_t)((k)-1)))
 #define W4(c, i, k) ((size_t)((i)-1) + (size_t)(c)->nxc * (size_t)((k)-1))

@@ -138,6 +138,39 @@ def lib():
         L.orc_xsum.argtypes = [ctypes.c_int, _DP]
         L.orc_r2hc.argtypes = [ctypes.c_int, _DP, _DP]
         L.orc_hc2r.argtypes = [ctypes.c_int, _DP, _DP]
+        # 4th-order restatement
+        L.orc4_create.restype = ctypes.c_void_p
+        L.orc4_create.argtypes = [ctypes.POINTER(Orc4Params), _DP]
+        L.orc4_destroy.argtypes = [ctypes.c_void_p]
+        L.orc4_field.restype = _DP
+        L.orc4_field.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc4_metric.restype = _DP
+        L.orc4_metric.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc4_scalar.restype = ctypes.c_double
+        L.orc4_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc4_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc4_substep.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc4_iterate.argtypes = [ctypes.c_void_p]
+        L.orc4_project.argtypes = [ctypes.c_void_p]
+        L.orc4_PMcoeUpdate.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc4_Update_uy_ym.argtypes = [ctypes.c_void_p, _DP, _DP]
+        L.orc4_clcRhsXYZ.argtypes = [ctypes.c_void_p] + [_DP] * 10
+        L.orc4_clcU1Hat.argtypes = [ctypes.c_void_p, _DP, _DP]
+        L.orc4_clcU2Hat.argtypes = [ctypes.c_void_p, _DP, _DP, _DP]
+        L.orc4_clcU3Hat.argtypes = [ctypes.c_void_p, _DP, _DP]
+        L.orc4_SetBC_and_UpdateHalo.argtypes = [ctypes.c_void_p] + [_DP] * 4
+        L.orc4_SetBC_and_UpdateHaloForPrSrc.argtypes = [ctypes.c_void_p] + [_DP] * 4
+        L.orc4_SetBC_and_UpdateHalo_pr.argtypes = [ctypes.c_void_p, _DP]
+        L.orc4_clcPrSrc.argtypes = [ctypes.c_void_p] + [_DP] * 5
+        L.orc4_clcPPE.argtypes = [ctypes.c_void_p, _DP, _DP]
+        L.orc4_FluidVelUpdate.argtypes = [ctypes.c_void_p] + [_DP] * 4
+        L.orc4_PressureUpdate.argtypes = [ctypes.c_void_p, _DP, _DP]
+        for name in ("orc4_CalcMaxCFL", "orc4_CheckDivergence"):
+            getattr(L, name).restype = ctypes.c_double
+            getattr(L, name).argtypes = [ctypes.c_void_p] + [_DP] * 3
+        L.orc4_CalcVmax.argtypes = [ctypes.c_void_p] + [_DP] * 4
+        L.orc4_CalcUxAver.restype = ctypes.c_double
+        L.orc4_CalcUxAver.argtypes = [ctypes.c_void_p, _DP]
         _lib = L
     return _lib
 
@@ -305,3 +338,83 @@ class Oracle:
     def f(self, fid):
         """raw pointer argument for a context field (for .call)"""
         return self.field(fid)
+
+
+# ------------------------------------------------------- 4th-order context
+class Oracle4:
+    """The 4th-order channel step (src/CFD_4th).  Halo fields have shape
+    (nxc+4, nyc+2, nzc+4) and view[ic+1, jc, kc+1] == ux(ic, jc, kc) for the
+    reference's halo index range (-1:nxc+2, 0:nyc+1, -1:nzc+2)."""
+
+    def __init__(self, case: dict, yp=None):
+        self.case = dict(case)
+        p = Orc4Params()
+        p.nxc, p.nyc, p.nzc = case["nxc"], case["nyc"], case["nzc"]
+        p.xlx, p.yly, p.zlz = case["xlx"], case["yly"], case["zlz"]
+        p.xnu, p.dt = case["xnu"], case["dt"]
+        p.flow_type = case.get("flow_type", FT_CH)
+        for name in ("uxbc", "uybc", "uzbc"):
+            vals = case.get(name, (0.0,) * 6)
+            arr = getattr(p, name)
+            for d in range(6):
+                arr[d] = vals[d]
+        for d, g in enumerate(case.get("gravity", (0.0, 0.0, 0.0))):
+            p.gravity[d] = g
+        p.is_ux_const = int(case.get("is_ux_const", True))
+        p.u_crf = case.get("u_crf", 0.0)
+        p.ischeme = case.get("ischeme", 3)
+        self.params = p
+        if yp is None:
+            yp = channel_yp(p.nyc, p.yly, case.get("gamma", 1.5))
+        self.yp = np.ascontiguousarray(yp, dtype=np.float64)
+        h = lib().orc4_create(ctypes.byref(p), _ptr(self.yp))
+        if not h:
+            raise ValueError("orc4_create rejected the parameters")
+        self._h = ctypes.c_void_p(h)
+        self.nxc, self.nyc, self.nzc = p.nxc, p.nyc, p.nzc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orc4_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def shape(self, fid):
+        if fid in HALO_FIELDS:
+            return (self.nxc + 4, self.nyc + 2, self.nzc + 4)
+        if fid in INTERIOR_FIELDS:
+            return (self.nxc, self.nyc, self.nzc)
+        return (self.nxc, self.nzc)
+
+    def field(self, fid) -> np.ndarray:
+        shp = self.shape(fid)
+        ptr = lib().orc4_field(self._h, fid)
+        buf = np.ctypeslib.as_array(ptr, shape=(int(np.prod(shp)),))
+        return buf.reshape(shp, order="F")
+
+    def metric(self, mid, lo, hi) -> np.ndarray:
+        ptr = lib().orc4_metric(self._h, mid)
+        base = ctypes.cast(ctypes.addressof(ptr.contents) + 8 * lo, _DP)
+        return np.ctypeslib.as_array(base, shape=(hi - lo + 1,)).copy()
+
+    def scalar(self, sid) -> float:
+        return float(lib().orc4_scalar(self._h, sid))
+
+    def set_scalar(self, sid, v):
+        lib().orc4_set_scalar(self._h, sid, float(v))
+
+    def substep(self, ns):
+        lib().orc4_substep(self._h, int(ns))
+
+    def iterate(self):
+        lib().orc4_iterate(self._h)
+
+    def project(self):
+        lib().orc4_project(self._h)
+
+    def call(self, name, *arrays):
+        fn = getattr(lib(), "orc4_" + name)
+        args = [self._h] + [_ptr(a) if isinstance(a, np.ndarray) else a for a in arrays]
+        return fn(*args)
