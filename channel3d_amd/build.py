@@ -45,6 +45,7 @@ CXXFLAGS = [
 # the predicted velocity reappear amplified in the pressure.
 PER_FILE = {
     "k_stencil.hip": ["-ffp-contract=off"],
+    "k_stencil4.hip": ["-ffp-contract=off"],
     "k_adi.hip": ["-ffp-contract=off"],
     "k_ppe.hip": ["-ffp-contract=off"],
 }

@@ -6,6 +6,8 @@ m_Poisson.f90:16, m_ChannelSystem.f90:91-130); arrays move in and out in
 the reference's Fortran layouts:
 
   halo fields      (nxc+3, nyc+3, nzc+3), view[ic, jc, kc] == ux(ic, jc, kc)
+                   4th order (case["order"] == 4, src/CFD_4th):
+                   (nxc+4, nyc+2, nzc+4), view[ic+1, jc, kc+1] == ux(ic, jc, kc)
   interior fields  (nxc, nyc, nzc),       view[ic-1, jc-1, kc-1]
   wall planes      (nxc, nzc)
 
@@ -66,6 +68,10 @@ def make_params(case: dict, precision: int = L.FP64) -> L.Params:
     p.ybulk2 = float(case.get("ybulk2", case["yly"]))
     p.ischeme = int(case.get("ischeme", 3))
     p.precision = int(precision)
+    p.order = int(case.get("order", 2))
+    p.u_crf = float(case.get("u_crf", 0.0))
+    if p.order == 4 and "bc" not in case and case.get("flow_type", 1) == 2:
+        p.bc[3] = L.BC_FREESLIP        # half channel (FT_HC)
     return p
 
 
@@ -82,6 +88,7 @@ class Channel:
                                     ctypes.byref(h)), "c3d_create")
         self._h = h
         self.nxc, self.nyc, self.nzc = self.params.nxc, self.params.nyc, self.params.nzc
+        self.order = self.params.order
         self._state = {}
         for fid in range(12):
             v = ctypes.c_void_p()
@@ -109,6 +116,8 @@ class Channel:
     # fields ------------------------------------------------------------------
     def shape(self, layout):
         if layout == L.HALO:
+            if self.order == 4:
+                return (self.nxc + 4, self.nyc + 2, self.nzc + 4)
             return (self.nxc + 3, self.nyc + 3, self.nzc + 3)
         if layout == L.INTERIOR:
             return (self.nxc, self.nyc, self.nzc)
@@ -178,6 +187,11 @@ class Channel:
     def clcRhsZ(self, ux, uy, uz, RhsZ, HistZOld, pressure):
         self._call("clcRhsZ", ux, uy, uz, RhsZ, HistZOld, pressure)
 
+    def clcRhsXYZ(self, ux, uy, uz, RhsX, RhsY, RhsZ, HistXOld, HistYOld, HistZOld, pressure):
+        """4th-order contexts (src/CFD_4th/m_TScheme.f90:189-438)"""
+        self._call("clcRhsXYZ", ux, uy, uz, RhsX, RhsY, RhsZ, HistXOld, HistYOld, HistZOld,
+                   pressure)
+
     def clcU1Hat(self, ux, RhsX):
         self._call("clcU1Hat", ux, RhsX)
 
@@ -196,10 +210,12 @@ class Channel:
     def SetBC_and_UpdateHalo(self, ux, uy, uz, uy_ym):
         self._call("SetBC_and_UpdateHalo", ux, uy, uz, uy_ym)
 
-    def clcPrSrc(self, ux, uy, uz, prsrc, pressure):
+    def clcPrSrc(self, ux, uy, uz, prsrc, pressure=None):
+        """pressure is corrected by -nu/2 div on the 2nd-order path; the
+        4th-order routine has no pressure argument (pass None)"""
         out = ctypes.c_double()
-        L.check(self.lib.c3d_clcPrSrc(self._h, *[self.ptr(f) for f in (ux, uy, uz, prsrc,
-                                                                        pressure)],
+        pr = None if pressure is None else self.ptr(pressure)
+        L.check(self.lib.c3d_clcPrSrc(self._h, *[self.ptr(f) for f in (ux, uy, uz, prsrc)], pr,
                                       ctypes.byref(out)), "c3d_clcPrSrc")
         return out.value
 

@@ -231,6 +231,10 @@ static void compute_metrics(c3d_ctx* c, const double* yp_in) {
   if (c->p.bc[3] == C3D_BC_NOSLIP) {
     am2c[ny] = 4.0 * rdyc[ny] / (dyc[nyp] + 2.0 * dyc[ny]);
     ap2c[ny] = 4.0 * rdyc[nyp] / (dyc[nyp] + 2.0 * dyc[ny]);
+  } else if (c->order == 4) {
+    // half channel, src/CFD_4th/m_MeshAndMetries.f90:238-241
+    am2c[ny] = rdyp[ny] * rdyc[ny];
+    ap2c[ny] = 0.0;
   }
   for (int j = 1; j <= ny; ++j) ac2c[j] = -(am2c[j] + ap2c[j]);
   for (int j = 1; j <= ny; ++j) {
@@ -238,18 +242,55 @@ static void compute_metrics(c3d_ctx* c, const double* yp_in) {
     ap2p[j] = rdyc[j] * rdyp[j];
     ac2p[j] = -(am2p[j] + ap2p[j]);
   }
+  auto& accn = c->met[C3D_M_AC2CCN];
   for (int j = 1; j <= ny; ++j) {
     amcn[j] = am2c[j];
     apcn[j] = ap2c[j];
+    accn[j] = ac2c[j];
   }
-  amcn[1] = c->p.bc[2] == C3D_BC_NOSLIP ? 2.0 * am2c[1] : 0.0;
-  apcn[ny] = c->p.bc[3] == C3D_BC_NOSLIP ? 2.0 * ap2c[ny] : 0.0;
-  // modified wavenumbers for the periodic x and z (indices kx = i-1, kz = k-1)
   const double twoPi = 6.283185307179586476925286766559005768;
-  for (int kx = 0; kx < nx; ++kx)
-    c->met[C3D_M_WAVEX][kx] = 2.0 * c->rdx2 * (std::cos(twoPi * (double)kx / (double)nx) - 1.0);
-  for (int kz = 0; kz < nz; ++kz)
-    c->met[C3D_M_WAVEZ][kz] = 2.0 * c->rdz2 * (std::cos(twoPi * (double)kz / (double)nz) - 1.0);
+  if (c->order == 4) {
+    // src/CFD_4th/m_TScheme.f90:41-51: the wall ghosts folded into row 1 and
+    // (channel) row nyc; the diagonal keeps its own entry
+    apcn[1] = ap2c[1];
+    accn[1] = ac2c[1] - am2c[1];
+    amcn[1] = 2.0 * am2c[1];
+    if (c->p.bc[3] == C3D_BC_NOSLIP) {
+      amcn[ny] = am2c[ny];
+      accn[ny] = ac2c[ny] - ap2c[ny];
+      apcn[ny] = 2.0 * ap2c[ny];
+    }
+    // src/CFD_4th/m_Poisson.f90:40-47: Fourier symbol of the 4th-order div(grad)
+    for (int kx = 0; kx < nx; ++kx) {
+      const double wa1 = twoPi * (double)kx / (double)nx;
+      c->met[C3D_M_WAVEX][kx] = c->rdx2 / 288.0 * (std::cos(3.0 * wa1) - 54.0 * std::cos(2.0 * wa1) +
+                                                   783.0 * std::cos(wa1) - 730.0);
+    }
+    for (int kz = 0; kz < nz; ++kz) {
+      const double wa3 = twoPi * (double)kz / (double)nz;
+      c->met[C3D_M_WAVEZ][kz] = c->rdz2 / 288.0 * (std::cos(3.0 * wa3) - 54.0 * std::cos(2.0 * wa3) +
+                                                   783.0 * std::cos(wa3) - 730.0);
+    }
+    // src/CFD_4th/m_MeshAndMetries.f90:87-91
+    const double rdx = c->rdx, rdz = c->rdz, rdx2 = c->rdx2, rdz2 = c->rdz2;
+    c->dxc[0] = rdx / 24.0; c->dxc[1] = -rdx * 1.125; c->dxc[2] = rdx * 1.125; c->dxc[3] = -rdx / 24.0;
+    c->dzc[0] = rdz / 24.0; c->dzc[1] = -rdz * 1.125; c->dzc[2] = rdz * 1.125; c->dzc[3] = -rdz / 24.0;
+    c->dxxc[0] = -rdx2 / 12.0; c->dxxc[1] = 4.0 / 3.0 * rdx2; c->dxxc[2] = -2.5 * rdx2;
+    c->dxxc[3] = 4.0 / 3.0 * rdx2; c->dxxc[4] = -rdx2 / 12.0;
+    c->dzzc[0] = -rdz2 / 12.0; c->dzzc[1] = 4.0 / 3.0 * rdz2; c->dzzc[2] = -2.5 * rdz2;
+    c->dzzc[3] = 4.0 / 3.0 * rdz2; c->dzzc[4] = -rdz2 / 12.0;
+    c->icoe[0] = -1.0 / 16.0; c->icoe[1] = 9.0 / 16.0; c->icoe[2] = 9.0 / 16.0; c->icoe[3] = -1.0 / 16.0;
+  } else {
+    // m_TScheme.f90:73-85
+    amcn[1] = c->p.bc[2] == C3D_BC_NOSLIP ? 2.0 * am2c[1] : 0.0;
+    apcn[ny] = c->p.bc[3] == C3D_BC_NOSLIP ? 2.0 * ap2c[ny] : 0.0;
+    for (int j = 1; j <= ny; ++j) accn[j] = -(amcn[j] + apcn[j]);
+    // modified wavenumbers for the periodic x and z (indices kx = i-1, kz = k-1)
+    for (int kx = 0; kx < nx; ++kx)
+      c->met[C3D_M_WAVEX][kx] = 2.0 * c->rdx2 * (std::cos(twoPi * (double)kx / (double)nx) - 1.0);
+    for (int kz = 0; kz < nz; ++kz)
+      c->met[C3D_M_WAVEZ][kz] = 2.0 * c->rdz2 * (std::cos(twoPi * (double)kz / (double)nz) - 1.0);
+  }
   double normfft = 1.0;
   normfft = normfft * 1.0 * (double)nx;
   normfft = normfft * 1.0 * (double)nz;
@@ -332,7 +373,8 @@ static int build_adi_tables(c3d_ctx* c) {
     for (int j = 1; j <= ny; ++j) {
       cc[j] = -beta * c->met[C3D_M_AP2CCN][j];
       a[j] = -beta * c->met[C3D_M_AM2CCN][j];
-      b[j] = -cc[j] - a[j] + 1.0;
+      // 4th order: src/CFD_4th/m_TScheme.f90:453-457 (own diagonal)
+      b[j] = c->order == 4 ? -beta * c->met[C3D_M_AC2CCN][j] + 1.0 : -cc[j] - a[j] + 1.0;
     }
     thomas_factors(a, b, cc, ny, vecm, arr);
     double* yb = hb + nx + nz;
@@ -487,13 +529,24 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
     set_error("c3d_create: bad time-scheme parameters");
     return C3D_ERR_ARG;
   }
-  if (p->is_ux_const && !(p->ybulk2 > p->ybulk1)) {
+  const int order = p->order == 0 ? 2 : p->order;
+  if (order != 2 && order != 4) {
+    set_error("c3d_create: order must be 2 or 4");
+    return C3D_ERR_ARG;
+  }
+  if (order == 2 && p->is_ux_const && !(p->ybulk2 > p->ybulk1)) {
     set_error("c3d_create: ybulk1 OR ybulk2 wrong");
+    return C3D_ERR_ARG;
+  }
+  if (order == 4 && p->bc[2] != C3D_BC_NOSLIP) {
+    set_error("c3d_create: the 4th-order path has a no-slip bottom wall (channel or half channel)");
     return C3D_ERR_ARG;
   }
   c3d_ctx* c = new (std::nothrow) c3d_ctx();
   if (!c) return C3D_ERR_NOMEM;
   c->p = *p;
+  c->order = order;
+  c->p.order = order;
   c->prec = p->precision;
   c->esize = c->prec == C3D_FP64 ? 8 : 4;
   Geom& g = c->g;
@@ -710,6 +763,21 @@ static void pack(const c3d_ctx* c, int layout, const double* host, T* dev_img) {
     for (size_t q = 0; q < (size_t)nx * nz; ++q) dev_img[q] = (T)host[q];
     return;
   }
+  if (layout == C3D_HALO && c->order == 4) {
+    // (-1:nxc+2, 0:nyc+1, -1:nzc+2); the unused plane jj = ny+2 is zeroed
+    const size_t hx = (size_t)nx + 4, hy = (size_t)ny + 2;
+    for (int k = 0; k < nz; ++k)
+      for (int jj = 0; jj < g.nyh; ++jj) {
+        T* drow = dev_img + (size_t)fidx(g, 0, jj, k);
+        if (jj <= ny + 1) {
+          const double* hrow = host + 2 + hx * ((size_t)jj + hy * (size_t)(k + 2));
+          for (int i = 0; i < nx; ++i) drow[i] = (T)hrow[i];
+        } else {
+          for (int i = 0; i < nx; ++i) drow[i] = (T)0;
+        }
+      }
+    return;
+  }
   const size_t hx = (size_t)nx + 3, hy = (size_t)ny + 3;
   for (int k = 0; k < nz; ++k)
     for (int jj = 0; jj < g.nyh; ++jj) {
@@ -741,6 +809,19 @@ static void unpack(const c3d_ctx* c, int layout, const T* dev_img, double* host)
         double* hrow = host + (size_t)nx * ((size_t)(jj - 1) + (size_t)ny * k);
         for (int i = 0; i < nx; ++i) hrow[i] = (double)drow[i];
       }
+    return;
+  }
+  if (c->order == 4) {
+    // two periodic layers on each x/z side, y ghosts 0 and ny+1
+    const size_t hx = (size_t)nx + 4, hy = (size_t)ny + 2;
+    for (int kc = -1; kc <= nz + 2; ++kc) {
+      const int k = (kc - 1 + nz) % nz;
+      for (int jj = 0; jj <= ny + 1; ++jj) {
+        const T* drow = dev_img + (size_t)fidx(g, 0, jj, k);
+        double* hrow = host + hx * ((size_t)jj + hy * (size_t)(kc + 1));
+        for (int ic = -1; ic <= nx + 2; ++ic) hrow[ic + 1] = (double)drow[(ic - 1 + nx) % nx];
+      }
+    }
     return;
   }
   // halo: interior x/z range plus the periodic halo layers the reference keeps
@@ -822,9 +903,17 @@ extern "C" int c3d_Update_uy_ym(c3d_ctx* c, void* uy_ym, void* duy_ym) {
   return launch_update_uy_ym(c, uy_ym, duy_ym);
 }
 
+static int only_order2(c3d_ctx* c, const char* what) {
+  if (c->order == 2) return C3D_OK;
+  set_error(std::string(what) + ": a 4th-order context computes the three right-hand sides "
+            "together (c3d_clcRhsXYZ)");
+  return C3D_ERR_ARG;
+}
+
 extern "C" int c3d_clcRhsX(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* rx,
                            void* hx, const void* pr) {
   CHECK_CTX(c);
+  if (int e = only_order2(c, "c3d_clcRhsX")) return e;
   int rc = launch_rhs(c, 1, ux, uy, uz, pr, rx, nullptr, nullptr, hx, nullptr, nullptr, 1);
   if (rc) return rc;
   if (c->p.is_ux_const) return launch_add_dpmdxns(c, rx);
@@ -834,13 +923,29 @@ extern "C" int c3d_clcRhsX(c3d_ctx* c, const void* ux, const void* uy, const voi
 extern "C" int c3d_clcRhsY(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* ry,
                            void* hy, const void* pr) {
   CHECK_CTX(c);
+  if (int e = only_order2(c, "c3d_clcRhsY")) return e;
   return launch_rhs(c, 2, ux, uy, uz, pr, nullptr, ry, nullptr, nullptr, hy, nullptr, 0);
 }
 
 extern "C" int c3d_clcRhsZ(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* rz,
                            void* hz, const void* pr) {
   CHECK_CTX(c);
+  if (int e = only_order2(c, "c3d_clcRhsZ")) return e;
   return launch_rhs(c, 4, ux, uy, uz, pr, nullptr, nullptr, rz, nullptr, nullptr, hz, 0);
+}
+
+extern "C" int c3d_clcRhsXYZ(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                             void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
+                             const void* pr) {
+  CHECK_CTX(c);
+  if (c->order != 4) {
+    set_error("c3d_clcRhsXYZ: the 2nd-order path uses c3d_clcRhsX/Y/Z");
+    return C3D_ERR_ARG;
+  }
+  const int flow = c->p.is_ux_const != 0;
+  int rc = launch_rhs4(c, ux, uy, uz, pr, rx, ry, rz, hx, hy, hz, flow);
+  if (rc) return rc;
+  return flow ? launch_sub_dp1ns(c, rx) : C3D_OK;
 }
 
 extern "C" int c3d_clcU1Hat(c3d_ctx* c, void* ux, void* rx) {
@@ -861,7 +966,8 @@ extern "C" int c3d_clcU3Hat(c3d_ctx* c, void* uz, void* rz) {
 extern "C" int c3d_SetBC_and_UpdateHaloForPrSrc(c3d_ctx* c, void* ux, void* uy, void* uz,
                                                 const void* uy_ym) {
   CHECK_CTX(c);
-  return launch_bc_vel(c, ux, uy, uz, uy_ym, 0);
+  return c->order == 4 ? launch_bc4_vel(c, ux, uy, uz, uy_ym, 0)
+                       : launch_bc_vel(c, ux, uy, uz, uy_ym, 0);
 }
 
 extern "C" int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* c, void* pr) {
@@ -872,7 +978,8 @@ extern "C" int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* c, void* pr) {
 extern "C" int c3d_SetBC_and_UpdateHalo(c3d_ctx* c, void* ux, void* uy, void* uz,
                                         const void* uy_ym) {
   CHECK_CTX(c);
-  return launch_bc_vel(c, ux, uy, uz, uy_ym, 1);
+  return c->order == 4 ? launch_bc4_vel(c, ux, uy, uz, uy_ym, 1)
+                       : launch_bc_vel(c, ux, uy, uz, uy_ym, 1);
 }
 
 static int read_scalars(c3d_ctx* c) {
@@ -884,7 +991,8 @@ static int read_scalars(c3d_ctx* c) {
 extern "C" int c3d_clcPrSrc(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
                             void* prsrc, void* pr, double* divmax) {
   CHECK_CTX(c);
-  int rc = launch_prsrc(c, ux, uy, uz, prsrc, pr, 0.0);
+  int rc = c->order == 4 ? launch_prsrc4(c, ux, uy, uz, prsrc, 0.0)
+                         : launch_prsrc(c, ux, uy, uz, prsrc, pr, 0.0);
   if (rc) return rc;
   if (divmax) {
     if ((rc = read_scalars(c))) return rc;
@@ -905,11 +1013,13 @@ extern "C" int c3d_ppe(c3d_ctx* c, void* prsrc, void* phi) {
 
 extern "C" int c3d_FluidVelUpdate(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz) {
   CHECK_CTX(c);
+  if (c->order == 4) return launch_proj4(c, phi, ux, uy, uz, nullptr, 1, 0, 0);
   return launch_proj(c, phi, ux, uy, uz, nullptr, 1, 0, 0);
 }
 
 extern "C" int c3d_PressureUpdate(c3d_ctx* c, void* pr, const void* phi) {
   CHECK_CTX(c);
+  if (c->order == 4) return launch_proj4(c, phi, nullptr, nullptr, nullptr, pr, 0, 1, 0);
   return launch_proj(c, phi, nullptr, nullptr, nullptr, pr, 0, 1, 0);
 }
 
@@ -934,7 +1044,8 @@ extern "C" int c3d_CheckDivergence(c3d_ctx* c, const void* ux, const void* uy, c
                                    double* divmax) {
   CHECK_CTX(c);
   if (!divmax) return C3D_ERR_ARG;
-  int rc = diag_read(c, ux, uy, uz, 2);
+  int rc = c->order == 4 ? launch_div4(c, ux, uy, uz) : launch_diag(c, ux, uy, uz, 2, 0);
+  if (!rc) rc = read_scalars(c);
   if (rc) return rc;
   *divmax = c->h_sc.divmax2;
   return C3D_OK;
@@ -962,8 +1073,37 @@ extern "C" int c3d_CalcUxAver(c3d_ctx* c, const void* ux, double* uxaver) {
 // ===========================================================================
 // fused, device-resident step (m_ChannelSystem.f90:73-130)
 // ===========================================================================
+// one substep of the 4th-order ChannelIterate (src/CFD_4th/m_ChannelSystem.f90:105-143)
+static int substep4(c3d_ctx* c, int ns) {
+  int rc;
+  void** s = c->state;
+  if ((rc = c3d_PMcoeUpdate(c, ns))) return rc;
+  if ((rc = launch_update_uy_ym(c, s[C3D_UY_YM], s[C3D_DUY_YM]))) return rc;
+  const int flow = c->p.is_ux_const != 0;
+  // clcRhsXYZ: RhsX -> RealArr1, RhsY -> RealArr2, RhsZ -> RealHalo (:111-117)
+  if ((rc = launch_rhs4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], s[C3D_ARR1], s[C3D_ARR2],
+                        s[C3D_HALOARR], s[C3D_HISTX], s[C3D_HISTY], s[C3D_HISTZ], flow)))
+    return rc;
+  // clcU1Hat..U3Hat (:120-122); RhsX's flow-rate shift is applied as the y
+  // sweep loads it
+  if ((rc = launch_adi(c, 0, ns, s[C3D_UX], s[C3D_ARR1], nullptr, flow))) return rc;
+  if ((rc = launch_adi(c, 1, ns, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
+  if ((rc = launch_adi(c, 2, ns, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
+  // :126-130
+  if ((rc = launch_bc4_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
+  if ((rc = launch_prsrc4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], 0.0))) return rc;
+  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  // :133-136, the Neumann ghosts of phi applied inside the projection
+  if ((rc = launch_proj4(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], 1, 1, 1)))
+    return rc;
+  // :138-139
+  if ((rc = launch_bc4_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
+  return launch_bc_pr(c, s[C3D_PR]);
+}
+
 extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   CHECK_CTX(c);
+  if (c->order == 4) return substep4(c, ns);
   int rc;
   void** s = c->state;
   if ((rc = c3d_PMcoeUpdate(c, ns))) return rc;
@@ -1001,8 +1141,14 @@ extern "C" int c3d_step(c3d_ctx* c) {
   for (int ns = 1; ns <= c->iadvance; ++ns)
     if ((rc = c3d_substep(c, ns))) return rc;
   // CheckDivergence, CalcVmax (:127-130) and the next step's CFL, one pass
+  // (order 4: the divergence with the 4th-order differences in a pass of its own)
   const int next = c->cfl_slot ^ 1;
-  if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 7, next))) return rc;
+  if (c->order == 4) {
+    if ((rc = launch_div4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ]))) return rc;
+    if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 5, next))) return rc;
+  } else {
+    if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 7, next))) return rc;
+  }
   c->last_cfl_slot = c->cfl_slot;
   c->cfl_slot = next;
   c->cfl_valid = true;
@@ -1033,6 +1179,15 @@ extern "C" int c3d_project(c3d_ctx* c) {
   c->cfl_valid = false;
   const double save = c->pmAlpha;
   c->pmAlpha = 1.0;
+  if (c->order == 4) {
+    if ((rc = launch_bc4_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
+    if ((rc = launch_prsrc4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], 1.0))) return rc;
+    if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+    if ((rc = launch_proj4(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr, 1, 0, 1)))
+      return rc;
+    c->pmAlpha = save;
+    return launch_bc4_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1);
+  }
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
   if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], nullptr, 1.0))) return rc;
   if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;

@@ -7,7 +7,9 @@
 //   k  in [0, nz)    z cell index (reference kc = k+1); z-periodic by index wrap
 // The reference's x and z halo layers (m_Decomp2d.f90:1652-1738) are never
 // stored: on one device they are exact copies of the opposite interior plane,
-// so kernels read the wrapped interior index instead.
+// so kernels read the wrapped interior index instead.  The 4th-order path
+// (src/CFD_4th) keeps one ghost plane per wall, jj in [0, ny+1], in the same
+// layout.
 //
 // Spectral (PPE) array: complex, element (kx, j0, kz) at kx + P * (j0 + ny * kz)
 //   kx in [0, nx/2] (pitch P >= nx/2+1), j0 = jj-1 in [0, ny), kz in [0, nz).
@@ -119,7 +121,19 @@ struct DevScalars {
   double uddxmax[2];  // CalcMaxCFL result; two slots so a step can compute the next one
   double vmax[3];
   double uxaver;
-  double pad[4];
+  double dp1ns;       // 4th order: pmAlpha * dpmdxns, subtracted from RhsX
+  double pad[3];
+};
+
+// 4th-order x/z stencil constants (src/CFD_4th/m_MeshAndMetries.f90:87-91)
+// and the per-substep scalars of the 4th-order routines
+template <typename T>
+struct Coef4 {
+  T dx1, dx2, dx3, dx4, dz1, dz2, dz3, dz4;
+  T dxx1, dxx2, dxx3, dxx4, dxx5, dzz1, dzz2, dzz3, dzz4, dzz5;
+  T ic1, ic2, ic3, ic4;
+  T ucrf, xnu, gx, gy, gz;
+  T pmGamma, pmTheta, pmAlpha, pmBeta;
 };
 
 enum Stage {
@@ -134,8 +148,11 @@ struct c3d_ctx {
   c3d_params p;
   c3d::Geom g;
   int prec;               // C3D_FP64 / C3D_FP32
+  int order;              // 2: src/CFD_2nd FIMP path, 4: src/CFD_4th path
   size_t esize;           // bytes per real element
   hipStream_t stream;
+  // 4th-order stencil constants in the reference's evaluation order
+  double dxc[4], dzc[4], dxxc[5], dzzc[5], icoe[4];
 
   // host metrics (reference indexing, vectors sized ny+3 / nx+1 / nz+1)
   std::vector<double> met[C3D_NMETRICS];
@@ -235,9 +252,27 @@ int launch_diag(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int 
 int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot);
 
 // ---- launchers (k_adi.hip) ----
-// comp: 0 ux (CN metrics), 1 uy (node metrics + wall row), 2 uz (CN metrics)
+// comp: 0 ux (CN metrics), 1 uy (node metrics + wall row), 2 uz (CN metrics).
+// add_dpmdxns: 2nd order adds dpmdxns to RhsX during the x sweep; 4th order
+// (y solve only) subtracts dp1ns from RhsX as the sweep loads it
 int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy_ym,
                int add_dpmdxns);
+
+// ---- launchers (k_stencil4.hip, 4th-order path) ----
+template <typename T>
+Coef4<T> make_coef4(const c3d_ctx* c);
+// clcRhsXYZ; want_flow: accumulate the wall-shear integral and finalize
+// dpmdxns / dp1ns / PrGradAver on the device
+int launch_rhs4(c3d_ctx* c, const void* ux, const void* uy, const void* uz, const void* pr,
+                void* rx, void* ry, void* rz, void* hx, void* hy, void* hz, int want_flow);
+int launch_sub_dp1ns(c3d_ctx* c, void* rx);
+int launch_prsrc4(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* prsrc,
+                  double alpha_override);
+int launch_proj4(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, void* pr, int do_vel,
+                 int do_pr, int ghost_from_interior);
+int launch_bc4_vel(c3d_ctx* c, void* ux, void* uy, void* uz, const void* uy_ym, int full);
+// CheckDivergence with the 4th-order x/z differences -> DevScalars::divmax2
+int launch_div4(c3d_ctx* c, const void* ux, const void* uy, const void* uz);
 
 // ---- launchers (k_fft.hip / k_ppe.hip) ----
 bool fft_size_supported(int n);

@@ -140,13 +140,16 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
 
 // Thomas sweeps along y with shared factors: a(j), vecm(j), arrmn(j) for
 // j = 1..ny (reference indexing).  WALLROW: row 1 is the wall condition
-// c=1, f=duy_ym (m_TSchemeFIMP_inc.f90:701-716).
-template <typename T, bool WALLROW>
+// c=1, f=duy_ym (m_TSchemeFIMP_inc.f90:701-716).  SHIFT: the right-hand side
+// is loaded as r - dp1ns (the 4th-order constant-flow-rate term,
+// src/CFD_4th/m_TScheme.f90:396-401, applied before the solve).
+template <typename T, bool WALLROW, bool SHIFT>
 __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict__ ta,
                                                const double* __restrict__ tvecm,
                                                const double* __restrict__ tarr,
                                                T* __restrict__ r, T* __restrict__ u,
-                                               const T* __restrict__ duy) {
+                                               const T* __restrict__ duy,
+                                               const DevScalars* __restrict__ sc) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.nx || k >= g.nz) return;
@@ -158,10 +161,12 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
   // carry no conditionals (every prefetch address is in range), so loads
   // stay outstanding across iterations; short guarded tails finish the line.
   constexpr int PD = 8;
+  const T shift = SHIFT ? (T)sc->dp1ns : T(0);
+  auto ld = [&](long long q) { return SHIFT ? r[q] - shift : r[q]; };
   T buf[PD];
 #pragma unroll
-  for (int q = 0; q < PD; ++q) buf[q] = r[base + sj * min(2 + q, ny)];
-  T f = WALLROW ? duy[i + (long long)g.nx * k] : r[base + sj];
+  for (int q = 0; q < PD; ++q) buf[q] = ld(base + sj * min(2 + q, ny));
+  T f = WALLROW ? duy[i + (long long)g.nx * k] : ld(base + sj);
   f = f / (T)tvecm[1];
   r[base + sj] = f;
   int jb = 2;
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
     for (int q = 0; q < PD; ++q) {
       const int jc = jb + q;
       const T rv = buf[q];
-      buf[q] = r[base + sj * (jc + PD)];
+      buf[q] = ld(base + sj * (jc + PD));
       f = (rv - (T)ta[jc] * f) / (T)tvecm[jc];
       r[base + sj * jc] = f;
     }
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
     }
   }
   for (int jc = jb + PD; jc <= ny; ++jc) {
-    f = (r[base + sj * jc] - (T)ta[jc] * f) / (T)tvecm[jc];
+    f = (ld(base + sj * jc) - (T)ta[jc] * f) / (T)tvecm[jc];
     if (jc < ny) r[base + sj * jc] = f;
   }
   u[base + sj * ny] += f;
@@ -268,6 +273,26 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
   const auto& tb = c->adi[ns - 1];
   const double N = (double)g.nx * g.ny * g.nz;
   hipEvent_t ev;
+  // 4th order: Crank-Nicolson in y only, the y sweep reads the right-hand side
+  // in place (src/CFD_4th/m_TScheme.f90:443-549)
+  if (c->order == 4) {
+    prof_begin(c, ST_ADI_Y, &ev);
+    dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
+    double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
+    const DevScalars* sc = (const DevScalars*)c->d_sc;
+    if (comp == 1)
+      hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)rhs, (T*)u, (const T*)duy_ym, sc);
+    else if (add)
+      hipLaunchKernelGGL((k_adi_y<T, false, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)rhs, (T*)u, (const T*)nullptr, sc);
+    else
+      hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)rhs, (T*)u, (const T*)nullptr, sc);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
+    return C3D_OK;
+  }
   // fused x and z sweeps: rhs -> tmpA, then y sweep + update from tmpA
   if (xz_fusable(g, tb)) {
     prof_begin(c, ST_ADI_XZ, &ev);
@@ -279,11 +304,11 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
     dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
     double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
     if (comp == 1)
-      hipLaunchKernelGGL((k_adi_y<T, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)c->tmpA, (T*)u, (const T*)duy_ym);
+      hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)c->tmpA, (T*)u, (const T*)duy_ym, (const DevScalars*)c->d_sc);
     else
-      hipLaunchKernelGGL((k_adi_y<T, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)c->tmpA, (T*)u, (const T*)nullptr);
+      hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                         (T*)c->tmpA, (T*)u, (const T*)nullptr, (const DevScalars*)c->d_sc);
     C3D_HIP(hipGetLastError());
     prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
     return C3D_OK;
@@ -311,11 +336,11 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
   dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   if (comp == 1)
-    hipLaunchKernelGGL((k_adi_y<T, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2], (T*)rhs,
-                       (T*)u, (const T*)duy_ym);
+    hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                       (T*)rhs, (T*)u, (const T*)duy_ym, (const DevScalars*)c->d_sc);
   else
-    hipLaunchKernelGGL((k_adi_y<T, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                       (T*)rhs, (T*)u, (const T*)nullptr);
+    hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                       (T*)rhs, (T*)u, (const T*)nullptr, (const DevScalars*)c->d_sc);
   C3D_HIP(hipGetLastError());
   prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
   return C3D_OK;

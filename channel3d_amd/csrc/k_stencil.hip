@@ -691,12 +691,14 @@ int launch_bc_pr(c3d_ctx* c, void* pr) {
 
 int launch_update_uy_ym(c3d_ctx* c, void* uy_ym, void* duy_ym) {
   const int n = c->g.nx * c->g.nz;
+  // the 4th-order code sets the wall value to zero (src/CFD_4th/m_FlowCase.f90:162-172)
+  const double v = c->order == 4 ? 0.0 : c->p.uybc[2];
   if (c->prec == C3D_FP64)
     hipLaunchKernelGGL(k_update_uy_ym<double>, dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
-                       (double)c->p.uybc[2], (double*)uy_ym, (double*)duy_ym);
+                       v, (double*)uy_ym, (double*)duy_ym);
   else
     hipLaunchKernelGGL(k_update_uy_ym<float>, dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
-                       (float)c->p.uybc[2], (float*)uy_ym, (float*)duy_ym);
+                       (float)v, (float*)uy_ym, (float*)duy_ym);
   C3D_HIP(hipGetLastError());
   return C3D_OK;
 }
@@ -753,6 +755,7 @@ static int diag_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, in
     case 8: DIAG_LAUNCH(8); break;
     case 7: DIAG_LAUNCH(7); break;
     case 6: DIAG_LAUNCH(6); break;
+    case 5: DIAG_LAUNCH(5); break;
     default: set_error("diag: unsupported selection"); return C3D_ERR_ARG;
   }
 #undef DIAG_LAUNCH

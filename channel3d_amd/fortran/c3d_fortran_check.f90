@@ -39,7 +39,7 @@ program c3d_fortran_check
   p%bc = [C3D_BC_PERIOD, C3D_BC_PERIOD, C3D_BC_NOSLIP, C3D_BC_NOSLIP, C3D_BC_PERIOD, C3D_BC_PERIOD]
   p%uxbc = 0.0d0; p%uybc = 0.0d0; p%uzbc = 0.0d0; p%gravity = 0.0d0
   p%is_ux_const = 1; p%ubulk = 1.0d0; p%ybulk1 = 0.0d0; p%ybulk2 = p%yly
-  p%ischeme = 3; p%precision = C3D_FP64
+  p%ischeme = 3; p%precision = C3D_FP64; p%order = 2; p%u_crf = 0.0d0
 
   allocate(yp(0:nyc+1))
   allocate(ux(0:nxc+2, 0:nyc+2, 0:nzc+2), uy(0:nxc+2, 0:nyc+2, 0:nzc+2), &

@@ -39,6 +39,8 @@ module channel3d_c
     real(c_double) :: ubulk, ybulk1, ybulk2
     integer(c_int) :: ischeme
     integer(c_int) :: precision
+    integer(c_int) :: order          ! 2 (src/CFD_2nd) or 4 (src/CFD_4th)
+    real(c_double) :: u_crf          ! uCRF of the 4th-order code
   end type c3d_params
 
   interface
@@ -135,6 +137,12 @@ module channel3d_c
     integer(c_int) function c3d_clcRhsZ(ctx, ux, uy, uz, rhs, hist, pr) bind(C, name="c3d_clcRhsZ")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, ux, uy, uz, rhs, hist, pr
+    end function
+    ! 4th-order contexts: clcRhsXYZ (src/CFD_4th/m_TScheme.f90:189-438)
+    integer(c_int) function c3d_clcRhsXYZ(ctx, ux, uy, uz, rx, ry, rz, hx, hy, hz, pr) &
+        bind(C, name="c3d_clcRhsXYZ")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, rx, ry, rz, hx, hy, hz, pr
     end function
     integer(c_int) function c3d_clcU1Hat(ctx, ux, rhs) bind(C, name="c3d_clcU1Hat")
       import :: c_int, c_ptr

@@ -25,13 +25,13 @@ STATE_LAYOUT = {UX: HALO, UY: HALO, UZ: HALO, PR: HALO, HISTX: INTERIOR, HISTY: 
 # metric ids
 (M_YP, M_YC, M_DYP, M_DYC, M_RDYP, M_RDYC, M_YINTERP, M_AM2PR, M_AC2PR, M_AP2PR,
  M_AM2C, M_AC2C, M_AP2C, M_AM2P, M_AC2P, M_AP2P, M_AM2CCN, M_AP2CCN,
- M_WAVEX, M_WAVEZ) = range(20)
+ M_WAVEX, M_WAVEZ, M_AC2CCN) = range(21)
 
 EXPORTS = (
     "c3d_create", "c3d_destroy", "c3d_last_error", "c3d_sync", "c3d_mesh_coords",
     "c3d_get_metric", "c3d_set_dt", "c3d_field_alloc", "c3d_field_free", "c3d_upload",
-    "c3d_download", "c3d_field_copy", "c3d_state_field", "c3d_PMcoeUpdate", "c3d_Update_uy_ym", "c3d_clcRhsX",
-    "c3d_clcRhsY", "c3d_clcRhsZ", "c3d_clcU1Hat", "c3d_clcU2Hat", "c3d_clcU3Hat",
+    "c3d_download", "c3d_field_copy", "c3d_state_field", "c3d_PMcoeUpdate",
+    "c3d_Update_uy_ym", "c3d_clcRhsX", "c3d_clcRhsY", "c3d_clcRhsZ", "c3d_clcRhsXYZ", "c3d_clcU1Hat", "c3d_clcU2Hat", "c3d_clcU3Hat",
     "c3d_SetBC_and_UpdateHaloForPrSrc", "c3d_SetBC_and_UpdateHalo_pr",
     "c3d_SetBC_and_UpdateHalo", "c3d_clcPrSrc", "c3d_clcPPE", "c3d_FluidVelUpdate",
     "c3d_PressureUpdate", "c3d_CalcMaxCFL", "c3d_CheckDivergence", "c3d_CalcVmax",
@@ -54,6 +54,8 @@ class Params(ctypes.Structure):
         ("ubulk", ctypes.c_double), ("ybulk1", ctypes.c_double), ("ybulk2", ctypes.c_double),
         ("ischeme", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("order", ctypes.c_int),
+        ("u_crf", ctypes.c_double),
     ]
 
 
@@ -97,6 +99,7 @@ def load(path: str | None = None):
         "c3d_clcRhsX": ([_VP] * 7, I),
         "c3d_clcRhsY": ([_VP] * 7, I),
         "c3d_clcRhsZ": ([_VP] * 7, I),
+        "c3d_clcRhsXYZ": ([_VP] * 11, I),
         "c3d_clcU1Hat": ([_VP] * 3, I),
         "c3d_clcU2Hat": ([_VP] * 4, I),
         "c3d_clcU3Hat": ([_VP] * 3, I),

@@ -22,6 +22,12 @@
  *      C3D_WALL      real(RK) (1:nxc, 1:nzc)                m_Variables.f90:96-97
  *  - One HIP stream per context; calls are asynchronous on it unless stated.
  *    Contexts are single-host-thread and not reentrant (as the reference).
+ *  - A context of order 4 replaces the 4th-order solver's routines
+ *    (src/CFD_4th/m_TScheme.f90:24-25, m_Poisson.f90:16); its C3D_HALO
+ *    layout is the 4th-order halo array
+ *      real(RK) (-1:nxc+2, 0:nyc+1, -1:nzc+2)             CFD_4th m_Variables.f90:46-58
+ *    and the same entry points dispatch to the 4th-order kernels, except
+ *    c3d_clcRhsX/Y/Z (use c3d_clcRhsXYZ, the 4th-order code's only RHS).
  */
 #ifndef CHANNEL3D_H
 #define CHANNEL3D_H
@@ -68,6 +74,12 @@ typedef struct c3d_params {
   double ubulk, ybulk1, ybulk2;
   int ischeme;                 /* 1 AB2, 2 RK2, 3 RK3                         */
   int precision;               /* C3D_FP64 or C3D_FP32                        */
+  int order;                   /* 2 (src/CFD_2nd FIMP path; 0 means 2) or 4
+                                  (src/CFD_4th path: bc[2] no-slip, bc[3]
+                                  no-slip = channel or free-slip = half
+                                  channel; ybulk1/2 and ubulk unused)         */
+  double u_crf;                /* uCRF, convective reference frame velocity
+                                  of the 4th-order code (0 unless IsUseCRF)   */
 } c3d_params;
 
 /* ---------------------------------------------------------------- setup -- */
@@ -93,7 +105,9 @@ enum {
   C3D_M_YP = 0, C3D_M_YC, C3D_M_DYP, C3D_M_DYC, C3D_M_RDYP, C3D_M_RDYC, C3D_M_YINTERP,
   C3D_M_AM2PR, C3D_M_AC2PR, C3D_M_AP2PR, C3D_M_AM2C, C3D_M_AC2C, C3D_M_AP2C,
   C3D_M_AM2P, C3D_M_AC2P, C3D_M_AP2P, C3D_M_AM2CCN, C3D_M_AP2CCN,
-  C3D_M_WAVEX, C3D_M_WAVEZ, C3D_NMETRICS
+  C3D_M_WAVEX, C3D_M_WAVEZ,
+  C3D_M_AC2CCN, /* ac2cForCN of the 4th-order code (m_TScheme.f90:41-51) */
+  C3D_NMETRICS
 };
 
 /* change dt (recomputes the implicit-operator tables), m_ChannelSystem.f90:81-88 */
@@ -132,7 +146,14 @@ int c3d_clcRhsY(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, vo
                 void* HistYOld, const void* pressure);
 int c3d_clcRhsZ(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* RhsZ,
                 void* HistZOld, const void* pressure);
-/* m_TSchemeFIMP_inc.f90:263-347 / 636-729 / 1035-1128 (the _010 variants) */
+/* 4th-order contexts only: clcRhsXYZ (src/CFD_4th/m_TScheme.f90:189-438), RhsZ
+ * a halo field; with IsUxConst RhsX already carries -pmAlpha*dpmdxns */
+int c3d_clcRhsXYZ(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* RhsX,
+                  void* RhsY, void* RhsZ, void* HistXOld, void* HistYOld, void* HistZOld,
+                  const void* pressure);
+/* m_TSchemeFIMP_inc.f90:263-347 / 636-729 / 1035-1128 (the _010 variants);
+ * order 4: src/CFD_4th/m_TScheme.f90:443-549 (Crank-Nicolson in y only).
+ * The right-hand side is used as scratch. */
 int c3d_clcU1Hat(c3d_ctx* ctx, void* ux, void* RhsX);
 int c3d_clcU2Hat(c3d_ctx* ctx, void* uy, void* RhsY, const void* duy_ym);
 int c3d_clcU3Hat(c3d_ctx* ctx, void* uz, void* RhsZ);
@@ -141,16 +162,21 @@ int c3d_SetBC_and_UpdateHaloForPrSrc(c3d_ctx* ctx, void* ux, void* uy, void* uz,
                                      const void* uy_ym);
 int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* ctx, void* pressure);
 int c3d_SetBC_and_UpdateHalo(c3d_ctx* ctx, void* ux, void* uy, void* uz, const void* uy_ym);
-/* m_TScheme.f90:246-276; divmax is a HOST pointer, written after a sync */
+/* m_TScheme.f90:246-276; divmax is a HOST pointer, written after a sync.
+ * order 4 (src/CFD_4th/m_TScheme.f90:114-143): no pressure correction,
+ * pressure may be NULL */
 int c3d_clcPrSrc(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* prsrc,
                  void* pressure, double* divmax);
-/* m_Poisson.f90:567-688 (clcPPE_x1 semantics: gauge pin, normfft) */
+/* m_Poisson.f90:567-688 (clcPPE_x1 semantics: gauge pin, normfft);
+ * order 4: src/CFD_4th/m_Poisson.f90:371-489 with its modified wavenumbers */
 int c3d_clcPPE(c3d_ctx* ctx, void* prsrc, void* prphiHalo);
-/* m_TScheme.f90:281-306 */
+/* m_TScheme.f90:281-306; order 4: src/CFD_4th/m_TScheme.f90:148-173 */
 int c3d_FluidVelUpdate(c3d_ctx* ctx, const void* prphiHalo, void* ux, void* uy, void* uz);
-/* m_TSchemeFIMP_inc.f90:1371-1386 */
+/* m_TSchemeFIMP_inc.f90:1371-1386; order 4: src/CFD_4th/m_TScheme.f90:590-610
+ * (pressure += (1 - pmBeta Lyy) phi) */
 int c3d_PressureUpdate(c3d_ctx* ctx, void* pressure, const void* prphiHalo);
-/* m_Tools.f90:21-46, 51-75, 80-104, 250-270 (results on the host, synchronous) */
+/* m_Tools.f90:21-46, 51-75, 80-104, 250-270 (results on the host, synchronous);
+ * order 4: src/CFD_4th/m_Tools.f90:24-110, 256-275 */
 int c3d_CalcMaxCFL(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
                    double* uddxmax);
 int c3d_CheckDivergence(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,

@@ -44,6 +44,7 @@ struct orc4_ctx {
   double pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, SimTime, dt;
   double dpmdxns, PrGradAver;
   double divmax1, divmax2, cfl, uxaver, vmax[3];
+  int ppe_route;
   double* f[ORC_NFIELDS];
   fftplan px, pz;
 };
@@ -256,6 +257,7 @@ void orc4_PMcoeUpdate(orc4_ctx* c, int ns) {
   c->pmBeta = 0.5 * c->aconst[s] * c->p.xnu * c->dt;
   c->SimTime = c->SimTime + c->dt * c->aconst[s];
   c->pmAlphaC = c->aconst[s];
+  if (ns == 1) c->PrGradAver = 0.0;
 }
 
 /* m_FlowCase.f90:162-172 */
@@ -657,30 +659,44 @@ void orc4_clcPrSrc(orc4_ctx* c, const double* ux, const double* uy, const double
 }
 
 /* ------------------------------------------------------------------------- */
-/* m_Poisson.f90:371-489 clcPPE_x1 on one rank (4th-order wavenumbers)       */
+/* m_Poisson.f90:371-489 clcPPE_x1 and :758-877 clcPPE_z1 on one rank, with  */
+/* the 4th-order wavenumbers.  The reference times its routes at start-up and */
+/* keeps the fastest (:68-160); both are restated, x-first is the default.    */
 /* ------------------------------------------------------------------------- */
-void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi) {
+static void ppe4_fft_x(orc4_ctx* c, double* arr, int inverse) {
   const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
   const size_t sj = (size_t)nxc, sk = (size_t)nxc * (size_t)nyc;
-  double* arr = prsrc;
-  /* forward x */
 #ifdef _OPENMP
 #pragma omp parallel OMP_NT
 #endif
   {
-    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)(nxc > nzc ? nxc : nzc));
-    double* in = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
-    double* out = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
+    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)nxc);
+    double* out = (double*)malloc(sizeof(double) * (size_t)nxc);
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
     for (int kc = 1; kc <= nzc; ++kc)
       for (int jc = 1; jc <= nyc; ++jc) {
         double* row = arr + (size_t)(jc - 1) * sj + (size_t)(kc - 1) * sk;
-        orci_r2hc(&c->px, row, out, wk);
+        if (inverse) orci_hc2r(&c->px, row, out, wk);
+        else orci_r2hc(&c->px, row, out, wk);
         memcpy(row, out, sizeof(double) * (size_t)nxc);
       }
-    /* forward z */
+    free(wk);
+    free(out);
+  }
+}
+
+static void ppe4_fft_z(orc4_ctx* c, double* arr, int inverse) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const size_t sj = (size_t)nxc, sk = (size_t)nxc * (size_t)nyc;
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+  {
+    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)nzc);
+    double* in = (double*)malloc(sizeof(double) * (size_t)nzc);
+    double* out = (double*)malloc(sizeof(double) * (size_t)nzc);
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
@@ -688,12 +704,20 @@ void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi) {
       for (int ic = 1; ic <= nxc; ++ic) {
         double* col = arr + (size_t)(ic - 1) + (size_t)(jc - 1) * sj;
         for (int k = 0; k < nzc; ++k) in[k] = col[(size_t)k * sk];
-        orci_r2hc(&c->pz, in, out, wk);
+        if (inverse) orci_hc2r(&c->pz, in, out, wk);
+        else orci_r2hc(&c->pz, in, out, wk);
         for (int k = 0; k < nzc; ++k) col[(size_t)k * sk] = out[k];
       }
-    free(wk); free(in); free(out);
+    free(wk);
+    free(in);
+    free(out);
   }
-  /* y solves, :411-464 */
+}
+
+/* y solves with the gauge pin, :411-464 / :806-859 */
+static void ppe4_tridiag(orc4_ctx* c, double* arr) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const size_t sj = (size_t)nxc, sk = (size_t)nxc * (size_t)nyc;
   const double *am2Pr = c->met[ORC_AM2PR], *ac2Pr = c->met[ORC_AC2PR], *ap2Pr = c->met[ORC_AP2PR];
   const double *WaveNumX = c->met[ORC_WAVEX], *WaveNumZ = c->met[ORC_WAVEZ];
 #ifdef _OPENMP
@@ -722,34 +746,26 @@ void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi) {
       }
     free(a); free(b); free(cc); free(w);
   }
-  /* inverse z, inverse x */
-#ifdef _OPENMP
-#pragma omp parallel OMP_NT
-#endif
-  {
-    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)(nxc > nzc ? nxc : nzc));
-    double* in = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
-    double* out = (double*)malloc(sizeof(double) * (size_t)(nxc > nzc ? nxc : nzc));
-#ifdef _OPENMP
-#pragma omp for schedule(static)
-#endif
-    for (int jc = 1; jc <= nyc; ++jc)
-      for (int ic = 1; ic <= nxc; ++ic) {
-        double* col = arr + (size_t)(ic - 1) + (size_t)(jc - 1) * sj;
-        for (int k = 0; k < nzc; ++k) in[k] = col[(size_t)k * sk];
-        orci_hc2r(&c->pz, in, out, wk);
-        for (int k = 0; k < nzc; ++k) col[(size_t)k * sk] = out[k];
-      }
-#ifdef _OPENMP
-#pragma omp for schedule(static)
-#endif
-    for (int kc = 1; kc <= nzc; ++kc)
-      for (int jc = 1; jc <= nyc; ++jc) {
-        double* row = arr + (size_t)(jc - 1) * sj + (size_t)(kc - 1) * sk;
-        orci_hc2r(&c->px, row, out, wk);
-        memcpy(row, out, sizeof(double) * (size_t)nxc);
-      }
-    free(wk); free(in); free(out);
+}
+
+void orc4_set_ppe_route(orc4_ctx* c, int route) { c->ppe_route = route ? 1 : 0; }
+
+void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const size_t sj = (size_t)nxc, sk = (size_t)nxc * (size_t)nyc;
+  double* arr = prsrc;
+  if (c->ppe_route == 0) {
+    ppe4_fft_x(c, arr, 0);
+    ppe4_fft_z(c, arr, 0);
+    ppe4_tridiag(c, arr);
+    ppe4_fft_z(c, arr, 1);
+    ppe4_fft_x(c, arr, 1);
+  } else {
+    ppe4_fft_z(c, arr, 0);
+    ppe4_fft_x(c, arr, 0);
+    ppe4_tridiag(c, arr);
+    ppe4_fft_x(c, arr, 1);
+    ppe4_fft_z(c, arr, 1);
   }
   const double normfft = c->normfft;
 #ifdef _OPENMP

@@ -65,6 +65,8 @@ void orc4_SetBC_and_UpdateHalo_pr(orc4_ctx* c, double* pr);
 void orc4_clcPrSrc(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
                    double* prsrc, double* divmax);
 void orc4_clcPPE(orc4_ctx* c, double* prsrc, double* prphi_halo);
+/* 0: clcPPE_x1 route (default), 1: clcPPE_z1 route (m_Poisson.f90:758-877) */
+void orc4_set_ppe_route(orc4_ctx* c, int route);
 void orc4_FluidVelUpdate(orc4_ctx* c, const double* prphi_halo, double* ux, double* uy,
                          double* uz);
 void orc4_PressureUpdate(orc4_ctx* c, double* pr, const double* prphi_halo);

@@ -163,6 +163,7 @@ def lib():
         L.orc4_SetBC_and_UpdateHalo_pr.argtypes = [ctypes.c_void_p, _DP]
         L.orc4_clcPrSrc.argtypes = [ctypes.c_void_p] + [_DP] * 5
         L.orc4_clcPPE.argtypes = [ctypes.c_void_p, _DP, _DP]
+        L.orc4_set_ppe_route.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc4_FluidVelUpdate.argtypes = [ctypes.c_void_p] + [_DP] * 4
         L.orc4_PressureUpdate.argtypes = [ctypes.c_void_p, _DP, _DP]
         for name in ("orc4_CalcMaxCFL", "orc4_CheckDivergence"):
@@ -413,6 +414,10 @@ class Oracle4:
 
     def project(self):
         lib().orc4_project(self._h)
+
+    def set_ppe_route(self, route):
+        """0: clcPPE_x1 (x first), 1: clcPPE_z1 (z first)"""
+        lib().orc4_set_ppe_route(self._h, int(route))
 
     def call(self, name, *arrays):
         fn = getattr(lib(), "orc4_" + name)

@@ -21,3 +21,12 @@ STIFF = dict(nxc=16, nyc=20, nzc=20, xlx=1.0, yly=2.0, zlz=1.0, xnu=0.05, dt=0.0
 # free-slip top wall (half-channel style boundary)
 FREESLIP = dict(nxc=32, nyc=24, nzc=24, xlx=4.0, yly=1.0, zlz=2.0, xnu=1.0 / 2000.0,
                 dt=0.005, gamma=1.5, ubulk=1.0, bc=(0, 0, -1, -2, 0, 0))
+
+# 4th-order channel (src/CFD_4th): Re_tau=180 config 192x128x160, Lx=4pi,
+# Lz=4pi/3, nu from Re_tau=0.1538 Re_b^0.887741, dt=0.005
+C2_4TH = dict(nxc=192, nyc=128, nzc=160, xlx=4 * math.pi, yly=2.0, zlz=4 * math.pi / 3,
+              xnu=3.4969e-4, dt=0.005, gamma=1.5, ubulk=1.0, order=4)
+SMALL4 = dict(C2_4TH, nxc=32, nyc=24, nzc=24)
+# half channel (free-slip top), odd-sized FFT lengths, stronger stretching
+HALF4 = dict(nxc=48, nyc=20, nzc=36, xlx=3.0, yly=1.0, zlz=2.0, xnu=1.0 / 1500.0, dt=0.004,
+             gamma=2.0, ubulk=1.0, order=4, flow_type=2)
