@@ -27,6 +27,9 @@ TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "fp64_step_768x384x576.md")
 
 PROD = dict(nxc=768, nyc=384, nzc=576, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
             xnu=9.9369e-5, dt=0.002, gamma=1.5, ubulk=1.0)
+# 4th-order channel at Re_tau=180 (192x128x160, Lx=4pi, Lz=4pi/3), reported beside
+FOURTH = dict(nxc=192, nyc=128, nzc=160, xlx=4 * math.pi, yly=2.0, zlz=4 * math.pi / 3,
+              xnu=3.4969e-4, dt=0.005, gamma=1.5, ubulk=1.0, order=4)
 
 
 def log(msg):
@@ -85,6 +88,91 @@ def run_cpu_baseline(case, ux, uy, uz, threads):
     return t_ppe, t_sub
 
 
+def run_fourth_order(steps, warmup):
+    """the 4th-order step at 192x128x160 on the same GPU: steps/s and the
+    per-stage split (inputs resident, HIP events on the library stream)"""
+    from channel3d_amd import Channel, synth, lib as L
+    c = FOURTH
+    nx, ny, nz = c["nxc"], c["nyc"], c["nzc"]
+    g = Channel(c)
+    ux, uy, uz = synth.laminar_noise(nx, ny, nz, g.yp, yly=c["yly"], ubulk=1.0, seed=0)
+
+    def to4(a):
+        b = np.zeros((nx + 4, ny + 2, nz + 4), order="F")
+        b[2:nx + 2, 0:ny + 2, 2:nz + 2] = a[1:nx + 1, 0:ny + 2, 1:nz + 1]
+        return b
+    g.upload(L.UX, to4(ux))
+    g.upload(L.UY, to4(uy))
+    g.upload(L.UZ, to4(uz))
+    g.project()
+    for _ in range(warmup):
+        g.step()
+    g.sync()
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step()
+    g.sync()
+    t = time.perf_counter() - t0
+    st = g.profile_stats()
+    g.profile(False)
+    d = g.diagnostics()
+    g.close()
+    stages = {k: dict(calls=v["calls"], avg_ms=round(v["total_ms"] / v["calls"], 4),
+                      gbs=round(v["bytes_per_call"] / (v["total_ms"] / v["calls"] * 1e-3) / 1e9, 1)
+                      if v["bytes_per_call"] else None) for k, v in st.items()}
+    return dict(workload=f"4th-order channel RK3 step {nx}x{ny}x{nz}, Re_tau=180, FP64",
+                steps=steps, ms_per_step=round(1e3 * t / steps, 3),
+                steps_per_s=round(steps / t, 2),
+                grid_points_steps_per_s=round(nx * ny * nz * steps / t, 1),
+                divmax_after_step=d["divmax2"], stages=stages)
+
+
+def run_precision_sweep(g, L, nsolves, fp64_per_s, nsteps):
+    """FP64 pressure solve against FP32 + s refinement sweeps on the same
+    source (the last step's): solves/s and relative max-norm difference of
+    phi from the FP64 solution; then whole steps with the refined solve."""
+    import numpy as np
+
+    def interior(a):
+        return a[1:-2, 1:-2, 1:-2]
+
+    g.set_ppe_refine(-1)
+    g.ppe(L.ARR1, L.HALOARR)
+    ref = interior(g.download(L.HALOARR)).copy()
+    scale = np.abs(ref).max()
+    rows = [dict(sweeps=-1, form="fp64", solves_per_s=round(fp64_per_s, 3), rel_err=0.0)]
+    for s in (0, 1, 2):
+        g.set_ppe_refine(s)
+        g.ppe(L.ARR1, L.HALOARR)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(nsolves):
+            g.ppe(L.ARR1, L.HALOARR)
+        g.sync()
+        rate = nsolves / (time.perf_counter() - t0)
+        err = float(np.abs(interior(g.download(L.HALOARR)) - ref).max() / scale)
+        rows.append(dict(sweeps=s, form=f"fp32+{s}", solves_per_s=round(rate, 3),
+                         rel_err=err))
+        log(f"PPE fp32 + {s} sweeps: {1e3 / rate:.2f} ms/solve, rel err vs fp64 {err:.2e}")
+    del ref
+    # whole steps with the 2-sweep solve (error <= 1e-10 form)
+    g.set_ppe_refine(2)
+    g.step()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        g.step()
+    g.sync()
+    ms = 1e3 * (time.perf_counter() - t0) / nsteps
+    d = g.diagnostics()
+    g.set_ppe_refine(-1)
+    log(f"step with fp32+2 pressure solve: {ms:.2f} ms/step, divmax {d['divmax2']:.2e}")
+    return dict(ppe=rows, step_fp32_2sweeps_ms=round(ms, 3), step_divmax=d["divmax2"],
+                note="FP64 fields throughout; only the pressure solve changes form")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +182,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="64^3 workload (quick check)")
+    ap.add_argument("--no-fourth", action="store_true",
+                    help="skip the 4th-order 192x128x160 line reported beside the main one")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32 + refinement pressure-solve sweep")
     ap.add_argument("--probe", choices=["copy"], default=None,
                     help="only run plain field copies (HBM reference for traffic counters)")
     a = ap.parse_args(argv)
@@ -180,6 +272,9 @@ def main(argv=None):
     t_ppe = time.perf_counter() - t0
     ppe_per_s = a.ppe_solves / t_ppe
     log(f"{a.ppe_solves} PPE solves: {1e3 * t_ppe / a.ppe_solves:.2f} ms each")
+    precision = None
+    if not a.no_precision:
+        precision = run_precision_sweep(g, L, a.ppe_solves, ppe_per_s, max(a.steps // 4, 2))
     copy_gbs, _ = copy_probe(10)
     log(f"copy probe: {copy_gbs:.1f} GB/s")
 
@@ -214,6 +309,11 @@ def main(argv=None):
                    sample=(f"oracle (C, FP64, OpenMP) on the host: one PPE solve and one RK3 "
                            f"substep at {nx}x{ny}x{nz}; steps/s = 1/(3 x substep time)"))
     del ux, uy, uz
+    fourth = None
+    if not a.no_fourth and not a.small:
+        log("4th-order step 192x128x160")
+        fourth = run_fourth_order(max(a.steps, 20), max(a.warmup, 2))
+        log(f"4th-order: {fourth['ms_per_step']} ms/step")
 
     out = {
         "metric": METRIC,
@@ -239,6 +339,8 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "stages": stage_rows,
         "divmax_after_step": d1["divmax2"],
+        "fourth_order": fourth,
+        "precision_sweep": precision,
     }
     print(json.dumps(out), flush=True)
     g.close()

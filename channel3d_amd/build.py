@@ -48,6 +48,7 @@ PER_FILE = {
     "k_stencil4.hip": ["-ffp-contract=off"],
     "k_adi.hip": ["-ffp-contract=off"],
     "k_ppe.hip": ["-ffp-contract=off"],
+    "k_stats.hip": ["-ffp-contract=off"],
 }
 
 

@@ -17,6 +17,7 @@ handles returned from ``alloc``.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -38,6 +39,15 @@ def mesh_coords(nyc, yly, section_length, nyc_section, stret_type, stret_option,
                                 cs.ctypes.data_as(L._DP), out.ctypes.data_as(L._DP)),
             "c3d_mesh_coords")
     return out
+
+
+def restart_name(restart_dir, run_name, ntime):
+    """RestartDir//"RestartFor"//RunName//I10.10 (m_IOAndVisu.f90:363)"""
+    lib = L.load()
+    buf = ctypes.create_string_buffer(4096)
+    L.check(lib.c3d_restart_name(os.fsencode(restart_dir), os.fsencode(run_name), int(ntime),
+                                 buf, len(buf)), "c3d_restart_name")
+    return buf.value.decode()
 
 
 def channel_yp(nyc, yly, gamma):
@@ -167,6 +177,10 @@ class Channel:
     def set_dt(self, dt):
         L.check(self.lib.c3d_set_dt(self._h, float(dt)), "c3d_set_dt")
 
+    def set_ppe_refine(self, sweeps):
+        """-1: FP64 pressure solve; s >= 0: FP32 solve + s FP64-residual sweeps"""
+        L.check(self.lib.c3d_set_ppe_refine(self._h, int(sweeps)), "c3d_set_ppe_refine")
+
     # reference routines --------------------------------------------------------
     def _call(self, name, *fields):
         fn = getattr(self.lib, "c3d_" + name)
@@ -265,6 +279,44 @@ class Channel:
         keys = ("cfl", "divmax1", "divmax2", "vmax_x", "vmax_y", "vmax_z", "PrGradNow",
                 "PrGradAver", "SimTime")
         return dict(zip(keys, list(out)))
+
+    # restart files (m_IOAndVisu.f90:352-558) ----------------------------------
+    def write_restart(self, path, ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR, hist=None):
+        """hist: None or the three history fields (HistXOld, HistYOld, HistZOld)"""
+        h = [None] * 3 if hist is None else [self.ptr(f) for f in hist]
+        L.check(self.lib.c3d_write_restart(self._h, os.fsencode(path), self.ptr(ux),
+                                           self.ptr(uy), self.ptr(uz), self.ptr(pressure), *h),
+                "c3d_write_restart")
+
+    def read_restart(self, path, ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR, hist=None):
+        h = [None] * 3 if hist is None else [self.ptr(f) for f in hist]
+        L.check(self.lib.c3d_read_restart(self._h, os.fsencode(path), self.ptr(ux),
+                                          self.ptr(uy), self.ptr(uz), self.ptr(pressure), *h),
+                "c3d_read_restart")
+
+    # channel statistics (m_FlowType_Channel.f90:135-363) ----------------------
+    def clcStat(self, ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR):
+        self._call("clcStat_CH", ux, uy, uz, pressure)
+
+    def stat_get(self):
+        """(SumStat as a (35, nyc+1) array, nfstime, PrGradsum)"""
+        s = np.zeros((L.NCHASTAT, self.nyc + 1), order="F")
+        n = ctypes.c_int()
+        pg = ctypes.c_double()
+        L.check(self.lib.c3d_stat_get(self._h, L.dptr(s), ctypes.byref(n), ctypes.byref(pg)),
+                "c3d_stat_get")
+        return s, n.value, pg.value
+
+    def stat_reset(self):
+        L.check(self.lib.c3d_stat_reset(self._h), "c3d_stat_reset")
+
+    def write_stat(self, path, itime, ivstats):
+        L.check(self.lib.c3d_write_stat(self._h, os.fsencode(path), int(itime), int(ivstats)),
+                "c3d_write_stat")
+
+    def append_prgrad(self, path, itime):
+        L.check(self.lib.c3d_append_prgrad(self._h, os.fsencode(path), int(itime)),
+                "c3d_append_prgrad")
 
     # profiling ----------------------------------------------------------------
     def profile(self, on=True):

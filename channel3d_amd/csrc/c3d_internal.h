@@ -136,10 +136,12 @@ struct Coef4 {
   T pmGamma, pmTheta, pmAlpha, pmBeta;
 };
 
+enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1, YS_PARTITIONED = 2 };
+
 enum Stage {
   ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_XZ, ST_ADI_Y, ST_BC, ST_PRSRC,
   ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
-  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_COUNT
+  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_STAT, ST_RESID, ST_COUNT
 };
 
 }  // namespace c3d
@@ -180,8 +182,13 @@ struct c3d_ctx {
     double* d_ztaps;
     double* d_yc[3];          // CN ux/uz operator: a, vecm, arr (ny+2 each)
     double* d_yp[3];          // uy operator
+    double* d_spk[2];         // partitioned-solve tables (cell / node operator)
   } adi[3];
   void* d_tabs = nullptr;
+  // y solver (YS_*); partitioned: SPK_P lanes per line, spike_m rows per lane
+  int ymode = 0;
+  int spike_m = 0;
+  void* d_spike = nullptr;
 
   // state and scratch
   void* state[C3D_NSTATE] = {};
@@ -205,6 +212,24 @@ struct c3d_ctx {
   int last_cfl_slot = 0;
   void* staging = nullptr;    // pinned host staging buffer for upload/download
   size_t staging_bytes = 0;
+
+  // mixed-precision pressure solve (FP64 context): FP32 twiddles and metrics,
+  // the FP32 residual field; ppe_sweeps < 0 keeps the plain FP64 solve
+  int ppe_sweeps = -1;
+  void* d_tw32 = nullptr;
+  const void* tw32_x = nullptr;
+  const void* tw32_xr = nullptr;
+  const void* tw32_z = nullptr;
+  void* d_met32 = nullptr;
+  float* resid32 = nullptr;
+  double* d_mean64 = nullptr; // the FP64 (0,0)-mode solve: 3 ny work, gauge value at
+                              // [3 ny], gauge partial sums from [3 (ny+1)]
+
+  // channel statistics: SumStat(35, ny+1), PrGradsum (+7 spare), then the
+  // per-workgroup partial rows; allocated on first use
+  double* d_stat = nullptr;
+  int stat_kruns = 0;         // z runs (workgroups) per wall-normal plane
+  int stat_n = 0;             // nfstime
 
   // profiling
   bool prof = false;
@@ -252,6 +277,11 @@ int launch_diag(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int 
 int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot);
 
 // ---- launchers (k_adi.hip) ----
+// partitioned y solve: threads per line, rows per thread chosen from SPK_MS
+constexpr int SPK_P = 16;
+constexpr int SPK_NM = 6;
+constexpr int SPK_MS[SPK_NM] = {2, 4, 8, 16, 24, 32};
+__host__ __device__ constexpr int spike_table_len(int m) { return 5 * SPK_P * m + 4 * SPK_P * SPK_P; }
 // comp: 0 ux (CN metrics), 1 uy (node metrics + wall row), 2 uz (CN metrics).
 // add_dpmdxns: 2nd order adds dpmdxns to RhsX during the x sweep; 4th order
 // (y solve only) subtracts dp1ns from RhsX as the sweep loads it
@@ -274,6 +304,11 @@ int launch_bc4_vel(c3d_ctx* c, void* ux, void* uy, void* uz, const void* uy_ym, 
 // CheckDivergence with the 4th-order x/z differences -> DevScalars::divmax2
 int launch_div4(c3d_ctx* c, const void* ux, const void* uy, const void* uz);
 
+// ---- launchers (k_stats.hip) ----
+int launch_stat(c3d_ctx* c, const void* ux, const void* uy, const void* uz, const void* pr);
+// Fortran ES24.15 edit descriptor
+void fmt_es24_15(double v, char out[32]);
+
 // ---- launchers (k_fft.hip / k_ppe.hip) ----
 bool fft_size_supported(int n);
 int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec);
@@ -281,5 +316,11 @@ int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale);
 int launch_fft_z(c3d_ctx* c, void* spec, int inverse);
 int launch_tridiag(c3d_ctx* c, void* spec);
 int run_ppe(c3d_ctx* c, void* prsrc, void* phi);
+// FP32 transforms of the mixed-precision solve
+enum { FFT32_X_FWD_F64, FFT32_X_FWD_F32, FFT32_Z_FWD, FFT32_Z_INV, FFT32_X_INV_SET,
+       FFT32_X_INV_ADD };
+int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale);
+// FP32 twiddle and metric tables of an FP64 context (c3d_api.hip)
+int ensure_mixed_tables(c3d_ctx* c);
 
 }  // namespace c3d

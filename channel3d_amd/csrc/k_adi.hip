@@ -226,6 +226,239 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same Thomas sweeps without writing the forward pass to memory.  The
+// line is cut into segments of C rows.  The forward pass loads a segment's
+// right-hand side (C loads in flight), runs the recurrence and keeps only
+// the value at the segment's end, in LDS.  The backward pass walks the
+// segments from the top: it recomputes the segment's forward values from
+// its checkpoint (the same operations on the same operands, so the same
+// values), then substitutes backward and updates u.  The top segment's
+// forward values are still in registers and are not recomputed.  Traffic:
+// the right-hand side is read twice, u once, u written once (4 words per
+// point instead of 5); the results equal k_adi_y's bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, int C, bool WALLROW, bool SHIFT>
+__global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
+                                                  const double* __restrict__ tvecm,
+                                                  const double* __restrict__ tarr,
+                                                  const T* __restrict__ r, T* __restrict__ u,
+                                                  const T* __restrict__ duy,
+                                                  const DevScalars* __restrict__ sc) {
+  extern __shared__ unsigned char ck_raw[];
+  T* ck = (T*)ck_raw;  // [segment][256 threads]
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || k >= g.nz) return;
+  const int ny = g.ny;
+  const int nseg = (ny + C - 1) / C;
+  const long long base = (long long)i + g.plane * k;
+  const long long sj = g.nx;
+  const T shift = SHIFT ? (T)sc->dp1ns : T(0);
+  auto ld = [&](int j) { return SHIFT ? r[base + sj * j] - shift : r[base + sj * j]; };
+  T y[C];
+  T f = T(0);
+  // forward: segment s holds rows s*C+1 .. min(s*C+C, ny)
+  for (int s = 0; s < nseg; ++s) {
+    const int lo = s * C + 1;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = lo + q;
+      y[q] = j <= ny ? ld(j) : T(0);
+    }
+    if (s == 0 && WALLROW) y[0] = duy[i + (long long)g.nx * k];
+    if (s == 0) {
+      f = y[0] / (T)tvecm[1];
+      y[0] = f;
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = lo + q;
+      if ((s > 0 || q > 0) && j <= ny) {
+        f = (y[q] - (T)ta[j] * f) / (T)tvecm[j];
+        y[q] = f;
+      }
+    }
+    if (s + 1 < nseg) ck[s * 256 + tid] = f;  // forward value at row (s+1)*C
+  }
+  // backward; y holds the top segment's forward values
+  T x = f;
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int lo = s * C + 1;
+    if (s < nseg - 1) {
+      T fp = s > 0 ? ck[(s - 1) * 256 + tid] : T(0);
+#pragma unroll
+      for (int q = 0; q < C; ++q) y[q] = ld(lo + q);  // a full segment: lo + C - 1 < ny
+      if (s == 0) {
+        if (WALLROW) y[0] = duy[i + (long long)g.nx * k];
+        fp = y[0] / (T)tvecm[1];
+        y[0] = fp;
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        if (s > 0 || q > 0) {
+          fp = (y[q] - (T)ta[lo + q] * fp) / (T)tvecm[lo + q];
+          y[q] = fp;
+        }
+      }
+    }
+    T uv[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = lo + q;
+      uv[q] = j <= ny ? u[base + sj * j] : T(0);
+    }
+#pragma unroll
+    for (int q = C - 1; q >= 0; --q) {
+      const int j = lo + q;
+      if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
+      if (j <= ny) u[base + sj * j] = uv[q] + x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same y solve, every line read once and written once.  Every (i, k)
+// line shares the matrix, so the line is split over SPK_P threads of M rows:
+// each thread solves its partition with the partition's own LU factors, the
+// threads publish their first and last values in LDS, the 2P boundary
+// unknowns follow from the precomputed inverse of the reduced system, and
+// each thread adds its spike corrections (x = y - x_left v - x_right w)
+// before the update u += x.  Rows stay in registers; the partial solution
+// never goes back to memory.  Tables (host-built, shared by all lines): a,
+// vecm, arr of the partition LUs, the spikes v and w, and the reduced
+// inverse.  A workgroup holds 16 lines (consecutive i, one 128-byte segment
+// per row) x 16 partitions.
+// ---------------------------------------------------------------------------
+template <typename T, int M, bool WALLROW, bool SHIFT>
+__global__ __launch_bounds__(256) void k_adi_y_spk(Geom g, const double* __restrict__ tab,
+                                                   const T* __restrict__ r, T* __restrict__ u,
+                                                   const T* __restrict__ duy,
+                                                   const DevScalars* __restrict__ sc) {
+  constexpr int P = SPK_P, PM = P * M, R = 2 * P;
+  __shared__ double st[spike_table_len(M)];
+  __shared__ T xb[R][16];
+  for (int q = threadIdx.x; q < spike_table_len(M); q += blockDim.x) st[q] = tab[q];
+  const double* ta = st;
+  const double* tv = st + PM;
+  const double* tr = st + 2 * PM;
+  const double* sv = st + 3 * PM;
+  const double* sw = st + 4 * PM;
+  const double* mi = st + 5 * PM;
+  const int l = threadIdx.x & 15, p = threadIdx.x >> 4;
+  const int lpk = (g.nx + 15) >> 4;                      // 16-line groups per k plane
+  const int k = blockIdx.x / lpk;
+  const int i = (blockIdx.x - k * lpk) * 16 + l;
+  const bool act = i < g.nx && k < g.nz;
+  const int ny = g.ny;
+  const long long base = (long long)(act ? i : 0) + g.plane * (long long)(act ? k : 0);
+  const long long sj = g.nx;
+  const int j0 = p * M + 1;                              // reference row of q = 0
+  const T shift = SHIFT ? (T)sc->dp1ns : T(0);
+  T y[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const int j = j0 + q;
+    y[q] = (act && j <= ny) ? (SHIFT ? r[base + sj * j] - shift : r[base + sj * j]) : T(0);
+  }
+  if (WALLROW && p == 0) y[0] = act ? duy[i + (long long)g.nx * k] : T(0);
+  __syncthreads();                                       // tables in LDS
+  // partition LU: forward and backward substitution (InverseTridiagonal's
+  // operation order within the partition)
+  y[0] = y[0] / (T)tv[p * M];
+#pragma unroll
+  for (int q = 1; q < M; ++q) y[q] = (y[q] - (T)ta[p * M + q] * y[q - 1]) / (T)tv[p * M + q];
+#pragma unroll
+  for (int q = M - 2; q >= 0; --q) y[q] = y[q] - (T)tr[p * M + q + 1] * y[q + 1];
+  xb[2 * p][l] = y[0];
+  xb[2 * p + 1][l] = y[M - 1];
+  __syncthreads();
+  T xl = T(0), xr = T(0);
+  if (p > 0) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) xl += (T)mi[(2 * p - 1) * R + m] * xb[m][l];
+  }
+  if (p < P - 1) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) xr += (T)mi[(2 * p + 2) * R + m] * xb[m][l];
+  }
+  if (!act) return;
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const int j = j0 + q;
+    if (j <= ny) {
+      const T x = y[q] - xl * (T)sv[p * M + q] - xr * (T)sw[p * M + q];
+      u[base + sj * j] = u[base + sj * j] + x;
+    }
+  }
+}
+
+template <typename T, bool WALLROW, bool SHIFT>
+static bool launch_y_spk(c3d_ctx* c, const double* tab, const void* rhs, void* u,
+                         const void* duy) {
+  const Geom& g = c->g;
+  const long long groups = (long long)((g.nx + 15) / 16) * g.nz;
+  const dim3 grid((unsigned)groups), blk(256);
+  const DevScalars* sc = (const DevScalars*)c->d_sc;
+  switch (c->spike_m) {
+#define SPK(MM)                                                                                 \
+  case MM:                                                                                      \
+    hipLaunchKernelGGL((k_adi_y_spk<T, MM, WALLROW, SHIFT>), grid, blk, 0, c->stream, g, tab,   \
+                       (const T*)rhs, (T*)u, (const T*)duy, sc);                                \
+    return true;
+    SPK(2) SPK(4) SPK(8) SPK(16) SPK(24) SPK(32)
+#undef SPK
+    default: return false;
+  }
+}
+
+// y solve + update of one component; the partitioned kernel when tables exist
+template <typename T>
+static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, void* u,
+                   const void* duy_ym, bool shift) {
+  const Geom& g = c->g;
+  if (c->spike_m > 0) {
+    const double* tab = tb.d_spk[comp == 1 ? 1 : 0];
+    bool ok = comp == 1 ? launch_y_spk<T, true, false>(c, tab, rhs, u, duy_ym)
+              : shift   ? launch_y_spk<T, false, true>(c, tab, rhs, u, nullptr)
+                        : launch_y_spk<T, false, false>(c, tab, rhs, u, nullptr);
+    if (ok) {
+      C3D_HIP(hipGetLastError());
+      return C3D_OK;
+    }
+  }
+  dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
+  double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
+  const DevScalars* sc = (const DevScalars*)c->d_sc;
+  constexpr int CSEG = 32;
+  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * 256 * sizeof(T);
+  if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
+    if (comp == 1)
+      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, true, false>), gy, by, ck_bytes, c->stream, g,
+                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc);
+    else if (shift)
+      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, true>), gy, by, ck_bytes, c->stream, g,
+                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+    else
+      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, false>), gy, by, ck_bytes, c->stream, g,
+                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+    C3D_HIP(hipGetLastError());
+    return C3D_OK;
+  }
+  if (comp == 1)
+    hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                       (T*)rhs, (T*)u, (const T*)duy_ym, sc);
+  else if (shift)
+    hipLaunchKernelGGL((k_adi_y<T, false, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                       (T*)rhs, (T*)u, (const T*)nullptr, sc);
+  else
+    hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
+                       (T*)rhs, (T*)u, (const T*)nullptr, sc);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
 static inline dim3 cgrid(const Geom& g) {
   return dim3((g.nx + ABX - 1) / ABX, (g.ny + ABY - 1) / ABY, (g.nz + AKCH - 1) / AKCH);
 }
@@ -273,23 +506,12 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
   const auto& tb = c->adi[ns - 1];
   const double N = (double)g.nx * g.ny * g.nz;
   hipEvent_t ev;
+  int rc;
   // 4th order: Crank-Nicolson in y only, the y sweep reads the right-hand side
   // in place (src/CFD_4th/m_TScheme.f90:443-549)
   if (c->order == 4) {
     prof_begin(c, ST_ADI_Y, &ev);
-    dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
-    double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
-    const DevScalars* sc = (const DevScalars*)c->d_sc;
-    if (comp == 1)
-      hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)rhs, (T*)u, (const T*)duy_ym, sc);
-    else if (add)
-      hipLaunchKernelGGL((k_adi_y<T, false, true>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)rhs, (T*)u, (const T*)nullptr, sc);
-    else
-      hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)rhs, (T*)u, (const T*)nullptr, sc);
-    C3D_HIP(hipGetLastError());
+    if ((rc = y_solve<T>(c, tb, comp, rhs, u, duy_ym, add != 0))) return rc;
     prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
     return C3D_OK;
   }
@@ -301,15 +523,7 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
     C3D_HIP(hipGetLastError());
     prof_end(c, ST_ADI_XZ, ev, 2.0 * N * c->esize);
     prof_begin(c, ST_ADI_Y, &ev);
-    dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
-    double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
-    if (comp == 1)
-      hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)c->tmpA, (T*)u, (const T*)duy_ym, (const DevScalars*)c->d_sc);
-    else
-      hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                         (T*)c->tmpA, (T*)u, (const T*)nullptr, (const DevScalars*)c->d_sc);
-    C3D_HIP(hipGetLastError());
+    if ((rc = y_solve<T>(c, tb, comp, c->tmpA, u, duy_ym, false))) return rc;
     prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
     return C3D_OK;
   }
@@ -333,15 +547,7 @@ static int adi_t(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* d
   prof_end(c, ST_ADI_Z, ev, 2.0 * N * c->esize);
   // y sweep + update
   prof_begin(c, ST_ADI_Y, &ev);
-  dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
-  double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
-  if (comp == 1)
-    hipLaunchKernelGGL((k_adi_y<T, true, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                       (T*)rhs, (T*)u, (const T*)duy_ym, (const DevScalars*)c->d_sc);
-  else
-    hipLaunchKernelGGL((k_adi_y<T, false, false>), gy, by, 0, c->stream, g, yt[0], yt[1], yt[2],
-                       (T*)rhs, (T*)u, (const T*)nullptr, (const DevScalars*)c->d_sc);
-  C3D_HIP(hipGetLastError());
+  if ((rc = y_solve<T>(c, tb, comp, rhs, u, duy_ym, false))) return rc;
   prof_end(c, ST_ADI_Y, ev, 3.0 * N * c->esize);
   return C3D_OK;
 }

@@ -201,12 +201,15 @@ constexpr int ZTK = 8;
 // ---------------------------------------------------------------------------
 // x forward: field rows (jj = 1..ny) -> spectral rows, kx = 0..N (N = nx/2)
 // ---------------------------------------------------------------------------
-template <typename T, int N>
-__global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const T* __restrict__ in,
+// TIN: precision of the field read (the mixed-precision solve transforms an
+// FP64 right-hand side in FP32)
+template <typename T, int N, typename TIN = T>
+__global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict__ in,
                                                    typename CT<T>::t* __restrict__ spec,
                                                    const typename CT<T>::t* __restrict__ tw,
                                                    const typename CT<T>::t* __restrict__ twr) {
   using T2 = typename CT<T>::t;
+  using TI2 = typename CT<TIN>::t;
   constexpr int LINES = xlines(N);
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
@@ -218,8 +221,9 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const T* __restrict__
     T2 v = mk<T2>(0, 0);
     if (row < nrows) {
       const int j0 = row % g.ny, k = row / g.ny;
-      const T2* rp = (const T2*)(in + (long long)g.nx * (j0 + 1) + g.plane * k);
-      v = rp[m];
+      const TI2* rp = (const TI2*)(in + (long long)g.nx * (j0 + 1) + g.plane * k);
+      const TI2 w = rp[m];
+      v = mk<T2>((T)w.x, (T)w.y);
     }
     sm[l * LDN + m] = v;
   }
@@ -241,13 +245,16 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const T* __restrict__
 // ---------------------------------------------------------------------------
 // x inverse: spectral rows -> field rows (jj = 1..ny), times scale
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+// TOUT: precision of the field written; ACC: add to it (the correction of a
+// refinement sweep) instead of overwriting it
+template <typename T, int N, typename TOUT = T, bool ACC = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>::t* __restrict__ spec,
-                                                   T* __restrict__ out,
+                                                   TOUT* __restrict__ out,
                                                    const typename CT<T>::t* __restrict__ tw,
                                                    const typename CT<T>::t* __restrict__ twr,
-                                                   T scale) {
+                                                   TOUT scale) {
   using T2 = typename CT<T>::t;
+  using TO2 = typename CT<TOUT>::t;
   constexpr int LINES = xlines(N);
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
@@ -279,9 +286,14 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
     const int l = e / N, m = e - l * N, row = row0 + l;
     if (row >= nrows) continue;
     const int j0 = row % g.ny, k = row / g.ny;
-    T2* rp = (T2*)(out + (long long)g.nx * (j0 + 1) + g.plane * k);
+    TO2* rp = (TO2*)(out + (long long)g.nx * (j0 + 1) + g.plane * k);
     const T2 z = sm[l * LDN + m];
-    rp[m] = mk<T2>(z.x * scale, z.y * scale);
+    if (ACC) {
+      const TO2 o = rp[m];
+      rp[m] = mk<TO2>(o.x + (TOUT)z.x * scale, o.y + (TOUT)z.y * scale);
+    } else {
+      rp[m] = mk<TO2>((TOUT)z.x * scale, (TOUT)z.y * scale);
+    }
   }
 }
 
@@ -392,6 +404,95 @@ static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
     default: set_error("fft_z: unsupported length"); return C3D_ERR_UNSUPPORTED;
   }
   C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+// ---- FP32 transforms of the mixed-precision solve (FP32 twiddle tables) --
+template <typename TIN>
+static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec) {
+  const Geom& g = c->g;
+  const int nrows = g.ny * g.nz;
+  switch (g.nk) {
+#define X(n_)                                                                                    \
+  case n_:                                                                                       \
+    hipLaunchKernelGGL((k_fft_x_fwd<float, n_, TIN>), dim3((nrows + xlines(n_) - 1) / xlines(n_)), \
+                       dim3(FNT), 0, c->stream, g, in, (float2*)spec, (const float2*)c->tw32_x,   \
+                       (const float2*)c->tw32_xr);                                               \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <bool ACC>
+static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale) {
+  const Geom& g = c->g;
+  const int nrows = g.ny * g.nz;
+  switch (g.nk) {
+#define X(n_)                                                                                    \
+  case n_:                                                                                       \
+    hipLaunchKernelGGL((k_fft_x_inv<float, n_, double, ACC>),                                   \
+                       dim3((nrows + xlines(n_) - 1) / xlines(n_)), dim3(FNT), 0, c->stream, g,  \
+                       (const float2*)spec, out, (const float2*)c->tw32_x,                       \
+                       (const float2*)c->tw32_xr, scale);                                        \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+static int fft_z32(c3d_ctx* c, void* spec, int inverse) {
+  const Geom& g = c->g;
+  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny);
+  switch (g.nz) {
+#define X(n_)                                                                                  \
+  case n_:                                                                                     \
+    if (inverse)                                                                               \
+      hipLaunchKernelGGL((k_fft_z<float, n_, 1>), grid, dim3(FNT), 0, c->stream, g,           \
+                         (float2*)spec, (const float2*)c->tw32_z);                             \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_fft_z<float, n_, -1>), grid, dim3(FNT), 0, c->stream, g,          \
+                         (float2*)spec, (const float2*)c->tw32_z);                             \
+    break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_z: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale) {
+  hipEvent_t ev;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  int rc = C3D_OK, st = ST_MISC;
+  double bytes = 0;
+  switch (what) {
+    case FFT32_X_FWD_F64: st = ST_FFT_X_FWD; bytes = N * (8 + 4); break;
+    case FFT32_X_FWD_F32: st = ST_FFT_X_FWD; bytes = N * (4 + 4); break;
+    case FFT32_Z_FWD: st = ST_FFT_Z_FWD; bytes = N * 8; break;
+    case FFT32_Z_INV: st = ST_FFT_Z_INV; bytes = N * 8; break;
+    case FFT32_X_INV_SET: st = ST_FFT_X_INV; bytes = N * (4 + 8); break;
+    case FFT32_X_INV_ADD: st = ST_FFT_X_INV; bytes = N * (4 + 16); break;
+  }
+  prof_begin(c, st, &ev);
+  switch (what) {
+    case FFT32_X_FWD_F64: rc = fft_x_fwd32<double>(c, (const double*)in, spec); break;
+    case FFT32_X_FWD_F32: rc = fft_x_fwd32<float>(c, (const float*)in, spec); break;
+    case FFT32_Z_FWD: rc = fft_z32(c, spec, 0); break;
+    case FFT32_Z_INV: rc = fft_z32(c, spec, 1); break;
+    case FFT32_X_INV_SET: rc = fft_x_inv32<false>(c, spec, out, scale); break;
+    case FFT32_X_INV_ADD: rc = fft_x_inv32<true>(c, spec, out, scale); break;
+    default: set_error("fft32: bad selector"); return C3D_ERR_ARG;
+  }
+  if (rc) return rc;
+  prof_end(c, st, ev, bytes);
   return C3D_OK;
 }
 

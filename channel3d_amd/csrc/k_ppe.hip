@@ -123,14 +123,139 @@ __global__ __launch_bounds__(256) void k_tridiag(Geom g, Metrics<T> m,
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same elimination without storing the factors and the forward pass.
+// The line is cut into segments of C rows.  The forward pass keeps, per
+// segment end, the running value f and pivot vecm (three reals per line and
+// segment, in the spectral scratch array).  The backward pass walks the
+// segments from the top: it reruns the segment's forward recurrence from
+// its checkpoint -- the same operations on the same operands, hence the same
+// pivots, factors and values -- then substitutes backward and stores the
+// solution.  The top segment's values are still in registers.  Traffic per
+// complex element: read, read again, write (48 B in FP64) instead of 80 B;
+// results equal k_tridiag's bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, int C>
+__device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T>& m,
+                                                  typename CT2<T>::t* __restrict__ spec,
+                                                  T* __restrict__ ckp) {
+  using T2 = typename CT2<T>::t;
+  const int kx = blockIdx.x * 64 + threadIdx.x;
+  const int kz = blockIdx.y * 4 + threadIdx.y;
+  if (kx > g.nk || kz >= g.nz) return;
+  const int ny = g.ny;
+  const int nseg = (ny + C - 1) / C;
+  const long long P = g.P;
+  const long long base = kx + P * (long long)ny * kz;
+  const long long cs = P * (long long)g.nz;             // checkpoint plane stride
+  T* ck = ckp + kx + P * (long long)kz;
+  const T lx = m.wavex[kx], lz = m.wavez[kz];
+  const bool pin = (kx == 0 && kz == 0);
+  T2 y[C];
+  T ar[C];
+  T vecm = T(1), cprev = T(0);
+  T2 f;
+  f.x = 0;
+  f.y = 0;
+  // one segment's forward recurrence from (f, vecm, cprev); rows lo..lo+C-1
+  auto run = [&](int lo) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j0 = lo + q;
+      if (j0 < ny) {
+        if (j0 == 0) {
+          vecm = pin ? T(1) : (m.ac2Pr[1] + lx) + lz;
+          cprev = pin ? T(0) : m.ap2Pr[1];
+          T2 fj = y[q];
+          if (pin) {
+            fj.x = 0;
+            fj.y = 0;
+          }
+          f.x = fj.x / vecm;
+          f.y = fj.y / vecm;
+          ar[q] = T(0);
+        } else {
+          const int jr = j0 + 1;
+          const T arr = cprev / vecm;
+          const T a = m.am2Pr[jr];
+          vecm = ((m.ac2Pr[jr] + lx) + lz) - a * arr;
+          f.x = (y[q].x - a * f.x) / vecm;
+          f.y = (y[q].y - a * f.y) / vecm;
+          cprev = m.ap2Pr[jr];
+          ar[q] = arr;
+        }
+        y[q] = f;
+      }
+    }
+  };
+  for (int s = 0; s < nseg; ++s) {
+    const int lo = s * C;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (lo + q < ny) y[q] = spec[base + P * (lo + q)];
+    }
+    run(lo);
+    if (s + 1 < nseg) {
+      ck[cs * (3 * s + 0)] = f.x;
+      ck[cs * (3 * s + 1)] = f.y;
+      ck[cs * (3 * s + 2)] = vecm;
+    }
+  }
+  // backward; y and ar hold the top segment
+  T2 x = f;
+  T arnext = T(0);
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int lo = s * C;
+    if (s < nseg - 1) {
+#pragma unroll
+      for (int q = 0; q < C; ++q) y[q] = spec[base + P * (lo + q)];  // full segment
+      if (s > 0) {
+        f.x = ck[cs * (3 * (s - 1) + 0)];
+        f.y = ck[cs * (3 * (s - 1) + 1)];
+        vecm = ck[cs * (3 * (s - 1) + 2)];
+        cprev = m.ap2Pr[lo];
+      }
+      run(lo);
+    }
+#pragma unroll
+    for (int q = C - 1; q >= 0; --q) {
+      const int j0 = lo + q;
+      if (j0 < ny) {
+        if (j0 < ny - 1) {
+          const T a1 = q + 1 < C ? ar[q + 1] : arnext;
+          x.x = y[q].x - a1 * x.x;
+          x.y = y[q].y - a1 * x.y;
+        }
+        spec[base + P * j0] = x;
+      }
+    }
+    arnext = ar[0];
+  }
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(256) void k_tridiag_ck(Geom g, Metrics<T> m,
+                                                    typename CT2<T>::t* __restrict__ spec,
+                                                    T* __restrict__ ckp) {
+  k_tridiag_ck_body<T, C>(g, m, spec, ckp);
+}
+
 template <typename T>
 static int tridiag_t(c3d_ctx* c, void* spec) {
   const Geom& g = c->g;
   dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4), blk(64, 4);
   const Metrics<T>& m = *(const Metrics<T>*)(c->prec == C3D_FP64 ? (const void*)&c->md
                                                                   : (const void*)&c->mf);
-  hipLaunchKernelGGL(k_tridiag<T>, grid, blk, 0, c->stream, g, m,
-                     (typename CT2<T>::t*)spec, (T*)c->sarr);
+  constexpr int CSEG = 16;
+  // checkpoints: 3 reals per line and segment, in the spectral scratch
+  const long long need = 3LL * ((g.ny + CSEG - 1) / CSEG) * g.P * g.nz;
+  if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
+    hipLaunchKernelGGL((k_tridiag_ck<T, CSEG>), grid, blk, 0, c->stream, g, m,
+                       (typename CT2<T>::t*)spec, (T*)c->sarr);
+  } else {
+    hipLaunchKernelGGL(k_tridiag<T>, grid, blk, 0, c->stream, g, m,
+                       (typename CT2<T>::t*)spec, (T*)c->sarr);
+  }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
 }
@@ -145,11 +270,164 @@ int launch_tridiag(c3d_ctx* c, void* spec) {
   return C3D_OK;
 }
 
+// ===========================================================================
+// Mixed-precision solve: FP32 transforms and y solves, FP64 residual of the
+// discrete operator, FP32 correction solves added to the FP64 solution.
+// ===========================================================================
+
+// The (0,0) mode in FP64: the Neumann y operator with the gauge row is the
+// ill-conditioned system of the solve (its FP32 error would bound what the
+// refinement can reach).  One thread; w holds 3 ny doubles, then the gauge
+// value at w[3 ny]: the first row is phi(0,0,1) = w[3 ny] (0 in the first
+// solve, as the reference pins it; in a correction solve, minus the current
+// solution's value, so the refined solution keeps the reference's gauge --
+// without it the FP32 rounding of the first solve's plane mean at j = 1
+// would survive every sweep as a constant offset).
+template <typename T2>
+__device__ void mean_mode_fp64(const Geom& g, const Metrics<double>& m, T2* spec, double* w) {
+  const int ny = g.ny;
+  const long long P = g.P;
+  double vecm = 1.0, cprev = 0.0, fx = w[3 * ny], fy = 0.0;  // row 0: the gauge row
+  w[0] = fx;
+  w[ny] = 0.0;
+  w[2 * ny] = 0.0;
+  for (int j0 = 1; j0 < ny; ++j0) {
+    const int jr = j0 + 1;
+    const double arr = cprev / vecm;
+    const double a = m.am2Pr[jr];
+    vecm = ((m.ac2Pr[jr] + m.wavex[0]) + m.wavez[0]) - a * arr;
+    const T2 fj = spec[P * j0];
+    fx = ((double)fj.x - a * fx) / vecm;
+    fy = ((double)fj.y - a * fy) / vecm;
+    w[j0] = fx;
+    w[ny + j0] = fy;
+    w[2 * ny + j0] = arr;
+    cprev = m.ap2Pr[jr];
+  }
+  T2 o;
+  o.x = fx;
+  o.y = fy;
+  spec[P * (ny - 1)] = o;
+  for (int j0 = ny - 2; j0 >= 0; --j0) {
+    fx = w[j0] - w[2 * ny + j0 + 1] * fx;
+    fy = w[ny + j0] - w[2 * ny + j0 + 1] * fy;
+    o.x = fx;
+    o.y = fy;
+    spec[P * j0] = o;
+  }
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(256) void k_tridiag_mixed(Geom g, Metrics<T> m, Metrics<double> md,
+                                                       typename CT2<T>::t* __restrict__ spec,
+                                                       T* __restrict__ ckp, double* w) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    mean_mode_fp64(g, md, spec, w);
+    return;
+  }
+  // every other mode: the checkpointed elimination (the (0,0) thread left)
+  k_tridiag_ck_body<T, C>(g, m, spec, ckp);
+}
+
+// r = f - L phi in FP64, L the discrete operator the spectral solve inverts:
+// 3-point x and z second differences and the Neumann y operator am2Pr/ac2Pr/
+// ap2Pr (whose wall entries are zero, so wall ghosts are never read)
+// The workgroups holding the plane j = 1 also write the sum of phi over
+// their 64 x-points to part[blockIdx.z * gridDim.x + blockIdx.x] (the gauge).
+template <typename TR>
+__global__ __launch_bounds__(256) void k_ppe_resid(Geom g, Metrics<double> m, double rdx2,
+                                                   double rdz2, const double* __restrict__ f,
+                                                   const double* __restrict__ phi,
+                                                   TR* __restrict__ r, double* __restrict__ part) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int jc = blockIdx.y * 4 + threadIdx.y + 1;
+  const int k = blockIdx.z;
+  if (blockIdx.y == 0 && threadIdx.y == 0) {
+    // wavefront 0 of the block: row j = 1
+    double v = i < g.nx ? phi[(long long)g.nx + g.plane * k + i] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (threadIdx.x == 0) part[(long long)blockIdx.z * gridDim.x + blockIdx.x] = v;
+  }
+  if (i >= g.nx || jc > g.ny) return;
+  const long long row = (long long)g.nx * jc + g.plane * k;
+  const long long c = row + i;
+  const int im = i == 0 ? g.nx - 1 : i - 1, ip = i == g.nx - 1 ? 0 : i + 1;
+  const int km = k == 0 ? g.nz - 1 : k - 1, kp = k == g.nz - 1 ? 0 : k + 1;
+  const double pc = phi[c];
+  double lap = (phi[row + ip] - 2.0 * pc + phi[row + im]) * rdx2 +
+               (phi[c + g.plane * (kp - k)] - 2.0 * pc + phi[c + g.plane * (km - k)]) * rdz2 +
+               m.ac2Pr[jc] * pc;
+  if (jc > 1) lap += m.am2Pr[jc] * phi[c - g.nx];
+  if (jc < g.ny) lap += m.ap2Pr[jc] * phi[c + g.nx];
+  r[c] = (TR)(f[c] - lap);
+}
+
+// w[3 ny] = -(sum of the partials, in index order): the gauge value of the
+// next correction solve; 0 resets it to the reference's pin
+__global__ void k_gauge(const double* __restrict__ part, int n, double* w, int ny, int reset) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double a = 0.0;
+  for (int q = t; q < n; q += 256) a += part[q];
+  sh[t] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) sh[t] += sh[t + h];
+    __syncthreads();
+  }
+  if (t == 0) w[3 * ny] = reset ? 0.0 : -sh[0];
+}
+
+static int run_ppe_mixed(c3d_ctx* c, void* prsrc, void* phi, int sweeps) {
+  int rc = ensure_mixed_tables(c);
+  if (rc) return rc;
+  const Geom& g = c->g;
+  void* spec = c->spec;   // the FP64 spectral array, used as FP32 storage
+  auto solve32 = [&]() -> int {
+    int r;
+    if ((r = launch_fft32(c, FFT32_Z_FWD, nullptr, spec, nullptr, 0))) return r;
+    hipEvent_t ev;
+    prof_begin(c, ST_TRIDIAG, &ev);
+    constexpr int CSEG = 16;
+    dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4), blk(64, 4);
+    hipLaunchKernelGGL((k_tridiag_mixed<float, CSEG>), grid, blk, 0, c->stream, g, c->mf, c->md,
+                       (float2*)spec, (float*)c->sarr, c->d_mean64);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_TRIDIAG, ev, 2.0 * (double)g.nx * g.ny * g.nz * 4);
+    return launch_fft32(c, FFT32_Z_INV, nullptr, spec, nullptr, 0);
+  };
+  double* gpart = c->d_mean64 + 3 * ((size_t)g.ny + 1);
+  const int ngp = ((g.nx + 63) / 64) * g.nz;
+  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, c->stream, gpart, 0, c->d_mean64, g.ny, 1);
+  C3D_HIP(hipGetLastError());
+  if ((rc = launch_fft32(c, FFT32_X_FWD_F64, prsrc, spec, nullptr, 0))) return rc;
+  if ((rc = solve32())) return rc;
+  if ((rc = launch_fft32(c, FFT32_X_INV_SET, nullptr, spec, (double*)phi, c->normfft))) return rc;
+  for (int s = 0; s < sweeps; ++s) {
+    hipEvent_t ev;
+    prof_begin(c, ST_RESID, &ev);
+    dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), blk(64, 4);
+    hipLaunchKernelGGL(k_ppe_resid<float>, grid, blk, 0, c->stream, g, c->md, c->rdx2, c->rdz2,
+                       (const double*)prsrc, (const double*)phi, c->resid32, gpart);
+    C3D_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, c->stream, gpart, ngp, c->d_mean64, g.ny,
+                       0);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_RESID, ev, (double)g.nx * g.ny * g.nz * (8 + 8 + 4));
+    if ((rc = launch_fft32(c, FFT32_X_FWD_F32, c->resid32, spec, nullptr, 0))) return rc;
+    if ((rc = solve32())) return rc;
+    if ((rc = launch_fft32(c, FFT32_X_INV_ADD, nullptr, spec, (double*)phi, c->normfft)))
+      return rc;
+  }
+  return C3D_OK;
+}
+
 // clcPPE_x1: forward x, forward z, y solves, inverse z, inverse x * normfft.
 // prsrc is read only (the reference overwrites it with unscaled phi; callers
 // of this library never read it back).
 int run_ppe(c3d_ctx* c, void* prsrc, void* phi) {
   int rc;
+  if (c->ppe_sweeps >= 0) return run_ppe_mixed(c, prsrc, phi, c->ppe_sweeps);
   if ((rc = launch_fft_x_fwd(c, prsrc, c->spec))) return rc;
   if ((rc = launch_fft_z(c, c->spec, 0))) return rc;
   if ((rc = launch_tridiag(c, c->spec))) return rc;

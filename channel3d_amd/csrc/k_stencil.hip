@@ -7,6 +7,8 @@
 // ghost planes exactly as the reference reads its halo arrays.
 #include "c3d_internal.h"
 
+#include <cstdlib>
+
 namespace c3d {
 
 constexpr int BX = 64, BY = 4, KCH = 8;
@@ -92,20 +94,20 @@ __device__ __forceinline__ void block_sum_fix(long long* v, long long (*sh)[4]) 
 // y neighbours of the current plane come from the tile's other threads
 // through the caches.  Expressions keep the reference's operation order.
 // ---------------------------------------------------------------------------
-template <typename T, int MASK>
-__global__ __launch_bounds__(BX* BY) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
+template <typename T, int MASK, int BYR>
+__global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
                                                 const T* __restrict__ ux, const T* __restrict__ uy,
                                                 const T* __restrict__ uz, const T* __restrict__ pr,
                                                 T* __restrict__ rx, T* __restrict__ ry,
                                                 T* __restrict__ rz, T* __restrict__ hx,
                                                 T* __restrict__ hy, T* __restrict__ hz,
                                                 double* __restrict__ partials, int kchunk) {
-  __shared__ long long shf[8][4];
-  const int cgx = (g.nx + BX - 1) / BX, cgy = (g.ny + BY - 1) / BY;
+  __shared__ long long shf[BYR][4];
+  const int cgx = (g.nx + BX - 1) / BX, cgy = (g.ny + BYR - 1) / BYR;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int cbx = lb % cgx, cby = (lb / cgx) % cgy, cbz = lb / (cgx * cgy);
   const int i = cbx * BX + threadIdx.x;
-  const int jc = cby * BY + threadIdx.y + 1;  // reference j
+  const int jc = cby * BYR + threadIdx.y + 1;  // reference j
   const int k0 = cbz * kchunk;
   long long s1[4] = {0, 0, 0, 0};
   if (i < g.nx && jc <= g.ny) {
@@ -530,8 +532,19 @@ const Metrics<float>& mets<float>(c3d_ctx* c) { return c->mf; }
 
 // z run of the RHS kernel: long enough to amortise the two lead-in planes,
 // short enough to give the device a few thousand workgroups
+// rows of j per RHS workgroup: the y-neighbour rows of a tile are re-read
+// by the tiles above and below, so taller tiles read less twice
+static int rhs_rows() {
+  static int rows = [] {
+    const char* e = std::getenv("C3D_RHS_ROWS");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 16) ? v : 4;
+  }();
+  return rows;
+}
+
 static int rhs_kchunk(const Geom& g) {
-  const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
+  const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + rhs_rows() - 1) / rhs_rows());
   int nch = (4096 + tiles - 1) / tiles;
   if (nch < 1) nch = 1;
   int kc = (g.nz + nch - 1) / nch;
@@ -540,7 +553,7 @@ static int rhs_kchunk(const Geom& g) {
 }
 static int rhs_nblocks(const Geom& g) {
   const int kc = rhs_kchunk(g);
-  return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + kc - 1) / kc);
+  return ((g.nx + BX - 1) / BX) * ((g.ny + rhs_rows() - 1) / rhs_rows()) * ((g.nz + kc - 1) / kc);
 }
 
 template <typename T>
@@ -548,21 +561,29 @@ static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const voi
                  const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz) {
   const Geom& g = c->g;
   const int kc = rhs_kchunk(g);
-  dim3 grid(rhs_nblocks(g)), blk(BX, BY);
+  const int rows = rhs_rows();
+  dim3 grid(rhs_nblocks(g)), blk(BX, rows);
   Coef<T> cf = make_coef<T>(c);
   const Metrics<T>& m = mets<T>(c);
   auto U = [](const void* p) { return (const T*)p; };
   auto W = [](void* p) { return (T*)p; };
-#define RHS_LAUNCH(M)                                                                         \
-  hipLaunchKernelGGL((k_rhs<T, M>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz),  \
+#define RHS_LAUNCH(M, R)                                                                       \
+  hipLaunchKernelGGL((k_rhs<T, M, R>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz), \
                      U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials, kc)
+#define RHS_ROWS(M)                         \
+  switch (rows) {                           \
+    case 8: RHS_LAUNCH(M, 8); break;        \
+    case 16: RHS_LAUNCH(M, 16); break;      \
+    default: RHS_LAUNCH(M, 4); break;       \
+  }
   switch (mask) {
-    case 1: RHS_LAUNCH(1); break;
-    case 2: RHS_LAUNCH(2); break;
-    case 4: RHS_LAUNCH(4); break;
-    case 7: RHS_LAUNCH(7); break;
+    case 1: RHS_ROWS(1); break;
+    case 2: RHS_ROWS(2); break;
+    case 4: RHS_ROWS(4); break;
+    case 7: RHS_ROWS(7); break;
     default: set_error("rhs: bad component mask"); return C3D_ERR_ARG;
   }
+#undef RHS_ROWS
 #undef RHS_LAUNCH
   C3D_HIP(hipGetLastError());
   return C3D_OK;

@@ -5,11 +5,14 @@
 !   1. one RK substep (ns=1) with the per-routine wrappers on host arrays,
 !      in the reference's call order and with its scratch aliasing
 !      (RhsX/prsrc => RealArr1, RhsY => RealArr2, RhsZ/prphiHalo => RealHalo);
-!   2. nsteps time steps on the device-resident state.
+!   2. nsteps time steps on the device-resident state, then clcStat, the
+!      stats file <fields_out>.stats<nsteps> and the restart file
+!      <fields_out>.RestartForCheck<nsteps>;
+!   3. a fresh context that reads the restart file back.
 ! usage: c3d_fortran_check <case.txt> <fields_in.bin> <fields_out.bin>
 !   case.txt: nxc nyc nzc / xlx yly zlz / xnu dt / nsteps
 !   fields_in.bin (stream): yp(0:nyc+1), ux, uy, uz, pressure (halo layout)
-!   fields_out.bin (stream): ux, uy, uz, pressure after 1., then after 2.
+!   fields_out.bin (stream): ux, uy, uz, pressure after 1., after 2., after 3.
 !-----------------------------------------------------------------------
 program c3d_fortran_check
   use iso_c_binding
@@ -90,6 +93,16 @@ program c3d_fortran_check
   end do
   write(*, '(A,I0,A,ES12.4,A,ES12.4,A,F10.6)') "device-resident ", nsteps, " steps: cfl ", cflmp, &
       "  divmax2 ", divmax2, "  SimTime ", SimTime
+  call gpu_download_state(ux, uy, uz, pressure, HistXOld, HistYOld, HistZOld)
+  write(u) ux, uy, uz, pressure
+  call gpu_clcStat()
+  call gpu_write_stat(trim(fout)//".", nsteps, 1)
+  call gpu_write_restart(nsteps, trim(fout)//".", "Check", .false.)
+  call gpu_finalize()
+
+  ! ---- 3. a fresh context from the restart file --------------------------
+  call gpu_init(p, yp)
+  call gpu_read_restart(nsteps, trim(fout)//".", "Check", .false.)
   call gpu_download_state(ux, uy, uz, pressure, HistXOld, HistYOld, HistZOld)
   write(u) ux, uy, uz, pressure
   close(u)

@@ -235,6 +235,63 @@ module channel3d_c
       import :: c_size_t, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function c3d_set_ppe_refine(ctx, sweeps) bind(C, name="c3d_set_ppe_refine")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sweeps
+    end function
+
+    ! ---- restart files (m_IOAndVisu.f90:352-558); path: NUL-terminated
+    integer(c_int) function c3d_write_restart(ctx, path, ux, uy, uz, pr, hx, hy, hz) &
+        bind(C, name="c3d_write_restart")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx, ux, uy, uz, pr, hx, hy, hz
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function c3d_read_restart(ctx, path, ux, uy, uz, pr, hx, hy, hz) &
+        bind(C, name="c3d_read_restart")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx, ux, uy, uz, pr, hx, hy, hz
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function c3d_restart_name(restart_dir, run_name, ntime, out, out_len) &
+        bind(C, name="c3d_restart_name")
+      import :: c_int, c_char, c_size_t
+      character(kind=c_char), intent(in) :: restart_dir(*), run_name(*)
+      integer(c_int), value :: ntime
+      character(kind=c_char), intent(out) :: out(*)
+      integer(c_size_t), value :: out_len
+    end function
+
+    ! ---- channel statistics (m_FlowType_Channel.f90:135-363)
+    integer(c_int) function c3d_clcStat_CH(ctx, ux, uy, uz, pr) bind(C, name="c3d_clcStat_CH")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, pr
+    end function
+    integer(c_int) function c3d_stat_get(ctx, sumstat, nfstime, prgradsum) &
+        bind(C, name="c3d_stat_get")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sumstat(*)
+      integer(c_int), intent(out) :: nfstime
+      real(c_double), intent(out) :: prgradsum
+    end function
+    integer(c_int) function c3d_stat_reset(ctx) bind(C, name="c3d_stat_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function c3d_write_stat(ctx, path, itime, ivstats) bind(C, name="c3d_write_stat")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: itime, ivstats
+    end function
+    integer(c_int) function c3d_append_prgrad(ctx, path, itime) bind(C, name="c3d_append_prgrad")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: itime
+    end function
   end interface
 
 contains

@@ -35,6 +35,7 @@ module m_channel3d_gpu
   public :: gpu_clcPrSrc, gpu_clcPPE, gpu_FluidVelUpdate, gpu_PressureUpdate
   public :: gpu_CalcMaxCFL, gpu_CheckDivergence, gpu_CalcVmax, gpu_CalcUxAver
   public :: gpu_upload_state, gpu_download_state, gpu_ChannelIterate
+  public :: gpu_write_restart, gpu_read_restart, gpu_clcStat, gpu_write_stat, gpu_append_prgrad
 
 contains
 
@@ -280,5 +281,81 @@ contains
     call c3d_check(c3d_step_diagnostics(ctx, d), "c3d_step_diagnostics")
     cflmp = d(1); divmax1 = d(2); divmax2 = d(3); vmaxabs = d(4:6); SimTime = d(9)
   end subroutine gpu_ChannelIterate
+
+  ! ------------------------------------------- restart files, statistics --
+  ! write_restart / read_restart (src/CFD_2nd/m_IOAndVisu.f90:352-558) on the
+  ! device-resident state; with_hist adds HistX/Y/ZOld (the AB2 restart)
+  subroutine gpu_write_restart(ntime, RestartDir, RunName, with_hist)
+    integer, intent(in) :: ntime
+    character(*), intent(in) :: RestartDir, RunName
+    logical, intent(in) :: with_hist
+    character(len=512) :: chFile
+    type(c_ptr) :: f(7)
+    write(chFile, "(A,I10.10)") trim(RestartDir)//"RestartFor"//trim(RunName), ntime
+    call state_ptrs(f, with_hist)
+    call c3d_check(c3d_write_restart(ctx, trim(chFile)//c_null_char, f(1), f(2), f(3), f(4), &
+                                     f(5), f(6), f(7)), "c3d_write_restart")
+  end subroutine gpu_write_restart
+
+  ! read_restart, then what ChannelInitialize does after it
+  ! (m_ChannelSystem.f90:54-62): wall values, ghosts and halos
+  subroutine gpu_read_restart(ntime, RestartDir, RunName, with_hist)
+    integer, intent(in) :: ntime
+    character(*), intent(in) :: RestartDir, RunName
+    logical, intent(in) :: with_hist
+    character(len=512) :: chFile
+    type(c_ptr) :: f(7), ym, dym
+    write(chFile, "(A,I10.10)") trim(RestartDir)//"RestartFor"//trim(RunName), ntime
+    call state_ptrs(f, with_hist)
+    call c3d_check(c3d_read_restart(ctx, trim(chFile)//c_null_char, f(1), f(2), f(3), f(4), &
+                                    f(5), f(6), f(7)), "c3d_read_restart")
+    call c3d_check(c3d_state_field(ctx, C3D_UY_YM, ym), "state")
+    call c3d_check(c3d_state_field(ctx, C3D_DUY_YM, dym), "state")
+    call c3d_check(c3d_Update_uy_ym(ctx, ym, dym), "Update_uy_ym")
+    call c3d_check(c3d_SetBC_and_UpdateHalo(ctx, f(1), f(2), f(3), ym), "SetBC_and_UpdateHalo")
+    call c3d_check(c3d_SetBC_and_UpdateHalo_pr(ctx, f(4)), "SetBC_and_UpdateHalo_pr")
+  end subroutine gpu_read_restart
+
+  subroutine state_ptrs(f, with_hist)
+    type(c_ptr), intent(out) :: f(7)
+    logical, intent(in) :: with_hist
+    call c3d_check(c3d_state_field(ctx, C3D_UX, f(1)), "state")
+    call c3d_check(c3d_state_field(ctx, C3D_UY, f(2)), "state")
+    call c3d_check(c3d_state_field(ctx, C3D_UZ, f(3)), "state")
+    call c3d_check(c3d_state_field(ctx, C3D_PR, f(4)), "state")
+    f(5:7) = c_null_ptr
+    if (with_hist) then
+      call c3d_check(c3d_state_field(ctx, C3D_HISTX, f(5)), "state")
+      call c3d_check(c3d_state_field(ctx, C3D_HISTY, f(6)), "state")
+      call c3d_check(c3d_state_field(ctx, C3D_HISTZ, f(7)), "state")
+    end if
+  end subroutine state_ptrs
+
+  ! clcStat_CH on the device-resident state (m_FlowType_Channel.f90:175-318)
+  subroutine gpu_clcStat()
+    type(c_ptr) :: f(7)
+    call state_ptrs(f, .false.)
+    call c3d_check(c3d_clcStat_CH(ctx, f(1), f(2), f(3), f(4)), "c3d_clcStat_CH")
+  end subroutine gpu_clcStat
+
+  ! the stats file of the samples since the last write (:330-361)
+  subroutine gpu_write_stat(ResultsDir, itime, ivstats)
+    character(*), intent(in) :: ResultsDir
+    integer, intent(in) :: itime, ivstats
+    character(len=512) :: filename
+    write(filename, "(A,I10.10)") trim(ResultsDir)//'stats', itime
+    call c3d_check(c3d_write_stat(ctx, trim(filename)//c_null_char, int(itime, c_int), &
+                                  int(ivstats, c_int)), "c3d_write_stat")
+  end subroutine gpu_write_stat
+
+  ! one line of the PrGrad file (:317-326)
+  subroutine gpu_append_prgrad(ResultsDir, ilast, itime)
+    character(*), intent(in) :: ResultsDir
+    integer, intent(in) :: ilast, itime
+    character(len=512) :: filename
+    write(filename, '(A,I10.10)') trim(ResultsDir)//"PrGrad", ilast
+    call c3d_check(c3d_append_prgrad(ctx, trim(filename)//c_null_char, int(itime, c_int)), &
+                   "c3d_append_prgrad")
+  end subroutine gpu_append_prgrad
 
 end module m_channel3d_gpu

@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libchannel3d.so")
 
 C3D_OK = 0
+C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_NOMEM, C3D_ERR_UNSUPPORTED, C3D_ERR_IO = -1, -2, -3, -4, -5
+NCHASTAT = 35
 BC_PERIOD, BC_NOSLIP, BC_FREESLIP = 0, -1, -2
 FP64, FP32 = 0, 1
 HALO, INTERIOR, WALL = 0, 1, 2
@@ -38,6 +40,9 @@ EXPORTS = (
     "c3d_CalcUxAver", "c3d_substep", "c3d_step", "c3d_step_diagnostics", "c3d_project",
     "c3d_ppe", "c3d_profile_enable", "c3d_profile_reset", "c3d_profile_count",
     "c3d_profile_get", "c3d_device_bytes",
+    "c3d_write_restart", "c3d_read_restart", "c3d_restart_name",
+    "c3d_clcStat_CH", "c3d_stat_get", "c3d_stat_reset", "c3d_write_stat", "c3d_append_prgrad",
+    "c3d_set_ppe_refine",
 )
 
 
@@ -125,6 +130,16 @@ def load(path: str | None = None):
         "c3d_profile_get": ([_VP, I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_long),
                              _DP, _DP], I),
         "c3d_device_bytes": ([_VP], ctypes.c_size_t),
+        "c3d_write_restart": ([_VP, ctypes.c_char_p] + [_VP] * 7, I),
+        "c3d_read_restart": ([_VP, ctypes.c_char_p] + [_VP] * 7, I),
+        "c3d_restart_name": ([ctypes.c_char_p, ctypes.c_char_p, I, ctypes.c_char_p,
+                              ctypes.c_size_t], I),
+        "c3d_clcStat_CH": ([_VP] * 5, I),
+        "c3d_stat_get": ([_VP, _DP, _IP, _DP], I),
+        "c3d_stat_reset": ([_VP], I),
+        "c3d_write_stat": ([_VP, ctypes.c_char_p, I, I], I),
+        "c3d_append_prgrad": ([_VP, ctypes.c_char_p, I], I),
+        "c3d_set_ppe_refine": ([_VP, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -43,6 +43,7 @@ extern "C" {
 #define C3D_ERR_HIP (-2)         /* HIP runtime error                            */
 #define C3D_ERR_NOMEM (-3)       /* device or host allocation failed             */
 #define C3D_ERR_UNSUPPORTED (-4) /* size not supported by the FFT plans          */
+#define C3D_ERR_IO (-5)          /* file missing, wrong size or write failure    */
 
 /* boundary codes, m_Parameters.f90:69-76 */
 #define C3D_BC_PERIOD 0
@@ -112,6 +113,15 @@ enum {
 
 /* change dt (recomputes the implicit-operator tables), m_ChannelSystem.f90:81-88 */
 int c3d_set_dt(c3d_ctx* ctx, double dt);
+
+/* Pressure solve precision of an FP64 2nd-order context (no counterpart in
+ * the reference, whose solve is FP64 only).  sweeps = -1: the FP64 solve
+ * (default).  sweeps >= 0: the transforms and wall-normal solves run in
+ * FP32 (the singular kx=kz=0 mode in FP64), followed by `sweeps` refinement
+ * sweeps: residual r = prsrc - L phi of the discrete operator in FP64, an
+ * FP32 solve of L d = r, phi += d in FP64.  Applies to c3d_clcPPE, c3d_ppe
+ * and the fused substeps. */
+int c3d_set_ppe_refine(c3d_ctx* ctx, int sweeps);
 
 /* --------------------------------------------------------------- fields -- */
 /* a device field of the given layout in the context's precision */
@@ -212,6 +222,50 @@ int c3d_profile_count(c3d_ctx* ctx);
 int c3d_profile_get(c3d_ctx* ctx, int idx, const char** name, long* calls, double* total_ms,
                     double* bytes_per_call);
 size_t c3d_device_bytes(c3d_ctx* ctx);
+
+/* ------------------------------------------------------- restart files -- */
+/* The reference's restart file (write_restart / read_restart,
+ * src/CFD_2nd/m_IOAndVisu.f90:352-438 / 443-558; CFD_4th m_IOAndVisu.f90:
+ * 384-489): the interior points (1:nxc, 1:nyc, 1:nzc) of ux, uy, uz and
+ * pressure, then, when history fields are given, HistXOld, HistYOld,
+ * HistZOld, each a native-endian real(8) array in Fortran order, back to
+ * back, no header.  A 4th-order context writes ux + uCRF and reads it back
+ * minus uCRF.  Reading fills interior points only; as in the reference the
+ * caller then applies SetBC_and_UpdateHalo and SetBC_and_UpdateHalo_pr.
+ * Field arguments are device fields; histx/y/z are all NULL or all given.
+ * Reading checks the file size like the reference: 7 arrays when histories
+ * are requested, 4 or 7 otherwise.  Both calls are synchronous. */
+int c3d_write_restart(c3d_ctx* ctx, const char* path, const void* ux, const void* uy,
+                      const void* uz, const void* pr, const void* histx, const void* histy,
+                      const void* histz);
+int c3d_read_restart(c3d_ctx* ctx, const char* path, void* ux, void* uy, void* uz, void* pr,
+                     void* histx, void* histy, void* histz);
+/* RestartDir//"RestartFor"//RunName//I10.10 (m_IOAndVisu.f90:363) */
+int c3d_restart_name(const char* restart_dir, const char* run_name, int ntime, char* out,
+                     size_t out_len);
+
+/* -------------------------------------------------- channel statistics -- */
+/* Running plane averages of the 2nd-order channel, the 35-column variant of
+ * InitStatVar_CH / clcStat_CH (src/CFD_2nd/m_FlowType_Channel.f90:135-167,
+ * 175-363).  SumStat(35, nyc+1) and PrGradsum live on the device.
+ *  c3d_clcStat_CH   adds one sample of ux, uy, uz, pressure (halo fields with
+ *                   valid ghosts, as after SetBC_and_UpdateHalo) and of
+ *                   PrGradAver; asynchronous
+ *  c3d_stat_get     copies SumStat (Fortran order, 35 x (nyc+1)), the sample
+ *                   count nfstime and PrGradsum to the host; synchronous
+ *  c3d_stat_reset   zeroes the sums and the count
+ *  c3d_write_stat   writes the reference's stats file for the samples since
+ *                   the last reset (:330-361: header, time-averaged pressure
+ *                   gradient, nyc+1 rows of 35 ES24.15 values), then resets
+ *  c3d_append_prgrad appends "itime SimTime PrGradAver" (I7,2ES24.15) to the
+ *                   PrGrad file (:317-326); synchronous */
+#define C3D_NCHASTAT 35
+int c3d_clcStat_CH(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
+                   const void* pr);
+int c3d_stat_get(c3d_ctx* ctx, double* sumstat, int* nfstime, double* prgradsum);
+int c3d_stat_reset(c3d_ctx* ctx);
+int c3d_write_stat(c3d_ctx* ctx, const char* path, int itime, int ivstats);
+int c3d_append_prgrad(c3d_ctx* ctx, const char* path, int itime);
 
 #ifdef __cplusplus
 }

@@ -1331,6 +1331,143 @@ void orc_project(orc_ctx* c) {
   free(prtmp);
 }
 
+/* ------------------------------------------------------------------------- */
+/* m_FlowType_Channel.f90:175-318 clcStat_CH (the 35-column variant,          */
+/* NCHASTAT :32).  Per plane jc=1..nyc the 35 sums run over kc outer, ic      */
+/* inner, in the reference's order; plane nyp holds only columns 27, 33, 35  */
+/* (:300-315).  sumstat is SumStat(35, nyp), Fortran order; SumVec*inxz is    */
+/* added to it (:287-289, :313-315).                                          */
+/* ------------------------------------------------------------------------- */
+static void stat_ch(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                    const double* pr, double* sumstat, int mag) {
+#define ACC(v, t) v = v + (mag ? fabs(t) : (t))
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nyp = nyc + 1;
+  const double *yint = c->met[ORC_YINTERP], *rdyc = c->met[ORC_RDYC], *rdyp = c->met[ORC_RDYP];
+  const double *am2c = c->met[ORC_AM2C], *ac2c = c->met[ORC_AC2C], *ap2c = c->met[ORC_AP2C];
+  const double *am2p = c->met[ORC_AM2P], *ac2p = c->met[ORC_AC2P], *ap2p = c->met[ORC_AP2P];
+  const double rdx = c->rdx, rdz = c->rdz, rdx2 = c->rdx2, rdz2 = c->rdz2;
+  const double rdxh = rdx * 0.5;
+  const double inxz = 1.0 / ((double)nxc * (double)nzc);
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(dynamic)
+#endif
+  for (int jc = 1; jc <= nyc; ++jc) {
+    const int jp = jc + 1, jm = jc - 1;
+    const double I1 = 0.5 * yint[jc], I2 = 0.5 - I1;
+    const double cac = rdyc[jc], cacU = rdyc[jp], caj = rdyp[jc];
+    double s[35];
+    for (int m = 0; m < 35; ++m) s[m] = 0.0;
+    for (int kc = 1; kc <= nzc; ++kc) {
+      const int km = kc - 1, kp = kc + 1;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const int im = ic - 1, ip = ic + 1;
+        const double uxloc = UX(ic, jc, kc), uyloc = UY(ic, jc, kc), uzloc = UZ(ic, jc, kc);
+        const double prloc = PR(ic, jc, kc);
+        const double uxCell = 0.5 * (UX(ic, jc, kc) + UX(ip, jc, kc));
+        const double uyCell = 0.5 * (UY(ic, jc, kc) + UY(ic, jp, kc));
+        const double uzCell = 0.5 * (UZ(ic, jc, kc) + UZ(ic, jc, kp));
+        const double prloc2 = I1 * (PR(im, jm, kc) + PR(ic, jm, kc)) + I2 * (PR(im, jc, kc) + PR(ic, jc, kc));
+        const double dudx = (UX(ip, jc, kc) - UX(ic, jc, kc)) * rdx;
+        const double dudy = (UX(ic, jc, kc) - UX(ic, jm, kc)) * cac;
+        const double dudz = (UX(ic, jc, kc) - UX(ic, jc, km)) * rdz;
+        const double dvdx = (UY(ic, jc, kc) - UY(im, jc, kc)) * rdx;
+        const double dvdy = (UY(ic, jp, kc) - UY(ic, jc, kc)) * caj;
+        const double dvdz = (UY(ic, jc, kc) - UY(ic, jc, km)) * rdz;
+        const double dwdx = (UZ(ic, jc, kc) - UZ(im, jc, kc)) * rdx;
+        const double dwdy = (UZ(ic, jc, kc) - UZ(ic, jm, kc)) * cac;
+        const double dwdz = (UZ(ic, jc, kp) - UZ(ic, jc, kc)) * rdz;
+        const double dudxx = (UX(ip, jc, kc) - 2.0 * UX(ic, jc, kc) + UX(im, jc, kc)) * rdx2;
+        const double dudyy = ap2c[jc] * UX(ic, jp, kc) + ac2c[jc] * UX(ic, jc, kc) + am2c[jc] * UX(ic, jm, kc);
+        const double dudzz = (UX(ic, jc, kp) - 2.0 * UX(ic, jc, kc) + UX(ic, jc, km)) * rdz2;
+        const double dvdxx = (UY(ip, jc, kc) - 2.0 * UY(ic, jc, kc) + UY(im, jc, kc)) * rdx2;
+        const double dvdyy = ap2p[jc] * UY(ic, jp, kc) + ac2p[jc] * UY(ic, jc, kc) + am2p[jc] * UY(ic, jm, kc);
+        const double dvdzz = (UY(ic, jc, kp) - 2.0 * UY(ic, jc, kc) + UY(ic, jc, km)) * rdz2;
+        const double dwdxx = (UZ(ip, jc, kc) - 2.0 * UZ(ic, jc, kc) + UZ(im, jc, kc)) * rdx2;
+        const double dwdyy = ap2c[jc] * UZ(ic, jp, kc) + ac2c[jc] * UZ(ic, jc, kc) + am2c[jc] * UZ(ic, jm, kc);
+        const double dwdzz = (UZ(ic, jc, kp) - 2.0 * UZ(ic, jc, kc) + UZ(ic, jc, km)) * rdz2;
+        const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+        const double dudxC = (UX(ip, jc, kc) - UX(im, jc, kc)) * rdxh;
+        const double dvdxU = (UY(ic, jp, kc) - UY(im, jp, kc)) * rdx;
+        const double dudyU = (UX(ic, jp, kc) - UX(ic, jc, kc)) * cacU;
+        const double dvdyM = (UY(im, jp, kc) - UY(im, jc, kc)) * caj;
+        const double dudzC = (I1 * (UX(ic, jm, kp) - UX(ic, jm, km)) + I2 * (UX(ic, jc, kp) - UX(ic, jc, km))) * rdz;
+        const double dvdzC = (UY(im, jc, kp) - UY(im, jc, km) + UY(ic, jc, kp) - UY(ic, jc, km)) * rdz * 0.25;
+        ACC(s[0], uxloc);
+        ACC(s[1], uyloc);
+        ACC(s[2], uzloc);
+        ACC(s[3], prloc);
+        ACC(s[4], uxloc * uxloc);
+        ACC(s[5], uyloc * uyloc);
+        ACC(s[6], uzloc * uzloc);
+        ACC(s[7], prloc * prloc);
+        ACC(s[8], uxCell * uyCell);
+        ACC(s[9], uyCell * uzCell);
+        ACC(s[10], uxCell * uzCell);
+        ACC(s[11], uxCell * prloc);
+        ACC(s[12], uyCell * prloc);
+        ACC(s[13], uzCell * prloc);
+        ACC(s[14], uxCell * uxCell * uyCell);
+        ACC(s[15], uyloc * uyloc * uyloc);
+        ACC(s[16], uzCell * uzCell * uyCell);
+        ACC(s[17], uxCell * uyCell * uyCell);
+        ACC(s[18], uxloc * uxloc * uxloc);
+        ACC(s[19], uzloc * uzloc * uzloc);
+        ACC(s[20], uxloc * uxloc * uxloc * uxloc);
+        ACC(s[21], uyloc * uyloc * uyloc * uyloc);
+        ACC(s[22], uzloc * uzloc * uzloc * uzloc);
+        ACC(s[23], prloc * dudx);
+        ACC(s[24], prloc * dvdy);
+        ACC(s[25], prloc * dwdz);
+        ACC(s[26], prloc2 * (dudy + dvdx));
+        ACC(s[27], uxloc * (dudxx + dudyy + dudzz));
+        ACC(s[28], uyloc * (dvdxx + dvdyy + dvdzz));
+        ACC(s[29], uzloc * (dwdxx + dwdyy + dwdzz));
+        s[30] = s[30] + (mag ? fabs(dudxC * (dvdxU + dvdx) * 0.5) + fabs((dudyU + dudy) * (dvdyM + dvdy) * 0.25)
+                          : dudxC * (dvdxU + dvdx) * 0.5 + (dudyU + dudy) * (dvdyM + dvdy) * 0.25);
+        ACC(s[31], dudzC * dvdzC);
+        ACC(s[32], vor_x * vor_x);
+        ACC(s[33], vor_y * vor_y);
+        ACC(s[34], vor_z * vor_z);
+      }
+    }
+    double* out = sumstat + (size_t)35 * (size_t)(jc - 1);
+    for (int m = 0; m < 35; ++m) out[m] = out[m] + s[m] * inxz;
+  }
+  {
+    const int jc = nyp, jm = jc - 1;
+    const double cac = rdyc[jc];
+    const double I1 = 0.5 * yint[jc], I2 = 0.5 - I1;
+    double s27 = 0.0, s33 = 0.0, s35 = 0.0;
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const int im = ic - 1;
+        const double prloc2 = I1 * (PR(im, jm, kc) + PR(ic, jm, kc)) + I2 * (PR(im, jc, kc) + PR(ic, jc, kc));
+        const double dudy = (UX(ic, jc, kc) - UX(ic, jm, kc)) * cac;
+        const double dwdy = (UZ(ic, jc, kc) - UZ(ic, jm, kc)) * cac;
+        const double vor_x = dwdy, vor_z = -dudy;
+        ACC(s27, prloc2 * (dudy + 0.0));
+        ACC(s33, vor_x * vor_x);
+        ACC(s35, vor_z * vor_z);
+      }
+    double* out = sumstat + (size_t)35 * (size_t)(nyp - 1);
+    out[26] = out[26] + s27 * inxz;
+    out[32] = out[32] + s33 * inxz;
+    out[34] = out[34] + s35 * inxz;
+  }
+#undef ACC
+}
+
+void orc_clcStat_CH(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                    const double* pr, double* sumstat) {
+  stat_ch(c, ux, uy, uz, pr, sumstat, 0);
+}
+
+/* the same sums of |term| (the scale of each sum's rounding error) */
+void orc_clcStat_CH_mag(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                        const double* pr, double* sumstat) {
+  stat_ch(c, ux, uy, uz, pr, sumstat, 1);
+}
+
 /* order-independent sum of n doubles (exposed for its own test) */
 double orc_xsum(int n, const double* v) {
   long long acc[4] = {0, 0, 0, 0};

@@ -114,6 +114,13 @@ double orc_CheckDivergence(orc_ctx* c, const double* ux, const double* uy, const
 void orc_CalcVmax(orc_ctx* c, const double* ux, const double* uy, const double* uz,
                   double* vmax3);
 double orc_CalcUxAver(orc_ctx* c, const double* ux);
+/* clcStat_CH, 35-column variant (m_FlowType_Channel.f90:175-318): adds the
+ * plane averages of this call to sumstat = SumStat(35, nyc+1), Fortran order */
+void orc_clcStat_CH(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                    const double* pr, double* sumstat);
+/* the same plane averages of |term|: the scale of each sum's rounding error */
+void orc_clcStat_CH_mag(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                        const double* pr, double* sumstat);
 
 /* one substep / one full time step on the context's own fields
  * (m_ChannelSystem.f90:91-125 / :73-130), dt fixed (iCFL=2) */

@@ -131,6 +131,8 @@ def lib():
         L.orc_CalcVmax.argtypes = [ctypes.c_void_p] + [_DP] * 4
         L.orc_CalcUxAver.restype = ctypes.c_double
         L.orc_CalcUxAver.argtypes = [ctypes.c_void_p, _DP]
+        L.orc_clcStat_CH.argtypes = [ctypes.c_void_p] + [_DP] * 5
+        L.orc_clcStat_CH_mag.argtypes = [ctypes.c_void_p] + [_DP] * 5
         L.orc_InverseTridiagonal.argtypes = [_DP, _DP, _DP, _DP, ctypes.c_int, ctypes.c_int]
         L.orc_InversePTriFixedCoe.argtypes = [ctypes.c_double] * 3 + [_DP, ctypes.c_int,
                                                                        ctypes.c_int]

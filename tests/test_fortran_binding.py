@@ -4,7 +4,9 @@ ChannelIterate drives its time-step routines, against the CPU oracle.
 The check program (channel3d_amd/fortran/c3d_fortran_check.f90, built by
 __graft_entry__.build() with amdflang) runs one RK substep through the
 per-routine wrappers on host arrays, in the reference's call order and
-scratch aliasing, then several time steps on the device-resident state.
+scratch aliasing, then several time steps on the device-resident state,
+takes a statistics sample and writes the stats and restart files, and
+finally reads the restart file into a fresh context.
 """
 import os
 import subprocess
@@ -55,8 +57,8 @@ def test_fortran_call_sites_match_oracle(oracle_mod, tmp_path):
     shp = (nx + 3, ny + 3, nz + 3)
     n = int(np.prod(shp))
     out = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
-    assert out.size == 8 * n
-    got = [out[q * n:(q + 1) * n].reshape(shp, order="F") for q in range(8)]
+    assert out.size == 12 * n
+    got = [out[q * n:(q + 1) * n].reshape(shp, order="F") for q in range(12)]
     # 1. one substep through the per-routine wrappers
     o.substep(1)
     for q, fid in enumerate((O.UX, O.UY, O.UZ, O.PR)):
@@ -73,5 +75,19 @@ def test_fortran_call_sites_match_oracle(oracle_mod, tmp_path):
         ref = _ymid(o2.field(fid))
         err = np.abs(_ymid(got[4 + q]) - ref).max() / np.abs(ref).max()
         assert err < 1e-12, (q, err)
+    # 3. the restart file holds the interiors; read back (and the ghosts
+    # rebuilt as ChannelInitialize does) it gives the state of 2. exactly
+    rst = np.fromfile(str(tmp_path / "out.bin.") + "RestartForCheck%010d" % nsteps,
+                      dtype=np.float64)
+    assert rst.size == 4 * nx * ny * nz
+    for q in range(4):
+        a = rst[q * nx * ny * nz:(q + 1) * nx * ny * nz].reshape((nx, ny, nz), order="F")
+        assert np.array_equal(a, got[4 + q][1:nx + 1, 1:ny + 1, 1:nz + 1])
+        assert np.array_equal(_ymid(got[8 + q])[:, :ny + 2], _ymid(got[4 + q])[:, :ny + 2]), q
+    # the stats file of one sample: nyc+1 rows of 35 values, mean u > 0
+    lines = open(str(tmp_path / "out.bin.") + "stats%010d" % nsteps).read().split("\n")
+    rows = [ln for ln in lines[5:] if ln.strip()]
+    assert len(rows) == ny + 1 and all(len(ln) == 35 * 24 for ln in rows)
+    assert float(rows[ny // 2][:24]) > 0.5
     o.close()
     o2.close()
