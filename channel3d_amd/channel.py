@@ -296,7 +296,19 @@ class Channel:
 
     # channel statistics (m_FlowType_Channel.f90:135-363) ----------------------
     def clcStat(self, ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR):
-        self._call("clcStat_CH", ux, uy, uz, pressure)
+        """clcStat_CH (2nd order) or the 4th-order code's clcStat"""
+        self._call("clcStat" if self.order == 4 else "clcStat_CH", ux, uy, uz, pressure)
+
+    def stat_set_grad(self, on=True):
+        L.check(self.lib.c3d_stat_set_grad(self._h, int(on)), "c3d_stat_set_grad")
+
+    def stat_get_grad(self):
+        q = np.zeros((L.NGRADSTAT, self.nyc + 1), order="F")
+        L.check(self.lib.c3d_stat_get_grad(self._h, L.dptr(q)), "c3d_stat_get_grad")
+        return q
+
+    def write_vgrad(self, path):
+        L.check(self.lib.c3d_write_vgrad(self._h, os.fsencode(path)), "c3d_write_vgrad")
 
     def stat_get(self):
         """(SumStat as a (35, nyc+1) array, nfstime, PrGradsum)"""

@@ -230,6 +230,7 @@ struct c3d_ctx {
   double* d_stat = nullptr;
   int stat_kruns = 0;         // z runs (workgroups) per wall-normal plane
   int stat_n = 0;             // nfstime
+  bool stat_grad = false;     // 4th order: also the 98 gradient moments
 
   // profiling
   bool prof = false;

@@ -286,6 +286,26 @@ module channel3d_c
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int), value :: itime, ivstats
     end function
+    ! ---- 4th-order statistics (CFD_4th m_FlowCase.f90:306-633)
+    integer(c_int) function c3d_clcStat(ctx, ux, uy, uz, pr) bind(C, name="c3d_clcStat")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, pr
+    end function
+    integer(c_int) function c3d_stat_set_grad(ctx, on) bind(C, name="c3d_stat_set_grad")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function c3d_stat_get_grad(ctx, sumgrad) bind(C, name="c3d_stat_get_grad")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sumgrad(*)
+    end function
+    integer(c_int) function c3d_write_vgrad(ctx, path) bind(C, name="c3d_write_vgrad")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
     integer(c_int) function c3d_append_prgrad(ctx, path, itime) bind(C, name="c3d_append_prgrad")
       import :: c_int, c_ptr, c_char
       type(c_ptr), value :: ctx

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libchannel3d.so")
 C3D_OK = 0
 C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_NOMEM, C3D_ERR_UNSUPPORTED, C3D_ERR_IO = -1, -2, -3, -4, -5
 NCHASTAT = 35
+NGRADSTAT = 98
 BC_PERIOD, BC_NOSLIP, BC_FREESLIP = 0, -1, -2
 FP64, FP32 = 0, 1
 HALO, INTERIOR, WALL = 0, 1, 2
@@ -42,7 +43,8 @@ EXPORTS = (
     "c3d_profile_get", "c3d_device_bytes",
     "c3d_write_restart", "c3d_read_restart", "c3d_restart_name",
     "c3d_clcStat_CH", "c3d_stat_get", "c3d_stat_reset", "c3d_write_stat", "c3d_append_prgrad",
-    "c3d_set_ppe_refine",
+    "c3d_set_ppe_refine", "c3d_clcStat", "c3d_stat_set_grad", "c3d_stat_get_grad",
+    "c3d_write_vgrad",
 )
 
 
@@ -140,6 +142,10 @@ def load(path: str | None = None):
         "c3d_write_stat": ([_VP, ctypes.c_char_p, I, I], I),
         "c3d_append_prgrad": ([_VP, ctypes.c_char_p, I], I),
         "c3d_set_ppe_refine": ([_VP, I], I),
+        "c3d_clcStat": ([_VP] * 5, I),
+        "c3d_stat_set_grad": ([_VP, I], I),
+        "c3d_stat_get_grad": ([_VP, _DP], I),
+        "c3d_write_vgrad": ([_VP, ctypes.c_char_p], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

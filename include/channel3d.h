@@ -260,8 +260,21 @@ int c3d_restart_name(const char* restart_dir, const char* run_name, int ntime, c
  *  c3d_append_prgrad appends "itime SimTime PrGradAver" (I7,2ES24.15) to the
  *                   PrGrad file (:317-326); synchronous */
 #define C3D_NCHASTAT 35
+#define C3D_NGRADSTAT 98
 int c3d_clcStat_CH(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
                    const void* pr);
+/* The 4th-order code's clcStat (src/CFD_4th/m_FlowCase.f90:306-633): the same
+ * 35 columns with its 4th-order x/z stencils and ux + uCRF, for an order-4
+ * context.  c3d_stat_set_grad(ctx, 1) adds the 98 HighOrderGradStat moments
+ * (:461-570; a compile-time option there, a switch here), read with
+ * c3d_stat_get_grad as SumGrad(98, nyc+1) and written by c3d_write_vgrad as
+ * the reference's vGrad file (:983-996; call it before c3d_write_stat, which
+ * resets).  The spectra and correlation fields of that routine (:641-941)
+ * are not computed. */
+int c3d_clcStat(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, const void* pr);
+int c3d_stat_set_grad(c3d_ctx* ctx, int on);
+int c3d_stat_get_grad(c3d_ctx* ctx, double* sumgrad);
+int c3d_write_vgrad(c3d_ctx* ctx, const char* path);
 int c3d_stat_get(c3d_ctx* ctx, double* sumstat, int* nfstime, double* prgradsum);
 int c3d_stat_reset(c3d_ctx* ctx);
 int c3d_write_stat(c3d_ctx* ctx, const char* path, int itime, int ivstats);

@@ -936,3 +936,224 @@ void orc4_project(orc4_ctx* c) {
   orc4_SetBC_and_UpdateHalo(c, ux, uy, uz, c->f[ORC_UY_YM]);
   c->pmAlpha = save_alpha;
 }
+
+/* ------------------------------------------------------------------------- */
+/* m_FlowCase.f90:306-633 clcStat (the 35-column variant, NCHASTAT :87, and  */
+/* the HighOrderGradStat moments, nGradStat :93), in the reference's loop    */
+/* order.  sumstat = SumStat(35, nyp) and sumgrad = SumGrad(98, nyp) (may be */
+/* NULL), Fortran order; SumVec*inxz is added to them.  mag != 0 sums |term| */
+/* instead (the scale of each sum's rounding error).                         */
+/* ------------------------------------------------------------------------- */
+static void stat4(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                  const double* pr, double* sumstat, double* sumgrad, int mag) {
+#define ACC(v, t) v = v + (mag ? fabs(t) : (t))
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nyp = nyc + 1;
+  const double *yint = c->met[ORC_YINTERP], *rdyc = c->met[ORC_RDYC], *rdyp = c->met[ORC_RDYP];
+  const double *am2c = c->met[ORC_AM2C], *ac2c = c->met[ORC_AC2C], *ap2c = c->met[ORC_AP2C];
+  const double *am2p = c->met[ORC_AM2P], *ac2p = c->met[ORC_AC2P], *ap2p = c->met[ORC_AP2P];
+  const double uCRF = c->p.u_crf;
+  const double dx1 = c->dxCoe1, dx2 = c->dxCoe2, dx3 = c->dxCoe3, dx4 = c->dxCoe4;
+  const double dz1 = c->dzCoe1, dz2 = c->dzCoe2, dz3 = c->dzCoe3, dz4 = c->dzCoe4;
+  const double dxx1 = c->dxxCoe1, dxx2 = c->dxxCoe2, dxx3 = c->dxxCoe3, dxx4 = c->dxxCoe4,
+               dxx5 = c->dxxCoe5;
+  const double dzz1 = c->dzzCoe1, dzz2 = c->dzzCoe2, dzz3 = c->dzzCoe3, dzz4 = c->dzzCoe4,
+               dzz5 = c->dzzCoe5;
+  const double ic1 = c->InterpCoe1, ic2 = c->InterpCoe2, ic3 = c->InterpCoe3, ic4 = c->InterpCoe4;
+  const double rdxt = c->rdx / 12.0, rdzt = c->rdz / 12.0;
+  const double inxz = 1.0 / ((double)nxc * (double)nzc);
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(dynamic)
+#endif
+  for (int jc = 1; jc <= nyc; ++jc) {
+    const int jp = jc + 1, jm = jc - 1;
+    const double Y1 = yint[jc], Y2 = 1.0 - Y1;
+    const double cac = rdyc[jc], cacU = rdyc[jp], caj = rdyp[jc];
+    double s[35], q[98];
+    for (int m = 0; m < 35; ++m) s[m] = 0.0;
+    for (int m = 0; m < 98; ++m) q[m] = 0.0;
+    for (int kc = 1; kc <= nzc; ++kc) {
+      const int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        const double uxloc = UX(ic, jc, kc) + uCRF;
+        const double uyloc = UY(ic, jc, kc);
+        const double uzloc = UZ(ic, jc, kc);
+        const double prloc = PR(ic, jc, kc);
+        const double uxCell = ic1 * UX(im, jc, kc) + ic2 * UX(ic, jc, kc) + ic3 * UX(ip, jc, kc) +
+                              ic4 * UX(iu, jc, kc) + uCRF;
+        const double uyCell = (uyloc + UY(ic, jp, kc)) * 0.5;
+        const double uy_xc = ic1 * UY(is, jc, kc) + ic2 * UY(im, jc, kc) + ic3 * UY(ic, jc, kc) +
+                             ic4 * UY(ip, jc, kc);
+        const double uy_xp = ic1 * UY(is, jp, kc) + ic2 * UY(im, jp, kc) + ic3 * UY(ic, jp, kc) +
+                             ic4 * UY(ip, jp, kc);
+        const double uzCell = ic1 * UZ(ic, jc, km) + ic2 * UZ(ic, jc, kc) + ic3 * UZ(ic, jc, kp) +
+                              ic4 * UZ(ic, jc, ku);
+        const double prloc2 = Y1 * (ic1 * PR(is, jm, kc) + ic2 * PR(im, jm, kc) +
+                                    ic3 * PR(ic, jm, kc) + ic4 * PR(ip, jm, kc)) +
+                              Y2 * (ic1 * PR(is, jc, kc) + ic2 * PR(im, jc, kc) +
+                                    ic3 * PR(ic, jc, kc) + ic4 * PR(ip, jc, kc));
+        const double dudx = dx1 * UX(im, jc, kc) + dx2 * UX(ic, jc, kc) + dx3 * UX(ip, jc, kc) +
+                            dx4 * UX(iu, jc, kc);
+        const double dudy = (UX(ic, jc, kc) - UX(ic, jm, kc)) * cac;
+        const double dudz = dz1 * UX(ic, jc, ks) + dz2 * UX(ic, jc, km) + dz3 * UX(ic, jc, kc) +
+                            dz4 * UX(ic, jc, kp);
+        const double dvdx = dx1 * UY(is, jc, kc) + dx2 * UY(im, jc, kc) + dx3 * UY(ic, jc, kc) +
+                            dx4 * UY(ip, jc, kc);
+        const double dvdy = (UY(ic, jp, kc) - uyloc) * caj;
+        const double dvdz = dz1 * UY(ic, jc, ks) + dz2 * UY(ic, jc, km) + dz3 * UY(ic, jc, kc) +
+                            dz4 * UY(ic, jc, kp);
+        const double dwdx = dx1 * UZ(is, jc, kc) + dx2 * UZ(im, jc, kc) + dx3 * UZ(ic, jc, kc) +
+                            dx4 * UZ(ip, jc, kc);
+        const double dwdy = (uzloc - UZ(ic, jm, kc)) * cac;
+        const double dwdz = dz1 * UZ(ic, jc, km) + dz2 * UZ(ic, jc, kc) + dz3 * UZ(ic, jc, kp) +
+                            dz4 * UZ(ic, jc, ku);
+        const double dudxx = dxx1 * UX(is, jc, kc) + dxx2 * UX(im, jc, kc) + dxx3 * UX(ic, jc, kc) +
+                             dxx4 * UX(ip, jc, kc) + dxx5 * UX(iu, jc, kc);
+        const double dudyy = ap2c[jc] * UX(ic, jp, kc) + ac2c[jc] * UX(ic, jc, kc) + am2c[jc] * UX(ic, jm, kc);
+        const double dudzz = dzz1 * UX(ic, jc, ks) + dzz2 * UX(ic, jc, km) + dzz3 * UX(ic, jc, kc) +
+                             dzz4 * UX(ic, jc, kp) + dzz5 * UX(ic, jc, ku);
+        const double dvdxx = dxx1 * UY(is, jc, kc) + dxx2 * UY(im, jc, kc) + dxx3 * UY(ic, jc, kc) +
+                             dxx4 * UY(ip, jc, kc) + dxx5 * UY(iu, jc, kc);
+        const double dvdyy = ap2p[jc] * UY(ic, jp, kc) + ac2p[jc] * UY(ic, jc, kc) + am2p[jc] * UY(ic, jm, kc);
+        const double dvdzz = dzz1 * UY(ic, jc, ks) + dzz2 * UY(ic, jc, km) + dzz3 * UY(ic, jc, kc) +
+                             dzz4 * UY(ic, jc, kp) + dzz5 * UY(ic, jc, ku);
+        const double dwdxx = dxx1 * UZ(is, jc, kc) + dxx2 * UZ(im, jc, kc) + dxx3 * UZ(ic, jc, kc) +
+                             dxx4 * UZ(ip, jc, kc) + dxx5 * UZ(iu, jc, kc);
+        const double dwdyy = ap2c[jc] * UZ(ic, jp, kc) + ac2c[jc] * UZ(ic, jc, kc) + am2c[jc] * UZ(ic, jm, kc);
+        const double dwdzz = dzz1 * UZ(ic, jc, ks) + dzz2 * UZ(ic, jc, km) + dzz3 * UZ(ic, jc, kc) +
+                             dzz4 * UZ(ic, jc, kp) + dzz5 * UZ(ic, jc, ku);
+        const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+        const double dudxC = (-UX(iu, jc, kc) + 8.0 * UX(ip, jc, kc) - 8.0 * UX(im, jc, kc) + UX(is, jc, kc)) * rdxt;
+        const double dvdxU = dx1 * UY(is, jp, kc) + dx2 * UY(im, jp, kc) + dx3 * UY(ic, jp, kc) +
+                             dx4 * UY(ip, jp, kc);
+        const double dudyU = (UX(ic, jp, kc) - UX(ic, jc, kc)) * cacU;
+        const double dudzC = (Y1 * (-UX(ic, jm, ku) + 8.0 * UX(ic, jm, kp) - 8.0 * UX(ic, jm, km) + UX(ic, jm, ks)) +
+                              Y2 * (-UX(ic, jc, ku) + 8.0 * UX(ic, jc, kp) - 8.0 * UX(ic, jc, km) + UX(ic, jc, ks))) *
+                             rdzt;
+        const double dvdzC =
+            ((-UY(is, jc, ku) + 8.0 * UY(is, jc, kp) - 8.0 * UY(is, jc, km) + UY(is, jc, ks)) * ic1 +
+             (-UY(im, jc, ku) + 8.0 * UY(im, jc, kp) - 8.0 * UY(im, jc, km) + UY(im, jc, ks)) * ic2 +
+             (-UY(ic, jc, ku) + 8.0 * UY(ic, jc, kp) - 8.0 * UY(ic, jc, km) + UY(ic, jc, ks)) * ic3 +
+             (-UY(ip, jc, ku) + 8.0 * UY(ip, jc, kp) - 8.0 * UY(ip, jc, km) + UY(ip, jc, ks)) * ic4) *
+            rdzt;
+        ACC(s[0], uxloc);
+        ACC(s[1], uyloc);
+        ACC(s[2], uzloc);
+        ACC(s[3], prloc);
+        ACC(s[4], uxloc * uxloc);
+        ACC(s[5], uyloc * uyloc);
+        ACC(s[6], uzloc * uzloc);
+        ACC(s[7], prloc * prloc);
+        ACC(s[8], uxCell * uyCell);
+        ACC(s[9], uyCell * uzCell);
+        ACC(s[10], uxCell * uzCell);
+        ACC(s[11], uxCell * prloc);
+        ACC(s[12], uyCell * prloc);
+        ACC(s[13], uzCell * prloc);
+        ACC(s[14], uxCell * uxCell * uyCell);
+        ACC(s[15], uyloc * uyloc * uyloc);
+        ACC(s[16], uzCell * uzCell * uyCell);
+        ACC(s[17], uxCell * uyCell * uyCell);
+        ACC(s[18], uxloc * uxloc * uxloc);
+        ACC(s[19], uzloc * uzloc * uzloc);
+        ACC(s[20], uxloc * uxloc * uxloc * uxloc);
+        ACC(s[21], uyloc * uyloc * uyloc * uyloc);
+        ACC(s[22], uzloc * uzloc * uzloc * uzloc);
+        ACC(s[23], prloc * dudx);
+        ACC(s[24], prloc * dvdy);
+        ACC(s[25], prloc * dwdz);
+        ACC(s[26], prloc2 * (dudy + dvdx));
+        ACC(s[27], uxloc * (dudxx + dudyy + dudzz));
+        ACC(s[28], uyloc * (dvdxx + dvdyy + dvdzz));
+        ACC(s[29], uzloc * (dwdxx + dwdyy + dwdzz));
+        s[30] = s[30] + (mag ? (fabs(dudxC * (dvdxU + dvdx)) + fabs((dudyU + dudy) * (uy_xp - uy_xc) * caj)) * 0.5
+                             : (dudxC * (dvdxU + dvdx) + (dudyU + dudy) * (uy_xp - uy_xc) * caj) * 0.5);
+        ACC(s[31], dudzC * dvdzC);
+        ACC(s[32], vor_x * vor_x);
+        ACC(s[33], vor_y * vor_y);
+        ACC(s[34], vor_z * vor_z);
+        if (sumgrad) {
+          const double dpdx = dx1 * PR(is, jc, kc) + dx2 * PR(im, jc, kc) + dx3 * PR(ic, jc, kc) +
+                              dx4 * PR(ip, jc, kc);
+          const double dpdy = (PR(ic, jc, kc) - PR(ic, jm, kc)) * cac;
+          const double dpdz = dz1 * PR(ic, jc, ks) + dz2 * PR(ic, jc, km) + dz3 * PR(ic, jc, kc) +
+                              dz4 * PR(ic, jc, kp);
+          const double dpdxx = dxx1 * PR(is, jc, kc) + dxx2 * PR(im, jc, kc) + dxx3 * PR(ic, jc, kc) +
+                               dxx4 * PR(ip, jc, kc) + dxx5 * PR(iu, jc, kc);
+          const double dpdyy = ap2c[jc] * PR(ic, jp, kc) + ac2c[jc] * PR(ic, jc, kc) + am2c[jc] * PR(ic, jm, kc);
+          const double dpdzz = dzz1 * PR(ic, jc, ks) + dzz2 * PR(ic, jc, km) + dzz3 * PR(ic, jc, kc) +
+                               dzz4 * PR(ic, jc, kp) + dzz5 * PR(ic, jc, ku);
+          const double g[24] = {dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy,
+                                dwdz, dpdx, dpdy, dpdz, dudxx, dudyy, dudzz, dvdxx,
+                                dvdyy, dvdzz, dwdxx, dwdyy, dwdzz, dpdxx, dpdyy, dpdzz};
+          ACC(q[0], prloc * prloc * prloc);
+          ACC(q[1], prloc * prloc * prloc * prloc);
+          for (int t = 0; t < 24; ++t) {
+            const double v = g[t];
+            ACC(q[2 + 4 * t], v);
+            ACC(q[3 + 4 * t], v * v);
+            ACC(q[4 + 4 * t], v * v * v);
+            ACC(q[5 + 4 * t], v * v * v * v);
+          }
+        }
+      }
+    }
+    double* out = sumstat + (size_t)35 * (size_t)(jc - 1);
+    for (int m = 0; m < 35; ++m) out[m] = out[m] + s[m] * inxz;
+    if (sumgrad) {
+      double* og = sumgrad + (size_t)98 * (size_t)(jc - 1);
+      for (int m = 0; m < 98; ++m) og[m] = og[m] + q[m] * inxz;
+    }
+  }
+  {
+    /* plane nyp (:584-624) */
+    const int jc = nyp, jm = jc - 1;
+    const double cac = rdyc[jc];
+    const double Y1 = yint[jc], Y2 = 1.0 - Y1;
+    double s27 = 0.0, s33 = 0.0, s35 = 0.0, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int kc = 1; kc <= nzc; ++kc)
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const int is = ic - 2, im = ic - 1, ip = ic + 1;
+        const double prloc2 = Y1 * (ic1 * PR(is, jm, kc) + ic2 * PR(im, jm, kc) + ic3 * PR(ic, jm, kc) +
+                                    ic4 * PR(ip, jm, kc)) +
+                              Y2 * (ic1 * PR(is, jc, kc) + ic2 * PR(im, jc, kc) + ic3 * PR(ic, jc, kc) +
+                                    ic4 * PR(ip, jc, kc));
+        const double dudy = (UX(ic, jc, kc) - UX(ic, jm, kc)) * cac;
+        const double dwdy = (UZ(ic, jc, kc) - UZ(ic, jm, kc)) * cac;
+        const double vor_x = dwdy, vor_z = -dudy;
+        ACC(s27, prloc2 * (dudy + 0.0));
+        ACC(s33, vor_x * vor_x);
+        ACC(s35, vor_z * vor_z);
+        ACC(g[0], dudy);
+        ACC(g[1], dudy * dudy);
+        ACC(g[2], dudy * dudy * dudy);
+        ACC(g[3], dudy * dudy * dudy * dudy);
+        ACC(g[4], dwdy);
+        ACC(g[5], dwdy * dwdy);
+        ACC(g[6], dwdy * dwdy * dwdy);
+        ACC(g[7], dwdy * dwdy * dwdy * dwdy);
+      }
+    double* out = sumstat + (size_t)35 * (size_t)(nyp - 1);
+    out[26] = out[26] + s27 * inxz;
+    out[32] = out[32] + s33 * inxz;
+    out[34] = out[34] + s35 * inxz;
+    if (sumgrad) {
+      double* og = sumgrad + (size_t)98 * (size_t)(nyp - 1);
+      for (int t = 0; t < 4; ++t) {
+        og[6 + t] = og[6 + t] + g[t] * inxz;
+        og[30 + t] = og[30 + t] + g[4 + t] * inxz;
+      }
+    }
+  }
+#undef ACC
+}
+
+void orc4_clcStat(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                  const double* pr, double* sumstat, double* sumgrad) {
+  stat4(c, ux, uy, uz, pr, sumstat, sumgrad, 0);
+}
+
+void orc4_clcStat_mag(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                      const double* pr, double* sumstat, double* sumgrad) {
+  stat4(c, ux, uy, uz, pr, sumstat, sumgrad, 1);
+}

@@ -76,6 +76,15 @@ void orc4_CalcVmax(orc4_ctx* c, const double* ux, const double* uy, const double
                    double* vmax3);
 double orc4_CalcUxAver(orc4_ctx* c, const double* ux);
 
+/* clcStat, 35 columns and the 98 HighOrderGradStat moments (m_FlowCase.f90:
+ * 306-633): adds this call's plane averages to sumstat = SumStat(35, nyc+1)
+ * and, when not NULL, sumgrad = SumGrad(98, nyc+1), Fortran order */
+void orc4_clcStat(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                  const double* pr, double* sumstat, double* sumgrad);
+/* the same plane averages of |term| */
+void orc4_clcStat_mag(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                      const double* pr, double* sumstat, double* sumgrad);
+
 /* one substep / one time step on the context's fields (m_ChannelSystem.f90:83-149) */
 void orc4_substep(orc4_ctx* c, int ns);
 void orc4_iterate(orc4_ctx* c);

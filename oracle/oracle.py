@@ -174,6 +174,8 @@ def lib():
         L.orc4_CalcVmax.argtypes = [ctypes.c_void_p] + [_DP] * 4
         L.orc4_CalcUxAver.restype = ctypes.c_double
         L.orc4_CalcUxAver.argtypes = [ctypes.c_void_p, _DP]
+        L.orc4_clcStat.argtypes = [ctypes.c_void_p] + [_DP] * 6
+        L.orc4_clcStat_mag.argtypes = [ctypes.c_void_p] + [_DP] * 6
         _lib = L
     return _lib
 

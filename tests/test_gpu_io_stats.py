@@ -276,3 +276,81 @@ def test_restart_fourth_order_crf(tmp_path):
     for a, b in zip(back[1:], fields[1:]):
         assert np.array_equal(a, b)
     g.close()
+
+
+# ------------------------------------------------------- 4th-order statistics --
+def _periodic4(a):
+    """2-deep periodic x and z halo layers of a 4th-order halo array"""
+    nx, nz = a.shape[0] - 4, a.shape[2] - 4
+    a[0:2] = a[nx:nx + 2]
+    a[nx + 2:nx + 4] = a[2:4]
+    a[:, :, 0:2] = a[:, :, nz:nz + 2]
+    a[:, :, nz + 2:nz + 4] = a[:, :, 2:4]
+    return a
+
+
+@pytest.mark.parametrize("crf", [0.0, 0.4])
+def test_stats4_random_fields(O, crf):
+    from channel3d_amd import Channel
+    from channel3d_amd import lib as L
+    case = dict(cases.SMALL4, u_crf=crf)
+    o = O.Oracle4(case)
+    g = Channel(case, yp=o.yp)
+    rng = np.random.default_rng(8)
+    for fid, gid in ((O.UX, L.UX), (O.UY, L.UY), (O.UZ, L.UZ), (O.PR, L.PR)):
+        a = _periodic4(np.asfortranarray(rng.standard_normal(g.shape(L.HALO))))
+        o.field(fid)[...] = a
+        g.upload(gid, a)
+    g.stat_reset()
+    g.stat_set_grad(True)
+    g.clcStat()
+    g.clcStat()
+    s_g, n, _ = g.stat_get()
+    q_g = g.stat_get_grad()
+    ny = o.nyc
+    s_o = np.zeros((35, ny + 1), order="F")
+    q_o = np.zeros((98, ny + 1), order="F")
+    s_m = np.zeros((35, ny + 1), order="F")
+    q_m = np.zeros((98, ny + 1), order="F")
+    for _ in range(2):
+        o.call("clcStat", o.field(O.UX), o.field(O.UY), o.field(O.UZ), o.field(O.PR), s_o, q_o)
+        o.call("clcStat_mag", o.field(O.UX), o.field(O.UY), o.field(O.UZ), o.field(O.PR), s_m,
+               q_m)
+    assert n == 2
+    nt = o.nxc * o.nzc
+    r1, z1 = _sum_err(s_g, s_o, s_m, nt)
+    r2, z2 = _sum_err(q_g, q_o, q_m, nt)
+    print("4th-order stats error / rounding bound", r1, "grad moments", r2, flush=True)
+    assert r1 <= 1.0 and z1 and r2 <= 1.0 and z2
+    g.close()
+    o.close()
+
+
+def test_stat4_files(tmp_path, O):
+    from channel3d_amd import Channel, synth
+    from channel3d_amd import lib as L
+    case = dict(cases.SMALL4, is_ux_const=False)
+    g = Channel(case)
+    rng = np.random.default_rng(3)
+    for gid in (L.UX, L.UY, L.UZ, L.PR):
+        g.upload(gid, _periodic4(np.asfortranarray(rng.standard_normal(g.shape(L.HALO)))))
+    g.stat_reset()
+    g.stat_set_grad(True)
+    g.clcStat()
+    s_g, n, _ = g.stat_get()
+    q_g = g.stat_get_grad()
+    vp = str(tmp_path / "vGrad0000000010")
+    g.write_vgrad(vp)
+    sp = str(tmp_path / "stats0000000010")
+    g.write_stat(sp, itime=10, ivstats=10)
+    rows = [ln for ln in open(vp).read().split("\n") if ln]
+    assert len(rows) == g.nyc + 1 and all(len(ln) == 98 * 24 for ln in rows)
+    got = np.array([[float(ln[24 * q:24 * q + 24]) for q in range(98)] for ln in rows]).T
+    assert np.array_equal(got, np.array([[float("%.15E" % v) for v in col] for col in q_g]))
+    lines = open(sp).read().split("\n")
+    assert lines[2].startswith("  Variable velocity in x-dir while adding a constant body force.")
+    rdyc = g.metric(L.M_RDYC, 0, g.nyc + 1)
+    dudy = abs(s_g[0, 0]) * 2.0 * rdyc[1] + abs(s_g[0, g.nyc - 1]) * 2.0 * rdyc[g.nyc + 1]
+    assert float(lines[2][len("  Variable velocity in x-dir while adding a constant body force."):]) \
+        == float("%.15E" % (case["xnu"] * dudy / case["yly"]))
+    g.close()
