@@ -101,34 +101,59 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
   const bool active = rowok && i < nx;
   const int k0 = bz * kchunk, k1 = min(nz, k0 + kchunk);
   const T add = ADD ? (T)sc->dpmdxns : T(0);
-  const int w = 64 + 2 * mx;
-  auto xconv = [&](int kk) -> T {
+  const int w = 64 + 2 * mx;  // <= 128: two row elements per thread
+  // x positions this thread stages (periodic), fixed for the whole march
+  int ix[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int ii = (i0 - mx + tx + 64 * q) % nx;
+    ix[q] = ii < 0 ? ii + nx : ii;
+  }
+  auto wrapk = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
+  // the row of plane kk, into registers (issued one plane ahead of its use)
+  auto load = [&](int kk, T (&v)[2]) {
     const long long row = (long long)nx * jc + g.plane * kk;
-    for (int t = tx; t < w; t += 64) {
-      int ii = (i0 - mx + t) % nx;
-      if (ii < 0) ii += nx;
-      T v = T(0);
-      if (rowok) v = ADD ? f[row + ii] + add : f[row + ii];
-      sm[ty][t] = v;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      v[q] = T(0);
+      if (rowok && tx + 64 * q < w) v[q] = ADD ? f[row + ix[q]] + add : f[row + ix[q]];
     }
+  };
+  auto xconv = [&](const T (&v)[2]) -> T {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (tx + 64 * q < w) sm[ty][tx + 64 * q] = v[q];
     __syncthreads();
     T acc = 0;
     for (int t = 0; t <= 2 * mx; ++t) acc += (T)xt[t] * sm[ty][tx + t];
     __syncthreads();
     return acc;
   };
-  auto wrapk = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
   // ring of x-convolved planes: plane p sits in slot (p - k0 + MZ) mod NZT;
-  // unrolling by NZT makes every slot index a compile-time constant
+  // unrolling by NZT makes every slot index a compile-time constant.  The
+  // planes are consumed in the order k0-MZ, k0-MZ+1, ..., k1+MZ-1; cur holds
+  // the next one's row, nxt the one after it.
   T win[NZT];
+  T cur[2], nxt[2];
+  int kn = k0 - MZ;            // next plane to convolve
+  const int klast = k1 + MZ;   // one past the last plane needed
+  load(wrapk(kn), cur);
+  auto step = [&]() -> T {
+    if (kn + 1 < klast) load(wrapk(kn + 1), nxt);
+    const T r = xconv(cur);
 #pragma unroll
-  for (int q = 0; q < NZT - 1; ++q) win[q] = xconv(wrapk(k0 - MZ + q));
+    for (int q = 0; q < 2; ++q) cur[q] = nxt[q];
+    ++kn;
+    return r;
+  };
+#pragma unroll
+  for (int q = 0; q < NZT - 1; ++q) win[q] = step();
   for (int kb = k0; kb < k1; kb += NZT) {
 #pragma unroll
     for (int s = 0; s < NZT; ++s) {
       const int k = kb + s;
       if (k < k1) {
-        win[(s + NZT - 1) % NZT] = xconv(wrapk(k + MZ));
+        win[(s + NZT - 1) % NZT] = step();
         T acc = 0;
 #pragma unroll
         for (int q = 0; q < NZT; ++q) acc += (T)zt[q] * win[(s + q) % NZT];

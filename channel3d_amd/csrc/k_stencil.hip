@@ -7,7 +7,6 @@
 // ghost planes exactly as the reference reads its halo arrays.
 #include "c3d_internal.h"
 
-#include <cstdlib>
 
 namespace c3d {
 
@@ -530,19 +529,15 @@ const Metrics<double>& mets<double>(c3d_ctx* c) { return c->md; }
 template <>
 const Metrics<float>& mets<float>(c3d_ctx* c) { return c->mf; }
 
+// rows of j per RHS workgroup.  The y-neighbour rows of a tile are re-read
+// by the tiles above and below, so taller tiles read less twice; measured at
+// 768x384x576 FP64, 4 rows (4.60 ms) beat 8 (4.75 ms) and 16, which lose
+// occupancy, and a 3-workgroups-per-CU register cap (4.78 ms) did not pay.
+static constexpr int RHS_ROWS = 4;
+static int rhs_rows() { return RHS_ROWS; }
+
 // z run of the RHS kernel: long enough to amortise the two lead-in planes,
 // short enough to give the device a few thousand workgroups
-// rows of j per RHS workgroup: the y-neighbour rows of a tile are re-read
-// by the tiles above and below, so taller tiles read less twice
-static int rhs_rows() {
-  static int rows = [] {
-    const char* e = std::getenv("C3D_RHS_ROWS");
-    const int v = e ? std::atoi(e) : 0;
-    return (v == 4 || v == 8 || v == 16) ? v : 4;
-  }();
-  return rows;
-}
-
 static int rhs_kchunk(const Geom& g) {
   const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + rhs_rows() - 1) / rhs_rows());
   int nch = (4096 + tiles - 1) / tiles;
@@ -570,20 +565,15 @@ static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const voi
 #define RHS_LAUNCH(M, R)                                                                       \
   hipLaunchKernelGGL((k_rhs<T, M, R>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz), \
                      U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials, kc)
-#define RHS_ROWS(M)                         \
-  switch (rows) {                           \
-    case 8: RHS_LAUNCH(M, 8); break;        \
-    case 16: RHS_LAUNCH(M, 16); break;      \
-    default: RHS_LAUNCH(M, 4); break;       \
-  }
+#define RHS_ROWS_LAUNCH(M) RHS_LAUNCH(M, RHS_ROWS)
   switch (mask) {
-    case 1: RHS_ROWS(1); break;
-    case 2: RHS_ROWS(2); break;
-    case 4: RHS_ROWS(4); break;
-    case 7: RHS_ROWS(7); break;
+    case 1: RHS_ROWS_LAUNCH(1); break;
+    case 2: RHS_ROWS_LAUNCH(2); break;
+    case 4: RHS_ROWS_LAUNCH(4); break;
+    case 7: RHS_ROWS_LAUNCH(7); break;
     default: set_error("rhs: bad component mask"); return C3D_ERR_ARG;
   }
-#undef RHS_ROWS
+#undef RHS_ROWS_LAUNCH
 #undef RHS_LAUNCH
   C3D_HIP(hipGetLastError());
   return C3D_OK;

@@ -110,7 +110,7 @@ __device__ __forceinline__ double block_max4(double v, double* sh) {
 // accumulated as reproducible fixed-point limbs.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(B4X* B4Y) void k_rhs4(Geom g, Metrics<T> m, Coef4<T> cf,
+__global__ __launch_bounds__(B4X* B4Y, 2) void k_rhs4(Geom g, Metrics<T> m, Coef4<T> cf,
                                                    const double* __restrict__ dyp,
                                                    const T* __restrict__ ux_, const T* __restrict__ uy_,
                                                    const T* __restrict__ uz_, const T* __restrict__ pr_,

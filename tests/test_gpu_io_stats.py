@@ -147,12 +147,19 @@ def test_stat_files(tmp_path, O):
     o.close()
 
 
-def test_stats_rejects_fourth_order():
+def test_stats_entry_points_check_the_order():
+    """clcStat_CH (35 columns) belongs to the 2nd-order solver and the
+    4th-order clcStat to the 4th-order one; each rejects the other context."""
     from channel3d_amd import Channel
-    g = Channel(cases.SMALL4)
+    from channel3d_amd import lib as L
+    g4 = Channel(cases.SMALL4)
     with pytest.raises(Exception, match="2nd-order"):
-        g.clcStat()
-    g.close()
+        g4._call("clcStat_CH", L.UX, L.UY, L.UZ, L.PR)
+    g4.close()
+    g2 = Channel(cases.SMALL)
+    with pytest.raises(Exception, match="order-4"):
+        g2._call("clcStat", L.UX, L.UY, L.UZ, L.PR)
+    g2.close()
 
 
 # ------------------------------------------------------------------ restart --
