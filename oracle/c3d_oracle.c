@@ -13,7 +13,7 @@
  * depend on the thread count.
  */
 #include "c3d_oracle.h"
-#include "c3d_oracle_int.h"
+#include "c3d_oracle2_int.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -181,24 +181,6 @@ double xsum_value(const long long* acc) {
 /* ------------------------------------------------------------------------- */
 /* context                                                                   */
 /* ------------------------------------------------------------------------- */
-struct orc_ctx {
-  orc_params p;
-  int nxc, nyc, nzc, nxp, nyp, nzp;
-  size_t hx, hy, hz, nhalo, nint, nwall;
-  double dx, dy, dz, rdx, rdy, rdz, rdx2, rdy2, rdz2;
-  double* mbuf[ORC_NMETRICS];   /* owning buffers                         */
-  double* met[ORC_NMETRICS];    /* index-shifted views                    */
-  double ac1c, am1c, ap1c, ac3c, am3c, ap3c; /* x/z metrics (periodic: const) */
-  double normfft;
-  double gconst[3], tconst[3], aconst[3];
-  int iadvance;
-  double pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, SimTime, dt;
-  double PrGradNow, PrGradAver;
-  double divmax1, divmax2, cfl, uxaver, vmax[3];
-  int ppe_route;
-  double* f[ORC_NFIELDS];
-  fftplan px, pz;
-};
 
 static int g_threads = 0;
 void orc_set_threads(int nthreads) { g_threads = nthreads; }
@@ -209,14 +191,6 @@ int orc_get_threads(void) {
   return 1;
 #endif
 }
-#ifdef _OPENMP
-#define OMP_NT num_threads(orc_get_threads())
-#endif
-
-#define HIDX(c, i, j, k) ((size_t)(i) + (c)->hx * ((size_t)(j) + (c)->hy * (size_t)(k)))
-#define IIDX(c, i, j, k) \
-  ((size_t)((i)-1) + (size_t)(c)->nxc * ((size_t)((j)-1) + (size_t)(c)->nyc * (size_t)((k)-1)))
-#define WIDX(c, i, k) ((size_t)((i)-1) + (size_t)(c)->nxc * (size_t)((k)-1))
 
 /* m_MeshAndMetries.f90:292-360 (clcCoord) */
 static void clc_coord(double* coordinate, const int ncoor[2], const double sec[2], int stret_type,
@@ -320,6 +294,7 @@ double orc_scalar(orc_ctx* c, int id) {
     case ORC_S_VMAX2: return c->vmax[1];
     case ORC_S_VMAX3: return c->vmax[2];
     case ORC_S_DT: return c->dt;
+    case ORC_S_PMBETAT: return c->pmBetaT;
   }
   return 0.0;
 }
@@ -347,6 +322,10 @@ orc_ctx* orc_create(const orc_params* p, const double* yp_in) {
   for (int d = 2; d <= 3; ++d)
     if (p->bc[d] != ORC_BC_NOSLIP && p->bc[d] != ORC_BC_FREESLIP) return NULL;
   if (p->nxc < 2 || p->nyc < 3 || p->nzc < 2) return NULL;
+  /* m_LESModels.f90:35-41 (FilterType), m_TScheme.f90:179-206 (LES runs the
+   * partially implicit scheme, channel walls only) */
+  if (p->les_type < 0 || p->les_type > 3) return NULL;
+  if (p->les_type > 0 && p->filter_type != 0 && p->filter_type != 1) return NULL;
   orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
   c->p = *p;
   const int nxc = p->nxc, nyc = p->nyc, nzc = p->nzc;
@@ -389,6 +368,7 @@ orc_ctx* orc_create(const orc_params* p, const double* yp_in) {
   double* ap2cCN = met_alloc(c, ORC_AP2CCN, 0, nyc);
   double* wx = met_alloc(c, ORC_WAVEX, 0, nxc);
   double* wz = met_alloc(c, ORC_WAVEZ, 0, nzc);
+  double* delta = met_alloc(c, ORC_DELTA, 0, nyp);
 
   for (int j = 1; j <= nyp; ++j) yp[j] = yp_in[j];
   /* m_MeshAndMetries.f90:171-225 */
@@ -404,6 +384,8 @@ orc_ctx* orc_create(const orc_params* p, const double* yp_in) {
   for (int j = 0; j <= nyp; ++j) rdyp[j] = 1.0 / dyp[j];
   for (int j = 1; j <= nyp; ++j) rdyc[j] = 1.0 / dyc[j];
   for (int j = 1; j <= nyp; ++j) yint[j] = dyp[j] / (dyp[j] + dyp[j - 1]);
+  /* :202-205  DeltaCell = (dyp dx dz)^(1/3) */
+  for (int j = 0; j <= nyp; ++j) delta[j] = pow(dyp[j] * c->dx * c->dz, 1.0 / 3.0);
   /* :227-240 */
   for (int j = 1; j <= nyc; ++j) {
     am2Pr[j] = rdyp[j] * rdyc[j];
@@ -481,6 +463,13 @@ orc_ctx* orc_create(const orc_params* p, const double* yp_in) {
   c->f[ORC_HALO] = (double*)calloc(c->nhalo, sizeof(double));
   c->f[ORC_UY_YM] = (double*)calloc(c->nwall, sizeof(double));
   c->f[ORC_DUY_YM] = (double*)calloc(c->nwall, sizeof(double));
+  /* :71-74 nut = 0 when LES_type > 0 */
+  if (p->les_type > 0) c->f[ORC_NUT] = (double*)calloc(c->nhalo, sizeof(double));
+  /* m_LESModels.f90:34-41 */
+  static const double trap[9] = {1, 2, 1, 2, 4, 2, 1, 2, 1};
+  static const double simp[9] = {1, 4, 1, 4, 16, 4, 1, 4, 1};
+  for (int q = 0; q < 9; ++q)
+    c->fcoe[q] = p->filter_type == 1 ? simp[q] / 36.0 : trap[q] / 16.0;
   fft_plan_init(&c->px, nxc);
   fft_plan_init(&c->pz, nzc);
   return c;
@@ -504,6 +493,7 @@ void orc_PMcoeUpdate(orc_ctx* c, int ns) {
   c->pmBeta = 0.5 * c->aconst[ns - 1] * c->p.xnu * dt;
   c->SimTime = c->SimTime + dt * c->aconst[ns - 1];
   c->pmAlphaC = c->aconst[ns - 1];
+  c->pmBetaT = 0.5 * c->aconst[ns - 1] * dt;
   if (ns == 1) c->PrGradAver = 0.0;
 }
 
@@ -927,7 +917,7 @@ void orc_clcU3Hat(orc_ctx* c, double* uz, double* rhsz) {
 /* halos and boundary conditions (m_BC_and_Halo.f90, m_Decomp2d.f90)          */
 /* ------------------------------------------------------------------------- */
 /* myupdate_halo on one rank, x and z periodic, hi%{x,z}{m,p}h given */
-static void halo_xz(orc_ctx* c, double* a, int xmh, int xph, int zmh, int zph) {
+void orc2_halo_xz(orc_ctx* c, double* a, int xmh, int xph, int zmh, int zph) {
   const int nxc = c->nxc, nzc = c->nzc;
   const int ymm = 0, ypm = c->nyc + 2, zmm = 0, zpm = nzc + 2;
   /* step3/4 (x): over all y and z of the allocated array (:1652-1700) */
@@ -975,9 +965,9 @@ void orc_SetBC_and_UpdateHalo(orc_ctx* c, double* ux, double* uy, double* uz, co
         UZ(ic, 0, kc) = UZ(ic, 1, kc) - c->p.uzbc[ym] * dyp[1];
       }
     }
-  halo_xz(c, ux, 1, 1, 1, 1);
-  halo_xz(c, uy, 1, 1, 1, 1);
-  halo_xz(c, uz, 1, 1, 1, 1);
+  orc2_halo_xz(c, ux, 1, 1, 1, 1);
+  orc2_halo_xz(c, uy, 1, 1, 1, 1);
+  orc2_halo_xz(c, uz, 1, 1, 1, 1);
 }
 
 /* m_BC_and_Halo.f90:349-405 */
@@ -990,9 +980,9 @@ void orc_SetBC_and_UpdateHaloForPrSrc(orc_ctx* c, double* ux, double* uy, double
       UY(ic, nyp, kc) = c->p.uybc[3];
       UY(ic, 1, kc) = uy_ym[WIDX(c, ic, kc)];
     }
-  halo_xz(c, ux, 0, 1, 0, 0);
+  orc2_halo_xz(c, ux, 0, 1, 0, 0);
   /* uy: no halo (y not periodic) */
-  halo_xz(c, uz, 0, 0, 0, 1);
+  orc2_halo_xz(c, uz, 0, 0, 0, 1);
 }
 
 /* m_BC_and_Halo.f90:410-456 */
@@ -1003,7 +993,7 @@ void orc_SetBC_and_UpdateHalo_pr(orc_ctx* c, double* pr) {
       PR(ic, nyp, kc) = PR(ic, nyc, kc);
       PR(ic, 0, kc) = PR(ic, 1, kc);
     }
-  halo_xz(c, pr, 1, 1, 1, 1);
+  orc2_halo_xz(c, pr, 1, 1, 1, 1);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1285,6 +1275,10 @@ double orc_CalcUxAver(orc_ctx* c, const double* ux) {
 /* m_ChannelSystem.f90:91-125 one substep, :73-130 one step (iCFL=2)          */
 /* ------------------------------------------------------------------------- */
 void orc_substep(orc_ctx* c, int ns) {
+  if (c->p.les_type > 0) {
+    orc2_substep_les(c, ns);
+    return;
+  }
   double *ux = c->f[ORC_UX], *uy = c->f[ORC_UY], *uz = c->f[ORC_UZ], *pr = c->f[ORC_PR];
   double *arr1 = c->f[ORC_ARR1], *arr2 = c->f[ORC_ARR2], *halo = c->f[ORC_HALO];
   orc_PMcoeUpdate(c, ns);
@@ -1308,6 +1302,7 @@ void orc_substep(orc_ctx* c, int ns) {
 void orc_iterate(orc_ctx* c) {
   double *ux = c->f[ORC_UX], *uy = c->f[ORC_UY], *uz = c->f[ORC_UZ];
   c->cfl = orc_CalcMaxCFL(c, ux, uy, uz) * c->dt;
+  if (c->p.les_type > 0) orc_Clc_SGS_Vis(c, ux, uy, uz, c->f[ORC_NUT]);
   for (int ns = 1; ns <= c->iadvance; ++ns) orc_substep(c, ns);
   c->divmax2 = orc_CheckDivergence(c, ux, uy, uz);
   orc_CalcVmax(c, ux, uy, uz, c->vmax);

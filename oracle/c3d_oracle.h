@@ -32,6 +32,8 @@ typedef struct orc_params {
   int is_ux_const;           /* IsUxConst (m_TScheme.f90:16)  */
   double ubulk, ybulk1, ybulk2;
   int ischeme;               /* 1 AB2, 2 RK2, 3 RK3 (m_Parameters.f90:135-150) */
+  int les_type;              /* LES_type: 0 none, 2 WALE, 3 MTS (m_Parameters.f90:102) */
+  int filter_type;           /* FilterType: 0 trapezoidal, 1 Simpson (:103)            */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -43,6 +45,7 @@ enum {
   ORC_ARR1, ORC_ARR2,                          /* interior scratch      */
   ORC_HALO,                                    /* halo scratch          */
   ORC_UY_YM, ORC_DUY_YM,                       /* wall planes           */
+  ORC_NUT,                                     /* halo array (LES only) */
   ORC_NFIELDS
 };
 
@@ -55,6 +58,7 @@ enum {
   ORC_AM2P, ORC_AC2P, ORC_AP2P,
   ORC_AM2CCN, ORC_AP2CCN,
   ORC_WAVEX, ORC_WAVEZ,
+  ORC_DELTA,                                   /* DeltaCell(0:nyp) */
   ORC_NMETRICS
 };
 
@@ -63,7 +67,7 @@ enum {
   ORC_S_PMGAMMA = 0, ORC_S_PMTHETA, ORC_S_PMALPHA, ORC_S_PMALPHAC, ORC_S_PMBETA,
   ORC_S_SIMTIME, ORC_S_PRGRADNOW, ORC_S_PRGRADAVER, ORC_S_NORMFFT,
   ORC_S_DIVMAX1, ORC_S_DIVMAX2, ORC_S_CFL, ORC_S_UXAVER,
-  ORC_S_VMAX1, ORC_S_VMAX2, ORC_S_VMAX3, ORC_S_DT,
+  ORC_S_VMAX1, ORC_S_VMAX2, ORC_S_VMAX3, ORC_S_DT, ORC_S_PMBETAT,
   ORC_NSCALARS
 };
 
@@ -121,6 +125,37 @@ void orc_clcStat_CH(orc_ctx* c, const double* ux, const double* uy, const double
 /* the same plane averages of |term|: the scale of each sum's rounding error */
 void orc_clcStat_CH_mag(orc_ctx* c, const double* ux, const double* uy, const double* uz,
                         const double* pr, double* sumstat);
+
+/* ---- LES layer (m_LESModels.f90, m_TSchemePIMP_inc.f90 *_PIMP_LES) ----
+ * A context with les_type > 0 steps with the partially implicit LES scheme
+ * (y-only Crank-Nicolson with the eddy viscosity, m_TScheme.f90:185-206):
+ * orc_substep and orc_iterate dispatch to it, and orc_iterate evaluates
+ * nut once per step before the substeps (m_ChannelSystem.f90:89).
+ * nut is the halo array ORC_NUT. */
+void orc_Clc_SGS_Vis(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                     double* nut);
+void orc_Clc_SGS_WALE(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                      double* nut);
+void orc_Clc_SGS_MTS(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                     double* nut);
+/* |S| = sqrt(2 Sij Sij) at cell centres, interior array (m_LESModels.f90:307-354) */
+void orc_ClcVelStrainRate(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                          double* strain);
+void orc_SetBC_and_UpdateHalo_SGS(orc_ctx* c, double* nut);
+void orc_clcRhsX_les(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                     double* rhsx, double* histx, const double* pr, const double* nut);
+void orc_clcRhsY_les(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                     double* rhsy, double* histy, const double* pr, const double* nut);
+void orc_clcRhsZ_les(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                     double* rhsz_halo, double* histz, const double* pr, const double* nut);
+void orc_clcU1Hat_les(orc_ctx* c, double* ux, double* rhsx, const double* nut);
+void orc_clcU2Hat_les(orc_ctx* c, double* uy, double* rhsy, const double* duy_ym,
+                      const double* nut);
+void orc_clcU3Hat_les(orc_ctx* c, double* uz, double* rhsz_halo, const double* nut);
+/* clcPrSrcOther (m_TScheme.f90:212-241): prsrc = div/pmAlpha, p untouched */
+void orc_clcPrSrcOther(orc_ctx* c, const double* ux, const double* uy, const double* uz,
+                       double* prsrc, double* divmax);
+void orc_PressureUpdate_les(orc_ctx* c, double* pr, const double* phi, const double* nut);
 
 /* one substep / one full time step on the context's own fields
  * (m_ChannelSystem.f90:91-125 / :73-130), dt fixed (iCFL=2) */

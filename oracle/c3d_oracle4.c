@@ -230,6 +230,7 @@ orc4_ctx* orc4_create(const orc4_params* p, const double* yp_in) {
   c->dt = p->dt;
 
   for (int id = 0; id < ORC_NFIELDS; ++id) {
+    if (id == ORC_NUT) continue; /* the LES field of the 2nd-order context */
     size_t n = (id <= ORC_PR || id == ORC_HALO) ? c->nhalo
              : (id == ORC_UY_YM || id == ORC_DUY_YM) ? c->nwall : c->nint;
     c->f[id] = (double*)calloc(n, sizeof(double));

@@ -25,21 +25,21 @@ _LIB_PATH = os.path.join(_HERE, "libc3d_oracle.so")
 BC_PERIOD, BC_NOSLIP, BC_FREESLIP = 0, -1, -2
 
 # field ids (c3d_oracle.h)
-UX, UY, UZ, PR, HISTX, HISTY, HISTZ, ARR1, ARR2, HALO, UY_YM, DUY_YM = range(12)
-HALO_FIELDS = (UX, UY, UZ, PR, HALO)
+UX, UY, UZ, PR, HISTX, HISTY, HISTZ, ARR1, ARR2, HALO, UY_YM, DUY_YM, NUT = range(13)
+HALO_FIELDS = (UX, UY, UZ, PR, HALO, NUT)
 INTERIOR_FIELDS = (HISTX, HISTY, HISTZ, ARR1, ARR2)
 WALL_FIELDS = (UY_YM, DUY_YM)
 
 (M_YP, M_YC, M_DYP, M_DYC, M_RDYP, M_RDYC, M_YINTERP, M_AM2PR, M_AC2PR, M_AP2PR,
  M_AM2C, M_AC2C, M_AP2C, M_AM2P, M_AC2P, M_AP2P, M_AM2CCN, M_AP2CCN,
- M_WAVEX, M_WAVEZ) = range(20)
+ M_WAVEX, M_WAVEZ, M_DELTA) = range(21)
 
 (S_PMGAMMA, S_PMTHETA, S_PMALPHA, S_PMALPHAC, S_PMBETA, S_SIMTIME, S_PRGRADNOW,
  S_PRGRADAVER, S_NORMFFT, S_DIVMAX1, S_DIVMAX2, S_CFL, S_UXAVER, S_VMAX1, S_VMAX2,
- S_VMAX3, S_DT) = range(17)
+ S_VMAX3, S_DT, S_PMBETAT) = range(18)
 
 # 4th-order oracle (c3d_oracle4.h)
-M4_AC2CCN = 20
+M4_AC2CCN = 21
 FT_CH, FT_HC = 1, 2
 
 
@@ -55,6 +55,7 @@ class OrcParams(ctypes.Structure):
         ("is_ux_const", ctypes.c_int),
         ("ubulk", ctypes.c_double), ("ybulk1", ctypes.c_double), ("ybulk2", ctypes.c_double),
         ("ischeme", ctypes.c_int),
+        ("les_type", ctypes.c_int), ("filter_type", ctypes.c_int),
     ]
 
 
@@ -136,6 +137,17 @@ def lib():
         L.orc_InverseTridiagonal.argtypes = [_DP, _DP, _DP, _DP, ctypes.c_int, ctypes.c_int]
         L.orc_InversePTriFixedCoe.argtypes = [ctypes.c_double] * 3 + [_DP, ctypes.c_int,
                                                                        ctypes.c_int]
+        for name in ("orc_Clc_SGS_Vis", "orc_Clc_SGS_WALE", "orc_Clc_SGS_MTS",
+                     "orc_ClcVelStrainRate"):
+            getattr(L, name).argtypes = [ctypes.c_void_p] + [_DP] * 4
+        L.orc_SetBC_and_UpdateHalo_SGS.argtypes = [ctypes.c_void_p, _DP]
+        for name in ("orc_clcRhsX_les", "orc_clcRhsY_les", "orc_clcRhsZ_les"):
+            getattr(L, name).argtypes = [ctypes.c_void_p] + [_DP] * 7
+        L.orc_clcU1Hat_les.argtypes = [ctypes.c_void_p, _DP, _DP, _DP]
+        L.orc_clcU2Hat_les.argtypes = [ctypes.c_void_p, _DP, _DP, _DP, _DP]
+        L.orc_clcU3Hat_les.argtypes = [ctypes.c_void_p, _DP, _DP, _DP]
+        L.orc_clcPrSrcOther.argtypes = [ctypes.c_void_p] + [_DP] * 5
+        L.orc_PressureUpdate_les.argtypes = [ctypes.c_void_p, _DP, _DP, _DP]
         L.orc_xsum.restype = ctypes.c_double
         L.orc_xsum.argtypes = [ctypes.c_int, _DP]
         L.orc_r2hc.argtypes = [ctypes.c_int, _DP, _DP]
@@ -277,6 +289,8 @@ class Oracle:
         p.ybulk1 = case.get("ybulk1", 0.0)
         p.ybulk2 = case.get("ybulk2", case["yly"])
         p.ischeme = case.get("ischeme", 3)
+        p.les_type = case.get("les_type", 0)
+        p.filter_type = case.get("filter_type", 1)
         self.params = p
         if yp is None:
             yp = channel_yp(p.nyc, p.yly, case.get("gamma", 1.5))
