@@ -49,6 +49,7 @@ PER_FILE = {
     "k_adi.hip": ["-ffp-contract=off"],
     "k_ppe.hip": ["-ffp-contract=off"],
     "k_stats.hip": ["-ffp-contract=off"],
+    "k_les.hip": ["-ffp-contract=off"],
 }
 
 

@@ -104,6 +104,8 @@ class Channel:
             v = ctypes.c_void_p()
             L.check(self.lib.c3d_state_field(self._h, fid, ctypes.byref(v)), "c3d_state_field")
             self._state[fid] = v
+        if case.get("les_type", 0):
+            self.set_les(case["les_type"], case.get("filter_type", 1))
 
     # lifetime ----------------------------------------------------------------
     def close(self):
@@ -180,6 +182,18 @@ class Channel:
     def set_ppe_refine(self, sweeps):
         """-1: FP64 pressure solve; s >= 0: FP32 solve + s FP64-residual sweeps"""
         L.check(self.lib.c3d_set_ppe_refine(self._h, int(sweeps)), "c3d_set_ppe_refine")
+
+    def set_les(self, les_type, filter_type=1):
+        """LES_type 0 (off), 2 (WALE) or 3 (MTS); FilterType 0 trapezoidal, 1 Simpson
+        (InitFilterCoe and the LES branch of InitTimeScheme)"""
+        L.check(self.lib.c3d_set_les(self._h, int(les_type), int(filter_type)), "c3d_set_les")
+        if les_type:
+            v = ctypes.c_void_p()
+            L.check(self.lib.c3d_state_field(self._h, L.NUT, ctypes.byref(v)), "c3d_state_field")
+            self._state[L.NUT] = v
+
+    def Clc_SGS_Vis(self, ux=L.UX, uy=L.UY, uz=L.UZ, nut=L.NUT):
+        self._call("Clc_SGS_Vis", ux, uy, uz, nut)
 
     # reference routines --------------------------------------------------------
     def _call(self, name, *fields):

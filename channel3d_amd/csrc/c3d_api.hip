@@ -243,6 +243,8 @@ static void compute_metrics(c3d_ctx* c, const double* yp_in) {
     ap2p[j] = rdyc[j] * rdyp[j];
     ac2p[j] = -(am2p[j] + ap2p[j]);
   }
+  // DeltaCell (m_MeshAndMetries.f90:202-205)
+  for (int j = 0; j <= nyp; ++j) c->met[C3D_M_DELTA][j] = std::pow(dyp[j] * c->dx * c->dz, 1.0 / 3.0);
   auto& accn = c->met[C3D_M_AC2CCN];
   for (int j = 1; j <= ny; ++j) {
     amcn[j] = am2c[j];
@@ -644,7 +646,8 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
     if (c->state[i]) hipFree(c->state[i]);
   for (void* p : c->user_fields) hipFree(p);
   void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
-                  c->d_dyp, c->d_spike, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64};
+                  c->d_dyp, c->d_spike, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
+                  c->d_lesmet};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (c->staging) hipHostFree(c->staging);
@@ -805,6 +808,7 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   const size_t fbytes = (size_t)g.nfield * c->esize;
   const size_t wbytes = (size_t)g.nx * g.nz * c->esize;
   for (int i = 0; i < C3D_NSTATE; ++i) {
+    if (i == C3D_NUT) continue;  // LES only (c3d_set_les)
     const size_t b = (i == C3D_UY_YM || i == C3D_DUY_YM) ? wbytes : fbytes;
     if ((rc = dalloc(c, &c->state[i], b))) return fail(rc);
     e = hipMemset(c->state[i], 0, b);
@@ -936,6 +940,10 @@ extern "C" int c3d_field_copy(c3d_ctx* c, int layout, void* dst, const void* src
 extern "C" int c3d_state_field(c3d_ctx* c, int id, void** dptr) {
   if (!c || !dptr || id < 0 || id >= C3D_NSTATE) {
     set_error("c3d_state_field: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  if (!c->state[id]) {
+    set_error("c3d_state_field: nut exists only after c3d_set_les");
     return C3D_ERR_ARG;
   }
   *dptr = c->state[id];
@@ -1091,6 +1099,7 @@ extern "C" int c3d_PMcoeUpdate(c3d_ctx* c, int ns) {
   c->pmBeta = 0.5 * c->aconst[ns - 1] * c->p.xnu * dt;
   c->SimTime = c->SimTime + dt * c->aconst[ns - 1];
   c->pmAlphaC = c->aconst[ns - 1];
+  c->pmBetaT = 0.5 * c->aconst[ns - 1] * dt;  // for LES purpose only (:184)
   c->cur_ns = ns;
   return C3D_OK;
 }
@@ -1111,7 +1120,9 @@ extern "C" int c3d_clcRhsX(c3d_ctx* c, const void* ux, const void* uy, const voi
                            void* hx, const void* pr) {
   CHECK_CTX(c);
   if (int e = only_order2(c, "c3d_clcRhsX")) return e;
-  int rc = launch_rhs(c, 1, ux, uy, uz, pr, rx, nullptr, nullptr, hx, nullptr, nullptr, 1);
+  int rc = c->les_type
+               ? launch_rhs_les(c, 1, ux, uy, uz, pr, rx, nullptr, nullptr, hx, nullptr, nullptr, 1)
+               : launch_rhs(c, 1, ux, uy, uz, pr, rx, nullptr, nullptr, hx, nullptr, nullptr, 1);
   if (rc) return rc;
   if (c->p.is_ux_const) return launch_add_dpmdxns(c, rx);
   return C3D_OK;
@@ -1121,6 +1132,8 @@ extern "C" int c3d_clcRhsY(c3d_ctx* c, const void* ux, const void* uy, const voi
                            void* hy, const void* pr) {
   CHECK_CTX(c);
   if (int e = only_order2(c, "c3d_clcRhsY")) return e;
+  if (c->les_type)
+    return launch_rhs_les(c, 2, ux, uy, uz, pr, nullptr, ry, nullptr, nullptr, hy, nullptr, 0);
   return launch_rhs(c, 2, ux, uy, uz, pr, nullptr, ry, nullptr, nullptr, hy, nullptr, 0);
 }
 
@@ -1128,6 +1141,8 @@ extern "C" int c3d_clcRhsZ(c3d_ctx* c, const void* ux, const void* uy, const voi
                            void* hz, const void* pr) {
   CHECK_CTX(c);
   if (int e = only_order2(c, "c3d_clcRhsZ")) return e;
+  if (c->les_type)
+    return launch_rhs_les(c, 4, ux, uy, uz, pr, nullptr, nullptr, rz, nullptr, nullptr, hz, 0);
   return launch_rhs(c, 4, ux, uy, uz, pr, nullptr, nullptr, rz, nullptr, nullptr, hz, 0);
 }
 
@@ -1147,16 +1162,19 @@ extern "C" int c3d_clcRhsXYZ(c3d_ctx* c, const void* ux, const void* uy, const v
 
 extern "C" int c3d_clcU1Hat(c3d_ctx* c, void* ux, void* rx) {
   CHECK_CTX(c);
+  if (c->les_type) return launch_les_y(c, 0, ux, rx, nullptr, 0);
   return launch_adi(c, 0, c->cur_ns, ux, rx, nullptr, 0);
 }
 
 extern "C" int c3d_clcU2Hat(c3d_ctx* c, void* uy, void* ry, const void* duy_ym) {
   CHECK_CTX(c);
+  if (c->les_type) return launch_les_y(c, 1, uy, ry, duy_ym, 0);
   return launch_adi(c, 1, c->cur_ns, uy, ry, duy_ym, 0);
 }
 
 extern "C" int c3d_clcU3Hat(c3d_ctx* c, void* uz, void* rz) {
   CHECK_CTX(c);
+  if (c->les_type) return launch_les_y(c, 2, uz, rz, nullptr, 0);
   return launch_adi(c, 2, c->cur_ns, uz, rz, nullptr, 0);
 }
 
@@ -1188,8 +1206,9 @@ static int read_scalars(c3d_ctx* c) {
 extern "C" int c3d_clcPrSrc(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
                             void* prsrc, void* pr, double* divmax) {
   CHECK_CTX(c);
+  // the LES scheme uses clcPrSrcOther (m_TScheme.f90:213-241): no pressure correction
   int rc = c->order == 4 ? launch_prsrc4(c, ux, uy, uz, prsrc, 0.0)
-                         : launch_prsrc(c, ux, uy, uz, prsrc, pr, 0.0);
+                         : launch_prsrc(c, ux, uy, uz, prsrc, c->les_type ? nullptr : pr, 0.0);
   if (rc) return rc;
   if (divmax) {
     if ((rc = read_scalars(c))) return rc;
@@ -1217,6 +1236,7 @@ extern "C" int c3d_FluidVelUpdate(c3d_ctx* c, const void* phi, void* ux, void* u
 extern "C" int c3d_PressureUpdate(c3d_ctx* c, void* pr, const void* phi) {
   CHECK_CTX(c);
   if (c->order == 4) return launch_proj4(c, phi, nullptr, nullptr, nullptr, pr, 0, 1, 0);
+  if (c->les_type) return launch_prupd_les(c, pr, phi);
   return launch_proj(c, phi, nullptr, nullptr, nullptr, pr, 0, 1, 0);
 }
 
@@ -1298,9 +1318,36 @@ static int substep4(c3d_ctx* c, int ns) {
   return launch_bc_pr(c, s[C3D_PR]);
 }
 
+// one substep of the LES scheme (m_ChannelSystem.f90:91-125 with the
+// pointers of m_TScheme.f90:185-203); nut was evaluated at the step start
+static int substep_les(c3d_ctx* c, int ns) {
+  int rc;
+  void** s = c->state;
+  if ((rc = c3d_PMcoeUpdate(c, ns))) return rc;
+  if ((rc = launch_update_uy_ym(c, s[C3D_UY_YM], s[C3D_DUY_YM]))) return rc;
+  if ((rc = launch_rhs_les(c, 7, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], s[C3D_ARR1],
+                           s[C3D_ARR2], s[C3D_HALOARR], s[C3D_HISTX], s[C3D_HISTY], s[C3D_HISTZ],
+                           1)))
+    return rc;
+  // the constant-flow-rate shift of RhsX is applied as the ux solve loads it
+  if ((rc = launch_les_y(c, 0, s[C3D_UX], s[C3D_ARR1], nullptr, c->p.is_ux_const))) return rc;
+  if ((rc = launch_les_y(c, 1, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
+  if ((rc = launch_les_y(c, 2, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
+  if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
+  if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], nullptr, 0.0))) return rc;
+  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  if ((rc = launch_bc_pr(c, s[C3D_HALOARR]))) return rc;
+  if ((rc = launch_proj(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr, 1, 0, 0)))
+    return rc;
+  if ((rc = launch_prupd_les(c, s[C3D_PR], s[C3D_HALOARR]))) return rc;
+  if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 1))) return rc;
+  return launch_bc_pr(c, s[C3D_PR]);
+}
+
 extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   CHECK_CTX(c);
   if (c->order == 4) return substep4(c, ns);
+  if (c->les_type) return substep_les(c, ns);
   int rc;
   void** s = c->state;
   if ((rc = c3d_PMcoeUpdate(c, ns))) return rc;
@@ -1335,6 +1382,10 @@ extern "C" int c3d_step(c3d_ctx* c) {
   if (!c->cfl_valid) {
     if ((rc = launch_diag(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], 1, c->cfl_slot))) return rc;
   }
+  // Clc_SGS_Vis once per step, before the substeps (:89)
+  if (c->les_type &&
+      (rc = launch_sgs(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_NUT])))
+    return rc;
   for (int ns = 1; ns <= c->iadvance; ++ns)
     if ((rc = c3d_substep(c, ns))) return rc;
   // CheckDivergence, CalcVmax (:127-130) and the next step's CFL, one pass
@@ -1395,13 +1446,98 @@ extern "C" int c3d_project(c3d_ctx* c) {
 }
 
 // ===========================================================================
+// LES layer (m_LESModels.f90, m_TScheme.f90:179-206)
+// ===========================================================================
+template <typename T>
+static void fill_lesmet(c3d_ctx* c, std::vector<T>& h, LesMet<T>& lm, T* d) {
+  const size_t L = (size_t)c->g.ny + 3;
+  for (size_t j = 0; j < L; ++j) {
+    const double dl = c->met[C3D_M_DELTA][j];
+    h[j] = (T)dl;
+    h[L + j] = (T)(dl != 0.0 ? 1.0 / dl : 0.0);
+    h[2 * L + j] = (T)c->met[C3D_M_AM2CCN][j];
+    h[3 * L + j] = (T)c->met[C3D_M_AP2CCN][j];
+  }
+  lm.delta = d;
+  lm.rdelta = d + L;
+  lm.am2ccn = d + 2 * L;
+  lm.ap2ccn = d + 3 * L;
+}
+
+extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
+  CHECK_CTX(c);
+  if (c->order != 2) {
+    set_error("c3d_set_les: the LES layer belongs to the 2nd-order solver");
+    return C3D_ERR_ARG;
+  }
+  if (les_type == 1) {
+    set_error("c3d_set_les: LES_type 1 (Smagorinsky) is an empty case in the reference "
+              "(m_LESModels.f90:53-54); use 2 (WALE) or 3 (MTS)");
+    return C3D_ERR_ARG;
+  }
+  if (les_type < 0 || les_type > 3) {
+    set_error("c3d_set_les: LES_type must be 0, 2 or 3");
+    return C3D_ERR_ARG;
+  }
+  if (les_type == 0) {
+    c->les_type = 0;
+    return C3D_OK;
+  }
+  if (filter_type != 0 && filter_type != 1) {
+    set_error("c3d_set_les: unrecognized FilterType (0 trapezoidal, 1 Simpson)");
+    return C3D_ERR_ARG;
+  }
+  int rc;
+  if (!c->state[C3D_NUT]) {
+    const size_t fbytes = (size_t)c->g.nfield * c->esize;
+    if ((rc = dalloc(c, &c->state[C3D_NUT], fbytes))) return rc;
+    C3D_HIP(hipMemset(c->state[C3D_NUT], 0, fbytes));  // nut = zero (m_Variables.f90:71-74)
+  }
+  if (!c->d_lesmet) {
+    const size_t L = (size_t)c->g.ny + 3;
+    if ((rc = dalloc(c, &c->d_lesmet, 4 * L * c->esize))) return rc;
+    if (c->prec == C3D_FP64) {
+      std::vector<double> h(4 * L);
+      fill_lesmet<double>(c, h, c->lmd, (double*)c->d_lesmet);
+      C3D_HIP(hipMemcpy(c->d_lesmet, h.data(), 4 * L * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> h(4 * L);
+      fill_lesmet<float>(c, h, c->lmf, (float*)c->d_lesmet);
+      C3D_HIP(hipMemcpy(c->d_lesmet, h.data(), 4 * L * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  // InitFilterCoe (m_LESModels.f90:34-41)
+  static const double trap[9] = {1, 2, 1, 2, 4, 2, 1, 2, 1};
+  static const double simp[9] = {1, 4, 1, 4, 16, 4, 1, 4, 1};
+  for (int q = 0; q < 9; ++q) c->les_fcoe[q] = filter_type == 1 ? simp[q] / 36.0 : trap[q] / 16.0;
+  c->les_type = les_type;
+  c->filter_type = filter_type;
+  return C3D_OK;
+}
+
+extern "C" int c3d_Clc_SGS_Vis(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                               void* nut) {
+  CHECK_CTX(c);
+  if (!c->les_type) {
+    set_error("c3d_Clc_SGS_Vis: LES is off (c3d_set_les)");
+    return C3D_ERR_ARG;
+  }
+  if (!ux || !uy || !uz || !nut) {
+    set_error("c3d_Clc_SGS_Vis: null field");
+    return C3D_ERR_ARG;
+  }
+  return launch_sgs(c, ux, uy, uz, nut);
+}
+
+// ===========================================================================
 // profiling
 // ===========================================================================
 static const char* kStageNames[ST_COUNT] = {
     "rhs_fused", "rhs_single", "bulk_finalize", "adi_x", "adi_z", "adi_xz", "adi_y_update",
     "bc_ghosts",
     "prsrc", "fft_x_fwd", "fft_z_fwd", "tridiag_y", "fft_z_inv", "fft_x_inv", "projection",
-    "vel_update", "pr_update", "diagnostics", "misc", "statistics", "ppe_residual"};
+    "vel_update", "pr_update", "diagnostics", "misc", "statistics", "ppe_residual",
+    "sgs_viscosity"};
 
 extern "C" int c3d_profile_enable(c3d_ctx* c, int on) {
   CHECK_CTX(c);

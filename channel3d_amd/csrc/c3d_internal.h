@@ -136,12 +136,23 @@ struct Coef4 {
   T pmGamma, pmTheta, pmAlpha, pmBeta;
 };
 
+// LES metric vectors, indexed by the reference j (0 .. ny+2): DeltaCell
+// (m_MeshAndMetries.f90:202-205), its reciprocal, and the wall-folded
+// Crank-Nicolson metrics am2cForCN / ap2cForCN (m_TScheme.f90:73-85)
+template <typename T>
+struct LesMet {
+  const T* delta;
+  const T* rdelta;
+  const T* am2ccn;
+  const T* ap2ccn;
+};
+
 enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1, YS_PARTITIONED = 2 };
 
 enum Stage {
   ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_XZ, ST_ADI_Y, ST_BC, ST_PRSRC,
   ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
-  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_STAT, ST_RESID, ST_COUNT
+  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_STAT, ST_RESID, ST_SGS, ST_COUNT
 };
 
 }  // namespace c3d
@@ -165,7 +176,7 @@ struct c3d_ctx {
   // RK tables and current substep scalars
   double gconst[3], tconst[3], aconst[3];
   int iadvance;
-  double dt, pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, SimTime;
+  double dt, pmGamma, pmTheta, pmAlpha, pmAlphaC, pmBeta, pmBetaT, SimTime;
   int cur_ns;
 
   // device metric storage (one block per precision in use)
@@ -224,6 +235,15 @@ struct c3d_ctx {
   float* resid32 = nullptr;
   double* d_mean64 = nullptr; // the FP64 (0,0)-mode solve: 3 ny work, gauge value at
                               // [3 ny], gauge partial sums from [3 (ny+1)]
+
+  // LES layer (c3d_set_les): LES_type (0 off, 2 WALE, 3 MTS), FilterType and
+  // its weights, the metric block; the eddy viscosity is state[C3D_NUT]
+  int les_type = 0;
+  int filter_type = 0;
+  double les_fcoe[9] = {};
+  void* d_lesmet = nullptr;
+  c3d::LesMet<double> lmd{};
+  c3d::LesMet<float> lmf{};
 
   // channel statistics: SumStat(35, ny+1), PrGradsum (+7 spare), then the
   // per-workgroup partial rows; allocated on first use
@@ -304,6 +324,17 @@ int launch_proj4(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, void
 int launch_bc4_vel(c3d_ctx* c, void* ux, void* uy, void* uz, const void* uy_ym, int full);
 // CheckDivergence with the 4th-order x/z differences -> DevScalars::divmax2
 int launch_div4(c3d_ctx* c, const void* ux, const void* uy, const void* uz);
+
+// ---- launchers (k_les.hip, LES layer of the 2nd-order path) ----
+// eddy viscosity of the context's model into nut, wall ghosts included
+int launch_sgs(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* nut);
+int launch_rhs_les(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
+                   const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
+                   int want_bulk);
+// comp 0 ux, 1 uy (wall row from duy_ym), 2 uz; the right-hand side is scratch
+int launch_les_y(c3d_ctx* c, int comp, void* u, void* rhs, const void* duy_ym, int add_dpmdxns);
+// pressure += (1 - pmBetaT d/dy nut d/dy) phi; phi's wall ghosts must be set
+int launch_prupd_les(c3d_ctx* c, void* pr, const void* phi);
 
 // ---- launchers (k_stats.hip) ----
 int launch_stat(c3d_ctx* c, const void* ux, const void* uy, const void* uz, const void* pr);

@@ -20,15 +20,15 @@ FP64, FP32 = 0, 1
 HALO, INTERIOR, WALL = 0, 1, 2
 
 # state field ids
-(UX, UY, UZ, PR, HISTX, HISTY, HISTZ, ARR1, ARR2, HALOARR, UY_YM, DUY_YM) = range(12)
+(UX, UY, UZ, PR, HISTX, HISTY, HISTZ, ARR1, ARR2, HALOARR, UY_YM, DUY_YM, NUT) = range(13)
 STATE_LAYOUT = {UX: HALO, UY: HALO, UZ: HALO, PR: HALO, HISTX: INTERIOR, HISTY: INTERIOR,
                 HISTZ: INTERIOR, ARR1: INTERIOR, ARR2: INTERIOR, HALOARR: HALO,
-                UY_YM: WALL, DUY_YM: WALL}
+                UY_YM: WALL, DUY_YM: WALL, NUT: HALO}
 
 # metric ids
 (M_YP, M_YC, M_DYP, M_DYC, M_RDYP, M_RDYC, M_YINTERP, M_AM2PR, M_AC2PR, M_AP2PR,
  M_AM2C, M_AC2C, M_AP2C, M_AM2P, M_AC2P, M_AP2P, M_AM2CCN, M_AP2CCN,
- M_WAVEX, M_WAVEZ, M_AC2CCN) = range(21)
+ M_WAVEX, M_WAVEZ, M_AC2CCN, M_DELTA) = range(22)
 
 EXPORTS = (
     "c3d_create", "c3d_destroy", "c3d_last_error", "c3d_sync", "c3d_mesh_coords",
@@ -44,7 +44,7 @@ EXPORTS = (
     "c3d_write_restart", "c3d_read_restart", "c3d_restart_name",
     "c3d_clcStat_CH", "c3d_stat_get", "c3d_stat_reset", "c3d_write_stat", "c3d_append_prgrad",
     "c3d_set_ppe_refine", "c3d_clcStat", "c3d_stat_set_grad", "c3d_stat_get_grad",
-    "c3d_write_vgrad",
+    "c3d_write_vgrad", "c3d_set_les", "c3d_Clc_SGS_Vis",
 )
 
 
@@ -146,6 +146,8 @@ def load(path: str | None = None):
         "c3d_stat_set_grad": ([_VP, I], I),
         "c3d_stat_get_grad": ([_VP, _DP], I),
         "c3d_write_vgrad": ([_VP, ctypes.c_char_p], I),
+        "c3d_set_les": ([_VP, I, I], I),
+        "c3d_Clc_SGS_Vis": ([_VP] * 5, I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -108,6 +108,7 @@ enum {
   C3D_M_AM2P, C3D_M_AC2P, C3D_M_AP2P, C3D_M_AM2CCN, C3D_M_AP2CCN,
   C3D_M_WAVEX, C3D_M_WAVEZ,
   C3D_M_AC2CCN, /* ac2cForCN of the 4th-order code (m_TScheme.f90:41-51) */
+  C3D_M_DELTA,  /* DeltaCell = (dyp dx dz)^(1/3), j = 0..nyc+1 (m_MeshAndMetries.f90:202-205) */
   C3D_NMETRICS
 };
 
@@ -140,6 +141,7 @@ enum {
   C3D_ARR1, C3D_ARR2,                       /* interior scratch (RealArr1/2) */
   C3D_HALOARR,                              /* halo scratch (RealHalo) */
   C3D_UY_YM, C3D_DUY_YM,                    /* wall planes */
+  C3D_NUT,                                  /* halo: eddy viscosity, LES only */
   C3D_NSTATE
 };
 int c3d_state_field(c3d_ctx* ctx, int id, void** dptr);
@@ -222,6 +224,32 @@ int c3d_profile_count(c3d_ctx* ctx);
 int c3d_profile_get(c3d_ctx* ctx, int idx, const char** name, long* calls, double* total_ms,
                     double* bytes_per_call);
 size_t c3d_device_bytes(c3d_ctx* ctx);
+
+/* ----------------------------------------------------------- LES layer -- */
+/* The 2nd-order code's large-eddy simulation (namelist LesOptions,
+ * src/CFD_2nd/m_Parameters.f90:102-121).  c3d_set_les(ctx, LES_type,
+ * FilterType) is InitFilterCoe (m_LESModels.f90:19-42) plus the LES branch of
+ * InitTimeScheme (m_TScheme.f90:179-206): LES_type 2 is the WALE model, 3 the
+ * mixed-time-scale model with the 3x3 x-z test filter of FilterType 0
+ * (trapezoidal) or 1 (Simpson); 0 switches LES off.  LES_type 1 is rejected:
+ * it is an empty case in the reference (m_LESModels.f90:53-54).  With LES on,
+ * the context owns the eddy viscosity nut (state field C3D_NUT, zero at
+ * first, a halo field) and the time scheme is the partially implicit LES
+ * one, exactly as the reference swaps its procedure pointers:
+ *   c3d_clcRhsX/Y/Z    clcRhsX/Y/Z_PIMP_LES (m_TSchemePIMP_inc.f90:80-156,
+ *                      214-272, 330-387), reading the context's nut
+ *   c3d_clcU1/2/3Hat   clcU1/2/3Hat_PIMP_LES (:466-499, 626-662, 776-809)
+ *   c3d_clcPrSrc       clcPrSrcOther (m_TScheme.f90:213-241): no pressure
+ *                      correction, pressure may be NULL
+ *   c3d_PressureUpdate PressureUpdate_PIMP_LES (:878-904); prphiHalo's wall
+ *                      ghosts must be set (SetBC_and_UpdateHalo_pr)
+ *   c3d_step           evaluates nut once before the substeps
+ *                      (m_ChannelSystem.f90:89); c3d_substep does not.
+ * 2nd-order contexts only. */
+int c3d_set_les(c3d_ctx* ctx, int les_type, int filter_type);
+/* Clc_SGS_Vis (m_LESModels.f90:47-62): nut from the velocity (halo fields
+ * with valid ghosts), including SetBC_and_UpdateHalo_SGS's wall ghosts */
+int c3d_Clc_SGS_Vis(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz, void* nut);
 
 /* ------------------------------------------------------- restart files -- */
 /* The reference's restart file (write_restart / read_restart,
