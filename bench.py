@@ -129,6 +129,45 @@ def run_fourth_order(steps, warmup):
                 divmax_after_step=d["divmax2"], stages=stages)
 
 
+def run_les(case, steps, warmup, label):
+    """the LES step (WALE or MTS, partially implicit scheme) on the same GPU:
+    steps/s and the per-stage split (inputs resident, HIP events on the
+    library stream)"""
+    from channel3d_amd import Channel, synth, lib as L
+    nx, ny, nz = case["nxc"], case["nyc"], case["nzc"]
+    g = Channel(case)
+    ux, uy, uz = synth.laminar_noise(nx, ny, nz, g.yp, yly=case["yly"], ubulk=1.0, seed=0)
+    g.upload(L.UX, ux)
+    g.upload(L.UY, uy)
+    g.upload(L.UZ, uz)
+    del ux, uy, uz
+    g.project()
+    for _ in range(warmup):
+        g.step()
+    g.sync()
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step()
+    g.sync()
+    t = time.perf_counter() - t0
+    st = g.profile_stats()
+    g.profile(False)
+    d = g.diagnostics()
+    g.close()
+    stages = {k: dict(calls=v["calls"], avg_ms=round(v["total_ms"] / v["calls"], 4),
+                      gbs=round(v["bytes_per_call"] / (v["total_ms"] / v["calls"] * 1e-3) / 1e9, 1)
+                      if v["bytes_per_call"] else None) for k, v in st.items()}
+    model = {2: "WALE", 3: "MTS"}[case["les_type"]]
+    return dict(workload=f"{label}: 2nd-order LES channel RK3 step {nx}x{ny}x{nz}, {model}, "
+                         "partially implicit scheme, FP64",
+                steps=steps, ms_per_step=round(1e3 * t / steps, 3),
+                steps_per_s=round(steps / t, 2),
+                grid_points_steps_per_s=round(nx * ny * nz * steps / t, 1),
+                divmax_after_step=d["divmax2"], stages=stages)
+
+
 def run_precision_sweep(g, L, nsolves, fp64_per_s, nsteps):
     """FP64 pressure solve against FP32 + s refinement sweeps on the same
     source (the last step's): solves/s and relative max-norm difference of
@@ -184,6 +223,8 @@ def main(argv=None):
     ap.add_argument("--small", action="store_true", help="64^3 workload (quick check)")
     ap.add_argument("--no-fourth", action="store_true",
                     help="skip the 4th-order 192x128x160 line reported beside the main one")
+    ap.add_argument("--no-les", action="store_true",
+                    help="skip the LES step measurements (96x96x72 and 384x192x288)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32 + refinement pressure-solve sweep")
     ap.add_argument("--probe", choices=["copy"], default=None,
@@ -314,6 +355,16 @@ def main(argv=None):
         log("4th-order step 192x128x160")
         fourth = run_fourth_order(max(a.steps, 20), max(a.warmup, 2))
         log(f"4th-order: {fourth['ms_per_step']} ms/step")
+    les = None
+    if not a.no_les and not a.small:
+        from tests import cases as C
+        les = {}
+        for key, case_l, lbl in (("les96_wale", C.LES96, "LES grid"),
+                                 ("les96_mts", dict(C.LES96, les_type=3), "LES grid"),
+                                 ("les384_wale", C.LES384, "DNS-resolution box")):
+            log(f"LES step {key}")
+            les[key] = run_les(case_l, max(a.steps, 20), max(a.warmup, 2), lbl)
+            log(f"{key}: {les[key]['ms_per_step']} ms/step")
 
     out = {
         "metric": METRIC,
@@ -340,6 +391,7 @@ def main(argv=None):
         "stages": stage_rows,
         "divmax_after_step": d1["divmax2"],
         "fourth_order": fourth,
+        "les": les,
         "precision_sweep": precision,
     }
     print(json.dumps(out), flush=True)

@@ -24,6 +24,7 @@ module channel3d_c
   integer(c_int), parameter :: C3D_HISTX = 4, C3D_HISTY = 5, C3D_HISTZ = 6
   integer(c_int), parameter :: C3D_ARR1 = 7, C3D_ARR2 = 8, C3D_HALOARR = 9
   integer(c_int), parameter :: C3D_UY_YM = 10, C3D_DUY_YM = 11
+  integer(c_int), parameter :: C3D_NUT = 12
 
   ! the module globals the reference reads (m_Parameters.f90:21-63,
   ! m_TScheme.f90:16-17); layout identical to struct c3d_params
@@ -239,6 +240,17 @@ module channel3d_c
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: sweeps
+    end function
+
+    ! ---- LES layer (LesOptions, m_LESModels.f90, m_TScheme.f90:179-206)
+    integer(c_int) function c3d_set_les(ctx, les_type, filter_type) bind(C, name="c3d_set_les")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: les_type, filter_type
+    end function
+    integer(c_int) function c3d_Clc_SGS_Vis(ctx, ux, uy, uz, nut) bind(C, name="c3d_Clc_SGS_Vis")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, nut
     end function
 
     ! ---- restart files (m_IOAndVisu.f90:352-558); path: NUL-terminated

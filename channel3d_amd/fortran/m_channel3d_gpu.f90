@@ -36,6 +36,7 @@ module m_channel3d_gpu
   public :: gpu_CalcMaxCFL, gpu_CheckDivergence, gpu_CalcVmax, gpu_CalcUxAver
   public :: gpu_upload_state, gpu_download_state, gpu_ChannelIterate
   public :: gpu_write_restart, gpu_read_restart, gpu_clcStat, gpu_write_stat, gpu_append_prgrad
+  public :: gpu_set_les, gpu_Clc_SGS_Vis
 
 contains
 
@@ -215,6 +216,28 @@ contains
     call c3d_check(c3d_PressureUpdate(ctx, dh(4), dh(6)), "PressureUpdate")
     call down(C3D_HALO, dh(4), pressure)
   end subroutine gpu_PressureUpdate
+
+  ! ---------------------------------------------------------- LES layer --
+  ! InitFilterCoe (m_LESModels.f90:19-42) and the LES procedure pointers of
+  ! InitTimeScheme (m_TScheme.f90:179-206): after this call gpu_clcRhsX/Y/Z,
+  ! gpu_clcU1/2/3Hat, gpu_clcPrSrc and gpu_PressureUpdate are the *_PIMP_LES
+  ! routines and clcPrSrcOther, reading the eddy viscosity the last
+  ! gpu_Clc_SGS_Vis left on the device (the reference's module array nut)
+  subroutine gpu_set_les(LES_type, FilterType)
+    integer, intent(in) :: LES_type, FilterType
+    call c3d_check(c3d_set_les(ctx, int(LES_type, c_int), int(FilterType, c_int)), "c3d_set_les")
+  end subroutine gpu_set_les
+
+  ! Clc_SGS_Vis (m_LESModels.f90:47-62); nut is the halo array
+  subroutine gpu_Clc_SGS_Vis(ux, uy, uz, nut)
+    real(c_double), intent(in) :: ux(*), uy(*), uz(*)
+    real(c_double), intent(inout) :: nut(*)
+    type(c_ptr) :: d
+    call up(C3D_HALO, dh(1), ux); call up(C3D_HALO, dh(2), uy); call up(C3D_HALO, dh(3), uz)
+    call c3d_check(c3d_state_field(ctx, C3D_NUT, d), "state")
+    call c3d_check(c3d_Clc_SGS_Vis(ctx, dh(1), dh(2), dh(3), d), "Clc_SGS_Vis")
+    call down(C3D_HALO, d, nut)
+  end subroutine gpu_Clc_SGS_Vis
 
   ! m_Tools.f90:21-104, 250-270
   subroutine gpu_CalcMaxCFL(ux, uy, uz, uddxmax)

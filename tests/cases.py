@@ -30,3 +30,11 @@ SMALL4 = dict(C2_4TH, nxc=32, nyc=24, nzc=24)
 # half channel (free-slip top), odd-sized FFT lengths, stronger stretching
 HALF4 = dict(nxc=48, nyc=20, nzc=36, xlx=3.0, yly=1.0, zlz=2.0, xnu=1.0 / 1500.0, dt=0.004,
              gamma=2.0, ubulk=1.0, order=4, flow_type=2)
+
+# LES grid of the learned-closure configuration: Re_tau=395 channel filtered
+# from 384x192x288 by 4 (x, z) and 2 (y) -> 96x96x72, Lx=2pi, Lz=pi,
+# Re_bulk ~ 13750 (nu = 2/Re_b), WALE eddy viscosity
+LES96 = dict(nxc=96, nyc=96, nzc=72, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
+             xnu=2.0 / 13750.0, dt=0.005, gamma=1.5, ubulk=1.0, les_type=2, filter_type=1)
+# the same box at the DNS resolution, with the LES scheme (kernel throughput)
+LES384 = dict(LES96, nxc=384, nyc=192, nzc=288, dt=0.002)

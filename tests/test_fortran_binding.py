@@ -6,7 +6,8 @@ __graft_entry__.build() with amdflang) runs one RK substep through the
 per-routine wrappers on host arrays, in the reference's call order and
 scratch aliasing, then several time steps on the device-resident state,
 takes a statistics sample and writes the stats and restart files, and
-finally reads the restart file into a fresh context.
+reads the restart file into a fresh context, and finally runs the LES
+scheme (WALE) the same two ways.
 """
 import os
 import subprocess
@@ -44,7 +45,7 @@ def test_fortran_call_sites_match_oracle(oracle_mod, tmp_path):
     init = [o.field(f).copy(order="F") for f in (O.UX, O.UY, O.UZ, O.PR)]
     (tmp_path / "case.txt").write_text(
         f"{nx} {ny} {nz}\n{case['xlx']!r} {case['yly']!r} {case['zlz']!r}\n"
-        f"{case['xnu']!r} {case['dt']!r}\n{nsteps}\n")
+        f"{case['xnu']!r} {case['dt']!r}\n{nsteps}\n2 1\n")
     with open(tmp_path / "in.bin", "wb") as f:
         np.asarray(o.yp, dtype=np.float64).tofile(f)
         for a in init:
@@ -57,8 +58,8 @@ def test_fortran_call_sites_match_oracle(oracle_mod, tmp_path):
     shp = (nx + 3, ny + 3, nz + 3)
     n = int(np.prod(shp))
     out = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
-    assert out.size == 12 * n
-    got = [out[q * n:(q + 1) * n].reshape(shp, order="F") for q in range(12)]
+    assert out.size == 21 * n
+    got = [out[q * n:(q + 1) * n].reshape(shp, order="F") for q in range(21)]
     # 1. one substep through the per-routine wrappers
     o.substep(1)
     for q, fid in enumerate((O.UX, O.UY, O.UZ, O.PR)):
@@ -89,5 +90,29 @@ def test_fortran_call_sites_match_oracle(oracle_mod, tmp_path):
     rows = [ln for ln in lines[5:] if ln.strip()]
     assert len(rows) == ny + 1 and all(len(ln) == 35 * 24 for ln in rows)
     assert float(rows[ny // 2][:24]) > 0.5
+    # 4. LES (WALE, Simpson filter): nut, one per-routine substep, nsteps steps
+    les = dict(case, les_type=2, filter_type=1)
+    o3 = O.Oracle(les)
+    for fid, a in zip((O.UX, O.UY, O.UZ, O.PR), init):
+        o3.field(fid)[...] = a
+    o3.call("Clc_SGS_Vis", o3.field(O.UX), o3.field(O.UY), o3.field(O.UZ), o3.field(O.NUT))
+    ref = _ymid(o3.field(O.NUT))[:, :ny + 2]
+    assert np.abs(_ymid(got[12])[:, :ny + 2] - ref).max() / np.abs(ref).max() < 1e-14
+    o3.substep(1)
+    for q, fid in enumerate((O.UX, O.UY, O.UZ, O.PR)):
+        ref = _ymid(o3.field(fid))
+        err = np.abs(_ymid(got[13 + q]) - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, ("les substep", q, err)
+    o4 = O.Oracle(les)
+    for fid, a in zip((O.UX, O.UY, O.UZ, O.PR), init):
+        o4.field(fid)[...] = a
+    for _ in range(nsteps):
+        o4.iterate()
+    for q, fid in enumerate((O.UX, O.UY, O.UZ, O.PR)):
+        ref = _ymid(o4.field(fid))
+        err = np.abs(_ymid(got[17 + q]) - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, ("les steps", q, err)
     o.close()
     o2.close()
+    o3.close()
+    o4.close()
