@@ -1158,3 +1158,173 @@ void orc4_clcStat_mag(orc4_ctx* c, const double* ux, const double* uy, const dou
                       const double* pr, double* sumstat, double* sumgrad) {
   stat4(c, ux, uy, uz, pr, sumstat, sumgrad, 1);
 }
+
+/* ------------------------------------------------------------------------- */
+/* 1-D energy spectra, cospectra and linear coherence spectra of clcStat     */
+/* (m_FlowCase.f90:644-842 with clcEnergySpectraX/Z :1152-1234,              */
+/* clcCospectraX/Z :1181-1263, clcLTS_x/z :1268-1632, transform_uy           */
+/* :2303-2332).  On one rank the x- and z-pencil arrays are the interior     */
+/* array itself, so the half-complex transforms are taken in place along x   */
+/* (axis 0) or z (axis 2) of interior-shaped arrays.                         */
+/* ------------------------------------------------------------------------- */
+/* element of an interior array at transform index q (1-based, along the     */
+/* axis), row j and other index o (k for axis 0, i for axis 2)               */
+static inline size_t sp_idx(const orc4_ctx* c, int axis, int q, int j, int o) {
+  return axis == 0 ? I4(c, q, j, o) : I4(c, o, j, q);
+}
+
+/* FFTW_R2HC of every line along the axis */
+static void sp_r2hc(orc4_ctx* c, int axis, const double* in, double* out) {
+  const int n = axis == 0 ? c->nxc : c->nzc, m = axis == 0 ? c->nzc : c->nxc;
+  const fftplan* pl = axis == 0 ? &c->px : &c->pz;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int j = 1; j <= c->nyc; ++j) {
+    double* line = (double*)malloc(sizeof(double) * (size_t)n);
+    double* res = (double*)malloc(sizeof(double) * (size_t)n);
+    double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)n);
+    for (int o = 1; o <= m; ++o) {
+      for (int q = 1; q <= n; ++q) line[q - 1] = in[sp_idx(c, axis, q, j, o)];
+      orci_r2hc(pl, line, res, wk);
+      for (int q = 1; q <= n; ++q) out[sp_idx(c, axis, q, j, o)] = res[q - 1];
+    }
+    free(line); free(res); free(wk);
+  }
+}
+
+/* 1/(nxc*nxc*nzc) for x, 1/(nxc*nzc*nzc) for z */
+static double sp_norm(const orc4_ctx* c, int axis) {
+  return axis == 0 ? 1.0 / ((double)c->nxc * (double)c->nxc * (double)c->nzc)
+                   : 1.0 / ((double)c->nxc * (double)c->nzc * (double)c->nzc);
+}
+
+/* clcCospectraX/Z (clcEnergySpectraX/Z with h1 = h2): column mcol of spec */
+static void sp_co(orc4_ctx* c, int axis, const double* h1, const double* h2, double* spec,
+                  int mcol) {
+  const int n = axis == 0 ? c->nxc : c->nzc, m = axis == 0 ? c->nzc : c->nxc;
+  const int nh = n / 2, nhp = nh + 1, ny = c->nyc;
+  const double norm = sp_norm(c, axis);
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+  for (int j = 1; j <= ny; ++j) {
+    double* E = (double*)calloc((size_t)nhp + 1, sizeof(double));
+    for (int o = 1; o <= m; ++o) {
+      E[1] = E[1] + h1[sp_idx(c, axis, 1, j, o)] * h2[sp_idx(c, axis, 1, j, o)];
+      E[nhp] = E[nhp] + h1[sp_idx(c, axis, nhp, j, o)] * h2[sp_idx(c, axis, nhp, j, o)];
+      for (int q = 2; q <= nh; ++q) {
+        const int qc = n + 2 - q;
+        E[q] = E[q] + (h1[sp_idx(c, axis, q, j, o)] * h2[sp_idx(c, axis, q, j, o)] +
+                       h1[sp_idx(c, axis, qc, j, o)] * h2[sp_idx(c, axis, qc, j, o)]) * 2.0;
+      }
+    }
+    double* S = spec + (size_t)nhp * ((size_t)(j - 1) + (size_t)ny * (size_t)(mcol - 1));
+    for (int q = 1; q <= nhp; ++q) S[q - 1] = S[q - 1] + E[q] * norm;
+    free(E);
+  }
+}
+
+/* the reference row of row j: jForLCS for the lower half of a channel, its
+ * mirror nyc+1-jForLCS for the upper half; the half channel uses jForLCS */
+static inline int sp_ref_row(const orc4_ctx* c, int j, int jr) {
+  if (c->p.flow_type == ORC4_FT_HC) return jr;
+  return j <= c->nyc / 2 ? jr : c->nyc + 1 - jr;
+}
+
+/* clcLTS_x / clcLTS_z: real part against jForLCS(1) (column n1), imaginary
+ * part against jForLCS(1) (n2), real part against jForLCS(2) (n3) */
+static void sp_lts(orc4_ctx* c, int axis, const double* h, double* spec, int n1, int n2, int n3,
+                   const int* jlcs) {
+  const int n = axis == 0 ? c->nxc : c->nzc, m = axis == 0 ? c->nzc : c->nxc;
+  const int nh = n / 2, nhp = nh + 1, ny = c->nyc;
+  const double norm = sp_norm(c, axis);
+  for (int pass = 0; pass < 3; ++pass) {
+    const int col = pass == 0 ? n1 : pass == 1 ? n2 : n3;
+    const int jr = pass == 2 ? jlcs[1] : jlcs[0];
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (int j = 1; j <= ny; ++j) {
+      const int r = sp_ref_row(c, j, jr);
+      double* E = (double*)calloc((size_t)nhp + 1, sizeof(double));
+      for (int o = 1; o <= m; ++o) {
+#define HV(q, jj) h[sp_idx(c, axis, q, jj, o)]
+        if (pass != 1) {
+          E[1] = E[1] + HV(1, j) * HV(1, r);
+          E[nhp] = E[nhp] + HV(nhp, j) * HV(nhp, r);
+          for (int q = 2; q <= nh; ++q) {
+            const int qc = n + 2 - q;
+            E[q] = E[q] + (HV(q, j) * HV(q, r) + HV(qc, j) * HV(qc, r));
+          }
+        } else {
+          /* the row's coefficients swapped (q <-> n+2-q), times the
+           * unswapped reference row */
+          for (int q = 2; q <= nh; ++q) {
+            const int qc = n + 2 - q;
+            E[q] = E[q] + (HV(qc, j) * HV(q, r) - HV(q, j) * HV(qc, r));
+          }
+        }
+#undef HV
+      }
+      double* S = spec + (size_t)nhp * ((size_t)(j - 1) + (size_t)ny * (size_t)(col - 1));
+      for (int q = 1; q <= nhp; ++q) S[q - 1] = S[q - 1] + E[q] * norm;
+      free(E);
+    }
+  }
+}
+
+void orc4_clcSpectra1D(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                       const double* pr, const int* jlcs, double* specx, double* specz) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const double uCRF = c->p.u_crf;
+  const size_t N = c->nint;
+  double* fux = (double*)malloc(sizeof(double) * N);   /* ux + uCRF          */
+  double* fuc = (double*)malloc(sizeof(double) * N);   /* ArrTemp1: ux centre */
+  double* fvc = (double*)malloc(sizeof(double) * N);   /* uy centre           */
+  double* fvt = (double*)malloc(sizeof(double) * N);   /* transform_uy        */
+  double* fw = (double*)malloc(sizeof(double) * N);
+  double* fp = (double*)malloc(sizeof(double) * N);
+  double* h1 = (double*)malloc(sizeof(double) * N);
+  double* h2 = (double*)malloc(sizeof(double) * N);
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int jc = 1; jc <= nyc; ++jc) {
+      /* :2303-2332 */
+      int jt;
+      if (c->p.flow_type == ORC4_FT_CH) jt = jc > nyc / 2 ? jc : jc + 1;
+      else jt = jc + 1;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const size_t q = I4(c, ic, jc, kc);
+        fux[q] = ux[H4(c, ic, jc, kc)] + uCRF;
+        /* :645-656 */
+        fuc[q] = c->InterpCoe1 * ux[H4(c, ic - 1, jc, kc)] + c->InterpCoe2 * ux[H4(c, ic, jc, kc)] +
+                 c->InterpCoe3 * ux[H4(c, ic + 1, jc, kc)] + c->InterpCoe4 * ux[H4(c, ic + 2, jc, kc)] +
+                 uCRF;
+        /* :673-682 */
+        fvc[q] = 0.5 * (uy[H4(c, ic, jc, kc)] + uy[H4(c, ic, jc + 1, kc)]);
+        fvt[q] = (c->p.flow_type == ORC4_FT_HC && jc == nyc) ? 0.0 : uy[H4(c, ic, jt, kc)];
+        fw[q] = uz[H4(c, ic, jc, kc)];
+        fp[q] = pr[H4(c, ic, jc, kc)];
+      }
+    }
+  for (int axis = 0; axis <= 2; axis += 2) {
+    double* spec = axis == 0 ? specx : specz;
+    sp_r2hc(c, axis, fux, h1);
+    sp_co(c, axis, h1, h1, spec, 1);
+    sp_lts(c, axis, h1, spec, 7, 8, 9, jlcs);
+    sp_r2hc(c, axis, fvc, h2);
+    sp_co(c, axis, h1, h2, spec, 6);
+    sp_r2hc(c, axis, fuc, h1);
+    sp_co(c, axis, h1, h2, spec, 5);
+    sp_r2hc(c, axis, fvt, h1);
+    sp_co(c, axis, h1, h1, spec, 2);
+    sp_lts(c, axis, h1, spec, 10, 11, 12, jlcs);
+    sp_r2hc(c, axis, fw, h1);
+    sp_co(c, axis, h1, h1, spec, 3);
+    sp_lts(c, axis, h1, spec, 13, 14, 15, jlcs);
+    sp_r2hc(c, axis, fp, h1);
+    sp_co(c, axis, h1, h1, spec, 4);
+    sp_lts(c, axis, h1, spec, 16, 17, 18, jlcs);
+  }
+  free(fux); free(fuc); free(fvc); free(fvt); free(fw); free(fp); free(h1); free(h2);
+}

@@ -91,6 +91,17 @@ void orc4_iterate(orc4_ctx* c);
 /* divergence-free projection of the context's velocity, unit time factor */
 void orc4_project(orc4_ctx* c);
 
+/* the 1-D spectra of clcStat (m_FlowCase.f90:644-842): adds one sample to
+ * specx = EnergySpecX(nxc/2+1, nyc, 18) and specz = EnergySpecZ(nzc/2+1,
+ * nyc, 18), Fortran order.  Columns 1-4 uu, vv, ww, pp energy; 5 the uv
+ * cospectrum with ux interpolated to the cell centre, 6 with ux on its face;
+ * 7-9, 10-12, 13-15, 16-18 the linear coherence spectra of u, v, w, p
+ * (real part against row jlcs[0], imaginary part against jlcs[0], real part
+ * against jlcs[1]; the upper half of a channel against the mirror rows).
+ * Fields are 4th-order halo arrays with valid halos. */
+void orc4_clcSpectra1D(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                       const double* pr, const int* jlcs, double* specx, double* specz);
+
 #ifdef __cplusplus
 }
 #endif
