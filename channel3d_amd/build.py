@@ -50,6 +50,7 @@ PER_FILE = {
     "k_ppe.hip": ["-ffp-contract=off"],
     "k_stats.hip": ["-ffp-contract=off"],
     "k_les.hip": ["-ffp-contract=off"],
+    "k_spec.hip": ["-ffp-contract=off"],
 }
 
 

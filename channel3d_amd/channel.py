@@ -183,6 +183,27 @@ class Channel:
         """-1: FP64 pressure solve; s >= 0: FP32 solve + s FP64-residual sweeps"""
         L.check(self.lib.c3d_set_ppe_refine(self._h, int(sweeps)), "c3d_set_ppe_refine")
 
+    def clcSpectra1D(self, jlcs=(1, 2), ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR):
+        """one sample of the 4th-order 1-D spectra (clcStat with clcSpectra1D)"""
+        L.check(self.lib.c3d_clcSpectra1D(self._h, *[self.ptr(f) for f in (ux, uy, uz, pressure)],
+                                          int(jlcs[0]), int(jlcs[1])), "c3d_clcSpectra1D")
+
+    def spectra_get(self):
+        """(EnergySpecX (nxc/2+1, nyc, 18), EnergySpecZ (nzc/2+1, nyc, 18), nSpectime)"""
+        sx = np.zeros((self.nxc // 2 + 1, self.nyc, L.NSPEC1D), order="F")
+        sz = np.zeros((self.nzc // 2 + 1, self.nyc, L.NSPEC1D), order="F")
+        n = ctypes.c_int()
+        L.check(self.lib.c3d_spectra_get(self._h, L.dptr(sx), L.dptr(sz), ctypes.byref(n)),
+                "c3d_spectra_get")
+        return sx, sz, n.value
+
+    def spectra_reset(self):
+        L.check(self.lib.c3d_spectra_reset(self._h), "c3d_spectra_reset")
+
+    def write_spectra(self, path_x, path_z):
+        L.check(self.lib.c3d_write_spectra(self._h, os.fsencode(path_x), os.fsencode(path_z)),
+                "c3d_write_spectra")
+
     def set_les(self, les_type, filter_type=1):
         """LES_type 0 (off), 2 (WALE) or 3 (MTS); FilterType 0 trapezoidal, 1 Simpson
         (InitFilterCoe and the LES branch of InitTimeScheme)"""

@@ -245,6 +245,13 @@ struct c3d_ctx {
   c3d::LesMet<double> lmd{};
   c3d::LesMet<float> lmf{};
 
+  // 1-D spectra of the 4th-order statistics: EnergySpecX (nx/2+1, ny, 18)
+  // then EnergySpecZ (nz/2+1, ny, 18), a second spectral buffer for the
+  // cospectra, and nSpectime; allocated on first use
+  double* d_spec1d = nullptr;
+  void* spec2 = nullptr;
+  int nspectime = 0;
+
   // channel statistics: SumStat(35, ny+1), PrGradsum (+7 spare), then the
   // per-workgroup partial rows; allocated on first use
   double* d_stat = nullptr;

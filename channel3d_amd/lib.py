@@ -15,6 +15,7 @@ C3D_OK = 0
 C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_NOMEM, C3D_ERR_UNSUPPORTED, C3D_ERR_IO = -1, -2, -3, -4, -5
 NCHASTAT = 35
 NGRADSTAT = 98
+NSPEC1D = 18
 BC_PERIOD, BC_NOSLIP, BC_FREESLIP = 0, -1, -2
 FP64, FP32 = 0, 1
 HALO, INTERIOR, WALL = 0, 1, 2
@@ -45,6 +46,7 @@ EXPORTS = (
     "c3d_clcStat_CH", "c3d_stat_get", "c3d_stat_reset", "c3d_write_stat", "c3d_append_prgrad",
     "c3d_set_ppe_refine", "c3d_clcStat", "c3d_stat_set_grad", "c3d_stat_get_grad",
     "c3d_write_vgrad", "c3d_set_les", "c3d_Clc_SGS_Vis",
+    "c3d_clcSpectra1D", "c3d_spectra_get", "c3d_spectra_reset", "c3d_write_spectra",
 )
 
 
@@ -148,6 +150,10 @@ def load(path: str | None = None):
         "c3d_write_vgrad": ([_VP, ctypes.c_char_p], I),
         "c3d_set_les": ([_VP, I, I], I),
         "c3d_Clc_SGS_Vis": ([_VP] * 5, I),
+        "c3d_clcSpectra1D": ([_VP] * 5 + [I, I], I),
+        "c3d_spectra_get": ([_VP, _DP, _DP, _IP], I),
+        "c3d_spectra_reset": ([_VP], I),
+        "c3d_write_spectra": ([_VP, ctypes.c_char_p, ctypes.c_char_p], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

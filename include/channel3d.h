@@ -225,6 +225,30 @@ int c3d_profile_get(c3d_ctx* ctx, int idx, const char** name, long* calls, doubl
                     double* bytes_per_call);
 size_t c3d_device_bytes(c3d_ctx* ctx);
 
+/* ------------------------------------------------- 4th-order 1-D spectra -- */
+/* The clcSpectra1D part of the 4th-order clcStat (src/CFD_4th/m_FlowCase.f90:
+ * 644-842, namelist SpectraOptions :190): one call adds one sample of the
+ * streamwise and spanwise spectra EnergySpecX(nxc/2+1, nyc, 18) and
+ * EnergySpecZ(nzc/2+1, nyc, 18), kept on the device.  Columns: 1-4 the uu,
+ * vv, ww, pp energy spectra (ux + uCRF on its face, uy on the node away from
+ * the nearer wall, uz, p); 5 and 6 the uv cospectra with ux interpolated to
+ * the cell centre and on its face; 7-9, 10-12, 13-15, 16-18 the linear
+ * coherence spectra of u, v, w, p: real part against row jForLCS1, imaginary
+ * part against jForLCS1, real part against jForLCS2 (the upper half of a
+ * channel against the mirror rows nyc+1-jForLCS).  FP64 order-4 contexts;
+ * fields as for c3d_clcStat.
+ *  c3d_spectra_get    copies both arrays (Fortran order) and nSpectime; sync
+ *  c3d_spectra_reset  zeroes them
+ *  c3d_write_spectra  writes the SpecX and SpecZ files (:998-1042: the 18
+ *                     columns back to back, native real(8), divided by
+ *                     nSpectime), then resets; synchronous */
+#define C3D_NSPEC1D 18
+int c3d_clcSpectra1D(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
+                     const void* pr, int jForLCS1, int jForLCS2);
+int c3d_spectra_get(c3d_ctx* ctx, double* specx, double* specz, int* nspectime);
+int c3d_spectra_reset(c3d_ctx* ctx);
+int c3d_write_spectra(c3d_ctx* ctx, const char* path_specx, const char* path_specz);
+
 /* ----------------------------------------------------------- LES layer -- */
 /* The 2nd-order code's large-eddy simulation (namelist LesOptions,
  * src/CFD_2nd/m_Parameters.f90:102-121).  c3d_set_les(ctx, LES_type,
