@@ -1328,3 +1328,157 @@ void orc4_clcSpectra1D(orc4_ctx* c, const double* ux, const double* uy, const do
   }
   free(fux); free(fuc); free(fvc); free(fvt); free(fw); free(fp); free(h1); free(h2);
 }
+
+/* ------------------------------------------------------------------------- */
+/* 2-D energy spectra, uv cospectrum and linear coherence spectra of clcStat */
+/* (m_FlowCase.f90:911-938 with clcSpecAndLCSR2DFrom1D :1896-2138 and        */
+/* clcCosSpectra2D :2143-2298; clcSpectraAndLCSR2D :1637-1891 computes the    */
+/* same from its own x transform).  The x half spectrum is split into its    */
+/* real parts (kx = 0..nxc/2) and imaginary parts (zero at kx = 0, nxc/2),    */
+/* each transformed with R2HC along z; arrays (nxc/2+1, nyc, nzc).            */
+/* ------------------------------------------------------------------------- */
+#define S2(i, j, k) ((size_t)((i)-1) + (size_t)nxhp * ((size_t)((j)-1) + (size_t)nyc * (size_t)((k)-1)))
+
+static void sp2_transform(orc4_ctx* c, const double* f, double* zr, double* zi) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc, nxh = nxc / 2, nxhp = nxh + 1;
+  double* h = (double*)malloc(sizeof(double) * c->nint);
+  sp_r2hc(c, 0, f, h);
+  double* ar = (double*)malloc(sizeof(double) * (size_t)nxhp * nyc * nzc);
+  double* ai = (double*)malloc(sizeof(double) * (size_t)nxhp * nyc * nzc);
+  for (int k = 1; k <= nzc; ++k)
+    for (int j = 1; j <= nyc; ++j) {
+      for (int i = 1; i <= nxhp; ++i) ar[S2(i, j, k)] = h[I4(c, i, j, k)];
+      ai[S2(1, j, k)] = 0.0;
+      ai[S2(nxhp, j, k)] = 0.0;
+      for (int i = 2; i <= nxh; ++i) ai[S2(i, j, k)] = h[I4(c, nxc + 2 - i, j, k)];
+    }
+  double* line = (double*)malloc(sizeof(double) * (size_t)nzc);
+  double* res = (double*)malloc(sizeof(double) * (size_t)nzc);
+  double* wk = (double*)malloc(sizeof(double) * 4 * (size_t)nzc);
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* a = pass ? ai : ar;
+    double* z = pass ? zi : zr;
+    for (int j = 1; j <= nyc; ++j)
+      for (int i = 1; i <= nxhp; ++i) {
+        for (int k = 1; k <= nzc; ++k) line[k - 1] = a[S2(i, j, k)];
+        orci_r2hc(&c->pz, line, res, wk);
+        for (int k = 1; k <= nzc; ++k) z[S2(i, j, k)] = res[k - 1];
+      }
+  }
+  free(line); free(res); free(wk); free(ar); free(ai); free(h);
+}
+
+/* arrz_xhzh of one part: products of two half-complex z spectra at (i, j, kc),
+ * optionally against another row r of the second (the LCS plane product) */
+static inline double sp2_prod(const orc4_ctx* c, const double* z1, const double* z2, int i, int j,
+                              int r, int kc) {
+  const int nxhp = c->nxc / 2 + 1, nyc = c->nyc, nzc = c->nzc, nzh = nzc / 2, nzhp = nzh + 1;
+  if (kc == 1 || kc == nzhp) return z1[S2(i, j, kc)] * z2[S2(i, r, kc)];
+  const int kct = nzc + 2 - kc;
+  return z1[S2(i, j, kc)] * z2[S2(i, r, kc)] + z1[S2(i, j, kct)] * z2[S2(i, r, kct)];
+}
+
+/* the same with the imaginary-part term first, added to the real-part one */
+static inline double sp2_both(const orc4_ctx* c, const double* r1, const double* i1,
+                              const double* r2, const double* i2, int i, int j, int r, int kc) {
+  const double e = sp2_prod(c, r1, r2, i, j, r, kc);
+  const int nxhp = c->nxc / 2 + 1, nyc = c->nyc, nzc = c->nzc, nzh = nzc / 2, nzhp = nzh + 1;
+  if (kc == 1 || kc == nzhp) return i1[S2(i, j, kc)] * i2[S2(i, r, kc)] + e;
+  const int kct = nzc + 2 - kc;
+  return i1[S2(i, j, kc)] * i2[S2(i, r, kc)] + i1[S2(i, j, kct)] * i2[S2(i, r, kct)] + e;
+}
+
+/* the LCS accumulation: real-part sum first, then + P(kc) + P(kct) of the
+ * imaginary part */
+static inline double sp2_lcs(const orc4_ctx* c, const double* zr, const double* zi, int i, int j,
+                             int r, int kc) {
+  const int nxhp = c->nxc / 2 + 1, nyc = c->nyc, nzc = c->nzc, nzh = nzc / 2, nzhp = nzh + 1;
+  double l;
+  if (kc == 1 || kc == nzhp) {
+    l = zr[S2(i, j, kc)] * zr[S2(i, r, kc)];
+    return l + zi[S2(i, j, kc)] * zi[S2(i, r, kc)];
+  }
+  const int kct = nzc + 2 - kc;
+  l = zr[S2(i, j, kc)] * zr[S2(i, r, kc)] + zr[S2(i, j, kct)] * zr[S2(i, r, kct)];
+  return l + zi[S2(i, j, kc)] * zi[S2(i, r, kc)] + zi[S2(i, j, kct)] * zi[S2(i, r, kct)];
+}
+
+void orc4_clcSpectra2D(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                       const double* pr, int jlcs, double* spec2d) {
+  const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
+  const int nxhp = nxc / 2 + 1, nzhp = nzc / 2 + 1;
+  const int ch = c->p.flow_type == ORC4_FT_CH;
+  const int nys = ch ? nyc / 2 : nyc;
+  double normEgy = 1.0 / ((double)nxc * (double)nzc);
+  normEgy = normEgy * normEgy;
+  if (ch) normEgy = normEgy * 0.5;
+  const size_t N = c->nint, M = (size_t)nxhp * nyc * nzc;
+  const double uCRF = c->p.u_crf;
+  double* fld[6];
+  for (int q = 0; q < 6; ++q) fld[q] = (double*)malloc(sizeof(double) * N);
+  /* 0 ux+uCRF, 1 ux centre + uCRF, 2 uy centre, 3 transform_uy, 4 uz, 5 p */
+  for (int kc = 1; kc <= nzc; ++kc)
+    for (int jc = 1; jc <= nyc; ++jc) {
+      int jt;
+      if (ch) jt = jc > nyc / 2 ? jc : jc + 1;
+      else jt = jc + 1;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        const size_t q = I4(c, ic, jc, kc);
+        fld[0][q] = ux[H4(c, ic, jc, kc)] + uCRF;
+        fld[1][q] = c->InterpCoe1 * ux[H4(c, ic - 1, jc, kc)] + c->InterpCoe2 * ux[H4(c, ic, jc, kc)] +
+                    c->InterpCoe3 * ux[H4(c, ic + 1, jc, kc)] +
+                    c->InterpCoe4 * ux[H4(c, ic + 2, jc, kc)] + uCRF;
+        fld[2][q] = 0.5 * (uy[H4(c, ic, jc, kc)] + uy[H4(c, ic, jc + 1, kc)]);
+        fld[3][q] = (!ch && jc == nyc) ? 0.0 : uy[H4(c, ic, jt, kc)];
+        fld[4][q] = uz[H4(c, ic, jc, kc)];
+        fld[5][q] = pr[H4(c, ic, jc, kc)];
+      }
+    }
+  double *zr1 = (double*)malloc(sizeof(double) * M), *zi1 = (double*)malloc(sizeof(double) * M);
+  double *zr2 = (double*)malloc(sizeof(double) * M), *zi2 = (double*)malloc(sizeof(double) * M);
+#define E2(i, j, k, m) spec2d[(size_t)((i)-1) + (size_t)nxhp * ((size_t)((j)-1) + (size_t)nys * ((size_t)((k)-1) + (size_t)nzhp * (size_t)((m)-1)))]
+  /* energy and LCS of u, v, w, p: columns (1,6) (2,7) (3,8) (4,9) */
+  static const int src[4] = {0, 3, 4, 5};
+  for (int f = 0; f < 4; ++f) {
+    sp2_transform(c, fld[src[f]], zr1, zi1);
+    const int m1 = f + 1, m2 = f + 6;
+    for (int jc = 1; jc <= nys; ++jc) {
+      const int jcs = nyc + 1 - jc;
+      const int r = jlcs, rs = nyc + 1 - jlcs;
+      for (int kc = 1; kc <= nzhp; ++kc)
+        for (int ic = 1; ic <= nxhp; ++ic) {
+          const double e = sp2_both(c, zr1, zi1, zr1, zi1, ic, jc, jc, kc);
+          const double l = sp2_lcs(c, zr1, zi1, ic, jc, r, kc);
+          if (ch) {
+            const double es = sp2_both(c, zr1, zi1, zr1, zi1, ic, jcs, jcs, kc);
+            const double ls = sp2_lcs(c, zr1, zi1, ic, jcs, rs, kc);
+            E2(ic, jc, kc, m1) = E2(ic, jc, kc, m1) + normEgy * (e + es);
+            E2(ic, jc, kc, m2) = E2(ic, jc, kc, m2) + normEgy * (l + ls);
+          } else {
+            E2(ic, jc, kc, m1) = E2(ic, jc, kc, m1) + normEgy * e;
+            E2(ic, jc, kc, m2) = E2(ic, jc, kc, m2) + normEgy * l;
+          }
+        }
+    }
+  }
+  /* the uv cospectrum, antisymmetric across the centreline (column 5) */
+  sp2_transform(c, fld[1], zr1, zi1);
+  sp2_transform(c, fld[2], zr2, zi2);
+  for (int jc = 1; jc <= nys; ++jc) {
+    const int jcs = nyc + 1 - jc;
+    for (int kc = 1; kc <= nzhp; ++kc)
+      for (int ic = 1; ic <= nxhp; ++ic) {
+        const double e = sp2_both(c, zr1, zi1, zr2, zi2, ic, jc, jc, kc);
+        if (ch) {
+          const double es = sp2_both(c, zr1, zi1, zr2, zi2, ic, jcs, jcs, kc);
+          E2(ic, jc, kc, 5) = E2(ic, jc, kc, 5) + normEgy * (e + -1.0 * es);
+        } else {
+          E2(ic, jc, kc, 5) = E2(ic, jc, kc, 5) + normEgy * e;
+        }
+      }
+  }
+#undef E2
+  free(zr1); free(zi1); free(zr2); free(zi2);
+  for (int q = 0; q < 6; ++q) free(fld[q]);
+}
+#undef S2

@@ -102,6 +102,14 @@ void orc4_project(orc4_ctx* c);
 void orc4_clcSpectra1D(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
                        const double* pr, const int* jlcs, double* specx, double* specz);
 
+/* the 2-D spectra of clcStat (m_FlowCase.f90:911-938): adds one sample to
+ * spec2d = EnergySpec2D(nxc/2+1, nySpec2D, nzc/2+1, 9), Fortran order, with
+ * nySpec2D = nyc/2 for a channel (the two halves folded) or nyc for a half
+ * channel.  Columns 1-4 uu, vv, ww, pp; 5 uv (antisymmetric fold); 6-9 the
+ * real linear coherence spectra of u, v, w, p against row jlcs. */
+void orc4_clcSpectra2D(orc4_ctx* c, const double* ux, const double* uy, const double* uz,
+                       const double* pr, int jlcs, double* spec2d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,6 +190,7 @@ def lib():
         L.orc4_clcStat_mag.argtypes = [ctypes.c_void_p] + [_DP] * 6
         L.orc4_clcSpectra1D.argtypes = [ctypes.c_void_p] + [_DP] * 4 + [
             ctypes.POINTER(ctypes.c_int), _DP, _DP]
+        L.orc4_clcSpectra2D.argtypes = [ctypes.c_void_p] + [_DP] * 4 + [ctypes.c_int, _DP]
         _lib = L
     return _lib
 

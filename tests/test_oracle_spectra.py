@@ -186,3 +186,97 @@ def test_vv_takes_the_node_away_from_the_wall(oracle_mod, flow):
         assert sz[0, j - 1, 1] == pytest.approx(want, rel=1e-13, abs=1e-30)
         assert np.all(np.abs(sx[1:, j - 1, 1]) < 1e-28)
     o.close()
+
+
+# ------------------------------------------------------------ 2-D spectra --
+def _spectra2d(O, o, ux, uy, uz, pr, jl=2):
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    nys = ny // 2 if o.params.flow_type == 1 else ny
+    s = np.zeros((nx // 2 + 1, nys, nz // 2 + 1, 9), order="F")
+    o.call("clcSpectra2D", ux, uy, uz, pr, jl, s)
+    return s
+
+
+def _ratio(nx, nz):
+    """the weight of each half-plane mode in the full (kx, kz) plane: the
+    modes kx = 0, nx/2 and kz = 0, nz/2 have no conjugate partner"""
+    r = np.full((nx // 2 + 1, nz // 2 + 1), 4.0)
+    r[0, :] = r[-1, :] = 2.0
+    r[:, 0] = r[:, -1] = 2.0
+    r[0, 0] = r[0, -1] = r[-1, 0] = r[-1, -1] = 1.0
+    return r
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+def test_parseval_2d(oracle_mod, flow):
+    O = oracle_mod
+    o = O.Oracle4(_case(flow_type=flow, u_crf=0.2))
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    rng = np.random.default_rng(9)
+    f = [rng.normal(size=(nx, ny + 2, nz)) for _ in range(4)]
+    ux, uy, uz, pr = (_halo(o, lambda i, j, k, a=a: a[i.astype(int) - 1, j.astype(int),
+                                                      k.astype(int) - 1]) for a in f)
+    jl = 2
+    s = _spectra2d(O, o, ux, uy, uz, pr, jl)
+    R = _ratio(nx, nz)[:, None, :]
+    nys = s.shape[1]
+
+    def plane(a, add=0.0):
+        return a[2:nx + 2, 1:ny + 1, 2:nz + 2] + add
+    for col, a, add in ((1, ux, 0.2), (3, uz, 0.0), (4, pr, 0.0)):
+        p = plane(a, add)
+        msq = (p ** 2).mean(axis=(0, 2))
+        total = (R * s[:, :, :, col - 1]).sum(axis=(0, 2))
+        want = 0.5 * (msq[:nys] + msq[::-1][:nys]) if flow == 1 else msq
+        assert np.allclose(total, want, rtol=1e-12, atol=0), col
+    # cross-Parseval of the coherence of w against row jl (and its mirror)
+    p = plane(uz)
+    cross = lambda j, r: (p[:, j - 1, :] * p[:, r - 1, :]).mean()
+    total = (R * s[:, :, :, 7]).sum(axis=(0, 2))
+    for j in range(1, nys + 1):
+        want = (0.5 * (cross(j, jl) + cross(ny + 1 - j, ny + 1 - jl)) if flow == 1
+                else cross(j, jl))
+        assert total[j - 1] == pytest.approx(want, rel=1e-12, abs=1e-14)
+    o.close()
+
+
+def test_single_mode_2d(oracle_mod):
+    O = oracle_mod
+    o = O.Oracle4(_case(flow_type=2))
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    z = _zero(o)
+    for mx, mz, sgn in ((3, 2, 1), (3, 2, -1), (0, 4, 1), (nx // 2, 1, 1)):
+        A, phi = 0.9, 0.4
+        pr = _halo(o, lambda i, j, k: A * np.cos(2 * np.pi * (mx * (i - 1) / nx
+                                                             + sgn * mz * (k - 1) / nz) + phi)
+                   + 0 * j)
+        s = _spectra2d(O, o, z, z, z, pr)[:, :, :, 3]
+        R = _ratio(nx, nz)
+        for j in range(ny):
+            e = s[:, j, :]
+            mask = np.zeros_like(e, dtype=bool)
+            mask[mx, mz] = True
+            assert np.abs(e[~mask]).max() < 1e-14, (mx, mz, sgn)
+            # every case varies along one direction at least: mean square A^2/2
+            assert R[mx, mz] * e[mx, mz] == pytest.approx(A * A / 2, rel=1e-12)
+    o.close()
+
+
+def test_uv_2d_folds_antisymmetrically(oracle_mod):
+    O = oracle_mod
+    o = O.Oracle4(_case(flow_type=1, u_crf=0.3))
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    a = lambda j: 0.5 + 0.2 * j              # ux: a function of the row only
+    b = lambda j: np.sin(0.7 * j) + 0.1      # uy on the nodes
+    ux = _halo(o, lambda i, j, k: a(j) + 0 * i * k)
+    uy = _halo(o, lambda i, j, k: b(j) + 0 * i * k)
+    z = _zero(o)
+    s = _spectra2d(O, o, ux, uy, z, z)[:, :, :, 4]
+    # the centre-interpolated ux is a(j) + uCRF (the 4-point weights sum to
+    # one), the centre uy the mean of its nodes; only the mean mode is set
+    uv = lambda j: (a(j) + 0.3) * 0.5 * (b(j) + b(j + 1))
+    for j in range(1, ny // 2 + 1):
+        want = 0.5 * (uv(j) - uv(ny + 1 - j))
+        assert s[0, j - 1, 0] == pytest.approx(want, rel=1e-12, abs=1e-15)
+        assert np.abs(s[:, j - 1, :]).sum() - abs(s[0, j - 1, 0]) < 1e-14
+    o.close()
