@@ -204,6 +204,23 @@ class Channel:
         L.check(self.lib.c3d_write_spectra(self._h, os.fsencode(path_x), os.fsencode(path_z)),
                 "c3d_write_spectra")
 
+    def clcSpectra2D(self, jlcs=1, ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR):
+        """one sample of the 4th-order 2-D spectra (clcStat with clcSpectra2D)"""
+        L.check(self.lib.c3d_clcSpectra2D(self._h, *[self.ptr(f) for f in (ux, uy, uz, pressure)],
+                                          int(jlcs)), "c3d_clcSpectra2D")
+
+    def spectra2d_get(self):
+        """(EnergySpec2D (nxc/2+1, nySpec2D, nzc/2+1, 9), nSpectime)"""
+        nys = self.nyc // 2 if self.params.bc[3] == L.BC_NOSLIP else self.nyc
+        s = np.zeros((self.nxc // 2 + 1, nys, self.nzc // 2 + 1, 9), order="F")
+        n = ctypes.c_int()
+        L.check(self.lib.c3d_spectra2d_get(self._h, L.dptr(s), ctypes.byref(n)),
+                "c3d_spectra2d_get")
+        return s, n.value
+
+    def write_spectra2d(self, path):
+        L.check(self.lib.c3d_write_spectra2d(self._h, os.fsencode(path)), "c3d_write_spectra2d")
+
     def set_les(self, les_type, filter_type=1):
         """LES_type 0 (off), 2 (WALE) or 3 (MTS); FilterType 0 trapezoidal, 1 Simpson
         (InitFilterCoe and the LES branch of InitTimeScheme)"""

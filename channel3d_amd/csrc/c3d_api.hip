@@ -647,7 +647,7 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
   for (void* p : c->user_fields) hipFree(p);
   void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
                   c->d_dyp, c->d_spike, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
-                  c->d_lesmet, c->d_spec1d, c->spec2};
+                  c->d_lesmet, c->d_spec1d, c->spec2, c->d_spec2d};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (c->staging) hipHostFree(c->staging);

@@ -251,6 +251,9 @@ struct c3d_ctx {
   double* d_spec1d = nullptr;
   void* spec2 = nullptr;
   int nspectime = 0;
+  // 2-D spectra: EnergySpec2D (nx/2+1, nySpec2D, nz/2+1, 9) and its count
+  double* d_spec2d = nullptr;
+  int nspec2d = 0;
 
   // channel statistics: SumStat(35, ny+1), PrGradsum (+7 spare), then the
   // per-workgroup partial rows; allocated on first use

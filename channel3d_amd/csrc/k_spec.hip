@@ -155,6 +155,90 @@ __global__ void k_spec_lts(Geom g, const double2* __restrict__ X, int jr1, int j
   c3[at] += e3 * norm;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D spectra (m_FlowCase.f90:911-938, clcSpecAndLCSR2DFrom1D :1896-2138,
+// clcCosSpectra2D :2143-2298).  After the x and z passes the spectral array
+// holds Z = A^ + i B^, the z transforms of the real parts A and imaginary
+// parts B of the x half spectrum; A^(kz) = (Z(kz) + conj Z(-kz))/2 and
+// B^(kz) = (Z(kz) - conj Z(-kz))/(2i).  The reference zeroes B at kx = 0 and
+// nx/2, so B^ is left out there.
+// ---------------------------------------------------------------------------
+struct Split {
+  Hc a, b;   // A^(kz), B^(kz): (real, imaginary)
+};
+
+__device__ __forceinline__ Split split_z(const Geom& g, const double2* Z, int kx, int j0, int kz) {
+  const int km = kz == 0 ? 0 : g.nz - kz;
+  const double2 p = Z[kx + (long long)g.P * (j0 + (long long)g.ny * kz)];
+  const double2 q = Z[kx + (long long)g.P * (j0 + (long long)g.ny * km)];
+  Split s;
+  s.a = Hc{0.5 * (p.x + q.x), 0.5 * (p.y - q.y)};
+  s.b = Hc{0.5 * (p.y + q.y), 0.5 * (q.x - p.x)};
+  if (kx == 0 || kx == g.nx / 2) s.b = Hc{0.0, 0.0};
+  return s;
+}
+
+// one part's product at a half-complex z mode: real parts only at kz = 0 and
+// nz/2, real*real + imaginary*imaginary elsewhere
+__device__ __forceinline__ double part_prod(const Hc& u, const Hc& v, bool edge) {
+  return edge ? u.r * v.r : u.r * v.r + u.i * v.i;
+}
+
+// energy / cospectrum: imaginary-part products first, added to the real-part ones
+__device__ __forceinline__ double both_prod(const Split& u, const Split& v, bool edge) {
+  const double e = part_prod(u.a, v.a, edge);
+  return edge ? u.b.r * v.b.r + e : u.b.r * v.b.r + u.b.i * v.b.i + e;
+}
+
+// the coherence: real-part sum, then + the imaginary-part terms one by one
+__device__ __forceinline__ double lcs_prod(const Split& u, const Split& v, bool edge) {
+  const double l = part_prod(u.a, v.a, edge);
+  return edge ? l + u.b.r * v.b.r : l + u.b.r * v.b.r + u.b.i * v.b.i;
+}
+
+// energy (col m1) and coherence against row jr (col m2) of one field, or
+// (m2 < 0) the cospectrum of two fields with sign rcoe on the mirror row
+__global__ void k_spec2d(Geom g, const double2* __restrict__ Z1, const double2* __restrict__ Z2,
+                         int ch, int nys, int jr, double rcoe, double norm,
+                         double* __restrict__ e1, double* __restrict__ e2) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int kz = blockIdx.y, j0 = blockIdx.z;
+  const int nxhp = g.nx / 2 + 1, nzhp = g.nz / 2 + 1;
+  if (kx >= nxhp) return;
+  const bool edge = kz == 0 || kz == g.nz / 2;
+  const long long at = kx + (long long)nxhp * (j0 + (long long)nys * kz);
+  const Split u = split_z(g, Z1, kx, j0, kz);
+  if (e2) {
+    const Split r = split_z(g, Z1, kx, jr - 1, kz);
+    const double e = both_prod(u, u, edge);
+    const double l = lcs_prod(u, r, edge);
+    if (ch) {
+      const int js = g.ny - 1 - j0, rs = g.ny - jr;
+      const Split us = split_z(g, Z1, kx, js, kz);
+      const Split vs = split_z(g, Z1, kx, rs, kz);
+      const double es = both_prod(us, us, edge);
+      const double ls = lcs_prod(us, vs, edge);
+      e1[at] += norm * (e + es);
+      e2[at] += norm * (l + ls);
+    } else {
+      e1[at] += norm * e;
+      e2[at] += norm * l;
+    }
+  } else {
+    const Split v = split_z(g, Z2, kx, j0, kz);
+    const double e = both_prod(u, v, edge);
+    if (ch) {
+      const int js = g.ny - 1 - j0;
+      const Split us = split_z(g, Z1, kx, js, kz), vs = split_z(g, Z2, kx, js, kz);
+      const double es = both_prod(us, vs, edge);
+      e1[at] += norm * (e + rcoe * es);
+    } else {
+      e1[at] += norm * e;
+    }
+  }
+  (void)nzhp;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -250,6 +334,53 @@ static int spec_lts(c3d_ctx* c, int axis, const void* X, int n1, const int* jr) 
     hipLaunchKernelGGL(k_spec_lts<2>, grid, dim3(64), 0, c->stream, g, (const double2*)X, jr[0],
                        jr[1], half, spec_norm(c, 2), spec_col(c, 2, n1), spec_col(c, 2, n1 + 1),
                        spec_col(c, 2, n1 + 2));
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+// ---- 2-D spectra ---------------------------------------------------------
+static int spec2d_nys(const c3d_ctx* c) {
+  return c->p.bc[3] == C3D_BC_NOSLIP ? c->g.ny / 2 : c->g.ny;
+}
+
+static size_t spec2d_len(const c3d_ctx* c) {
+  return (size_t)(c->g.nx / 2 + 1) * spec2d_nys(c) * (size_t)(c->g.nz / 2 + 1);
+}
+
+static int spec2d_ensure(c3d_ctx* c) {
+  int rc;
+  if ((rc = spec_ensure(c))) return rc;
+  if (c->d_spec2d) return C3D_OK;
+  const size_t n = 9 * spec2d_len(c);
+  hipError_t e = hipMalloc((void**)&c->d_spec2d, n * sizeof(double));
+  if (e != hipSuccess) return hip_fail(e, "2-D spectra allocation");
+  c->dev_bytes += n * sizeof(double);
+  C3D_HIP(hipMemset(c->d_spec2d, 0, n * sizeof(double)));
+  c->nspec2d = 0;
+  return C3D_OK;
+}
+
+// the z pass over the x half spectrum of one field: Z = A^ + i B^
+static int spec2d_transform(c3d_ctx* c, int mode, const void* f, void* X) {
+  int rc;
+  if ((rc = spec_transform(c, 0, mode, f, X))) return rc;
+  return launch_fft_z(c, X, 0);
+}
+
+static int spec2d_reduce(c3d_ctx* c, const void* Z1, const void* Z2, int m1, int m2, int jr,
+                         double rcoe) {
+  const Geom& g = c->g;
+  const int ch = c->p.bc[3] == C3D_BC_NOSLIP;
+  const int nys = spec2d_nys(c);
+  double norm = 1.0 / ((double)g.nx * (double)g.nz);
+  norm = norm * norm;
+  if (ch) norm = norm * 0.5;
+  const size_t L = spec2d_len(c);
+  double* e1 = c->d_spec2d + L * (m1 - 1);
+  double* e2 = m2 > 0 ? c->d_spec2d + L * (m2 - 1) : nullptr;
+  dim3 grid((g.nx / 2 + 1 + 63) / 64, g.nz / 2 + 1, nys);
+  hipLaunchKernelGGL(k_spec2d, grid, dim3(64), 0, c->stream, g, (const double2*)Z1,
+                     (const double2*)Z2, ch, nys, jr, rcoe, norm, e1, e2);
   C3D_HIP(hipGetLastError());
   return C3D_OK;
 }
@@ -376,4 +507,98 @@ extern "C" int c3d_write_spectra(c3d_ctx* c, const char* path_x, const char* pat
     }
   }
   return c3d_spectra_reset(c);
+}
+
+// ===========================================================================
+// 2-D spectra: clcSpecAndLCSR2DFrom1D for u, v, w, p and clcCosSpectra2D for
+// uv (m_FlowCase.f90:911-938); EnergySpec2D(nxc/2+1, nySpec2D, nzc/2+1, 9)
+// ===========================================================================
+extern "C" int c3d_clcSpectra2D(c3d_ctx* c, const void* ux, const void* uy, const void* uz,
+                                const void* pr, int jlcs) {
+  CHECK_SPEC_CTX(c, "c3d_clcSpectra2D");
+  if (!ux || !uy || !uz || !pr || jlcs < 1 || jlcs > c->g.ny) {
+    set_error("c3d_clcSpectra2D: bad arguments (jForLCS must lie in 1..nyc)");
+    return C3D_ERR_ARG;
+  }
+  int rc;
+  if ((rc = spec2d_ensure(c))) return rc;
+  hipEvent_t ev;
+  prof_begin(c, ST_STAT, &ev);
+  void* S1 = c->spec;
+  void* S2 = c->spec2;
+  static const int modes[4] = {SP_UXF, SP_UYT, SP_COPY, SP_COPY};
+  const void* src[4] = {ux, uy, uz, pr};
+  for (int f = 0; f < 4; ++f) {
+    if ((rc = spec2d_transform(c, modes[f], src[f], S1))) return rc;
+    if ((rc = spec2d_reduce(c, S1, nullptr, f + 1, f + 6, jlcs, 1.0))) return rc;
+  }
+  // uv: ux at the cell centre against uy at the cell centre, the mirror row
+  // entering with -1 (IsSymmetry false)
+  if ((rc = spec2d_transform(c, SP_UXC, ux, S1))) return rc;
+  if ((rc = spec2d_transform(c, SP_UYC, uy, S2))) return rc;
+  if ((rc = spec2d_reduce(c, S1, S2, 5, -1, jlcs, -1.0))) return rc;
+  c->nspec2d += 1;
+  prof_end(c, ST_STAT, ev, 0.0);
+  return C3D_OK;
+}
+
+extern "C" int c3d_spectra2d_get(c3d_ctx* c, double* spec2d, int* nspectime) {
+  CHECK_SPEC_CTX(c, "c3d_spectra2d_get");
+  int rc;
+  if ((rc = spec2d_ensure(c))) return rc;
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  if (spec2d)
+    C3D_HIP(hipMemcpy(spec2d, c->d_spec2d, 9 * spec2d_len(c) * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  if (nspectime) *nspectime = c->nspec2d;
+  return C3D_OK;
+}
+
+// Spec2D (m_FlowCase.f90:1044-1137): the sums times 1/nSpectime and the
+// half-plane weight (4; 2 on kx = 0, nx/2 or kz = 0, nz/2; 1 at the corners),
+// the 9 columns as (nxc/2+1, nySpec2D, nzc/2+1) real(8) blocks; then reset
+extern "C" int c3d_write_spectra2d(c3d_ctx* c, const char* path) {
+  CHECK_SPEC_CTX(c, "c3d_write_spectra2d");
+  if (!path) {
+    set_error("c3d_write_spectra2d: null path");
+    return C3D_ERR_ARG;
+  }
+  int rc;
+  if ((rc = spec2d_ensure(c))) return rc;
+  if (c->nspec2d < 1) {
+    set_error("c3d_write_spectra2d: no 2-D spectra samples since the last reset");
+    return C3D_ERR_ARG;
+  }
+  const size_t L = spec2d_len(c);
+  std::vector<double> h(9 * L);
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  C3D_HIP(hipMemcpy(h.data(), c->d_spec2d, 9 * L * sizeof(double), hipMemcpyDeviceToHost));
+  const int nxhp = c->g.nx / 2 + 1, nzhp = c->g.nz / 2 + 1, nys = spec2d_nys(c);
+  const double inv = 1.0 / (double)c->nspec2d;
+  for (int kc = 0; kc < nzhp; ++kc)
+    for (int ic = 0; ic < nxhp; ++ic) {
+      double r = 4.0;
+      const bool ex = ic == 0 || ic == nxhp - 1, ez = kc == 0 || kc == nzhp - 1;
+      if (ex || ez) r = 2.0;
+      if (ex && ez) r = 1.0;
+      const double w = inv * r;
+      for (int m = 0; m < 9; ++m)
+        for (int j = 0; j < nys; ++j) {
+          double& v = h[ic + (size_t)nxhp * (j + (size_t)nys * (kc + (size_t)nzhp * m))];
+          v = w * v;
+        }
+    }
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_error(std::string("c3d_write_spectra2d: cannot open file ") + path);
+    return C3D_ERR_IO;
+  }
+  const size_t wr = std::fwrite(h.data(), sizeof(double), h.size(), fp);
+  if (std::fclose(fp) != 0 || wr != h.size()) {
+    set_error(std::string("c3d_write_spectra2d: write failed: ") + path);
+    return C3D_ERR_IO;
+  }
+  C3D_HIP(hipMemset(c->d_spec2d, 0, 9 * L * sizeof(double)));
+  c->nspec2d = 0;
+  return C3D_OK;
 }

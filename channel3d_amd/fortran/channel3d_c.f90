@@ -242,6 +242,30 @@ module channel3d_c
       integer(c_int), value :: sweeps
     end function
 
+    ! ---- 4th-order spectra (SpectraOptions, CFD_4th m_FlowCase.f90:644-938)
+    integer(c_int) function c3d_clcSpectra1D(ctx, ux, uy, uz, pr, j1, j2) &
+        bind(C, name="c3d_clcSpectra1D")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, pr
+      integer(c_int), value :: j1, j2
+    end function
+    integer(c_int) function c3d_write_spectra(ctx, path_x, path_z) bind(C, name="c3d_write_spectra")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path_x(*), path_z(*)
+    end function
+    integer(c_int) function c3d_clcSpectra2D(ctx, ux, uy, uz, pr, j1) &
+        bind(C, name="c3d_clcSpectra2D")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ux, uy, uz, pr
+      integer(c_int), value :: j1
+    end function
+    integer(c_int) function c3d_write_spectra2d(ctx, path) bind(C, name="c3d_write_spectra2d")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+
     ! ---- LES layer (LesOptions, m_LESModels.f90, m_TScheme.f90:179-206)
     integer(c_int) function c3d_set_les(ctx, les_type, filter_type) bind(C, name="c3d_set_les")
       import :: c_int, c_ptr

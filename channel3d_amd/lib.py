@@ -47,6 +47,7 @@ EXPORTS = (
     "c3d_set_ppe_refine", "c3d_clcStat", "c3d_stat_set_grad", "c3d_stat_get_grad",
     "c3d_write_vgrad", "c3d_set_les", "c3d_Clc_SGS_Vis",
     "c3d_clcSpectra1D", "c3d_spectra_get", "c3d_spectra_reset", "c3d_write_spectra",
+    "c3d_clcSpectra2D", "c3d_spectra2d_get", "c3d_write_spectra2d",
 )
 
 
@@ -154,6 +155,9 @@ def load(path: str | None = None):
         "c3d_spectra_get": ([_VP, _DP, _DP, _IP], I),
         "c3d_spectra_reset": ([_VP], I),
         "c3d_write_spectra": ([_VP, ctypes.c_char_p, ctypes.c_char_p], I),
+        "c3d_clcSpectra2D": ([_VP] * 5 + [I], I),
+        "c3d_spectra2d_get": ([_VP, _DP, _IP], I),
+        "c3d_write_spectra2d": ([_VP, ctypes.c_char_p], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

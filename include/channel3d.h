@@ -248,6 +248,21 @@ int c3d_clcSpectra1D(c3d_ctx* ctx, const void* ux, const void* uy, const void* u
 int c3d_spectra_get(c3d_ctx* ctx, double* specx, double* specz, int* nspectime);
 int c3d_spectra_reset(c3d_ctx* ctx);
 int c3d_write_spectra(c3d_ctx* ctx, const char* path_specx, const char* path_specz);
+/* The clcSpectra2D part (:911-938, clcSpecAndLCSR2DFrom1D :1896-2138,
+ * clcCosSpectra2D :2143-2298): one sample of EnergySpec2D(nxc/2+1, nySpec2D,
+ * nzc/2+1, 9), nySpec2D = nyc/2 for a channel (each row folded with its
+ * mirror row) or nyc for a half channel.  Columns 1-4 uu, vv, ww, pp; 5 uv
+ * (ux and uy at the cell centre, the mirror row entering with -1); 6-9 the
+ * real linear coherence spectra of u, v, w, p against row jForLCS1.
+ *  c3d_spectra2d_get    copies the array (Fortran order) and its sample count
+ *  c3d_write_spectra2d  writes the Spec2D file (:1044-1137: the sums times
+ *                       1/nSpectime and the half-plane weight 4, 2 on the kx
+ *                       or kz edges, 1 at the corners; 9 blocks of real(8)),
+ *                       then resets */
+int c3d_clcSpectra2D(c3d_ctx* ctx, const void* ux, const void* uy, const void* uz,
+                     const void* pr, int jForLCS1);
+int c3d_spectra2d_get(c3d_ctx* ctx, double* spec2d, int* nspectime);
+int c3d_write_spectra2d(c3d_ctx* ctx, const char* path_spec2d);
 
 /* ----------------------------------------------------------- LES layer -- */
 /* The 2nd-order code's large-eddy simulation (namelist LesOptions,

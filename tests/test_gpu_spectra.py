@@ -112,3 +112,43 @@ def test_spectra_rejects(O):
     with pytest.raises(L.C3DError, match="no spectra samples"):
         g4.write_spectra(os.devnull, os.devnull)
     g4.close()
+
+
+# ------------------------------------------------------------ 2-D spectra --
+def _oracle_spectra2d(O, o, jl):
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    nys = ny // 2 if o.params.flow_type == 1 else ny
+    s = np.zeros((nx // 2 + 1, nys, nz // 2 + 1, 9), order="F")
+    o.call("clcSpectra2D", o.field(O.UX), o.field(O.UY), o.field(O.UZ), o.field(O.PR), jl, s)
+    return s
+
+
+@pytest.mark.parametrize("name,ucrf", [("small4", 0.3), ("half4", 0.0)])
+def test_spectra2d_match_oracle(O, name, ucrf, tmp_path):
+    case = dict({"small4": cases.SMALL4, "half4": cases.HALF4}[name], u_crf=ucrf)
+    o, g = make_pair(O, case, seed=8)
+    o.iterate()
+    _same_state(O, o, g)
+    jl = 3
+    ref = _oracle_spectra2d(O, o, jl)
+    g.clcSpectra2D(jl)
+    got, n = g.spectra2d_get()
+    assert n == 1 and got.shape == ref.shape
+    for c in range(9):
+        s = np.abs(ref[..., c]).max()
+        err = np.abs(got[..., c] - ref[..., c]).max() / s
+        assert err < 1e-12, (c, err)
+    # the Spec2D file: weights 4 / 2 on the kx or kz edges / 1 at the corners
+    p = tmp_path / "Spec2D0000000001"
+    g.write_spectra2d(str(p))
+    f = np.fromfile(p, dtype="<f8").reshape(got.shape, order="F")
+    nxh, nzh = got.shape[0] - 1, got.shape[2] - 1
+    w = np.full((nxh + 1, nzh + 1), 4.0)
+    w[[0, nxh], :] = 2.0
+    w[:, [0, nzh]] = 2.0
+    w[0, 0] = w[0, nzh] = w[nxh, 0] = w[nxh, nzh] = 1.0
+    assert np.allclose(f, got * w[:, None, :, None], rtol=1e-15, atol=0)
+    got2, n2 = g.spectra2d_get()
+    assert n2 == 0 and not got2.any()
+    g.close()
+    o.close()
