@@ -212,6 +212,38 @@ def run_precision_sweep(g, L, nsolves, fp64_per_s, nsteps):
                 note="FP64 fields throughout; only the pressure solve changes form")
 
 
+def run_fp32_fields(g, case, L, nsteps):
+    """the same step with FP32 fields and kernels (an FP32 context): ms/step
+    and the relative max-norm difference from the FP64 step after one step
+    from the same state"""
+    import numpy as np
+    from channel3d_amd import Channel
+
+    state = {f: g.download(f) for f in (L.UX, L.UY, L.UZ, L.PR)}
+    h = Channel(case, precision=L.FP32)
+    for f, a in state.items():
+        h.upload(f, a)
+    h.step()
+    g.step()
+    errs = {}
+    for f, nm in ((L.UX, "ux"), (L.UY, "uy"), (L.UZ, "uz"), (L.PR, "p")):
+        a, b = h.download(f)[1:-2, 1:-2, 1:-2], g.download(f)[1:-2, 1:-2, 1:-2]
+        errs[nm] = float(np.abs(a - b).max() / np.abs(b).max())
+    del state
+    h.sync()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        h.step()
+    h.sync()
+    ms = 1e3 * (time.perf_counter() - t0) / nsteps
+    d = h.diagnostics()
+    h.close()
+    log(f"step with FP32 fields: {ms:.2f} ms/step, one-step rel err vs FP64 {errs}")
+    return dict(ms_per_step=round(ms, 3), one_step_rel_err_vs_fp64=errs,
+                divmax_after_step=d["divmax2"],
+                note="FP32 fields and kernels throughout (an FP32 context)")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +348,7 @@ def main(argv=None):
     precision = None
     if not a.no_precision:
         precision = run_precision_sweep(g, L, a.ppe_solves, ppe_per_s, max(a.steps // 4, 2))
+        precision["fp32_fields"] = run_fp32_fields(g, case, L, max(a.steps // 4, 2))
     copy_gbs, _ = copy_probe(10)
     log(f"copy probe: {copy_gbs:.1f} GB/s")
 

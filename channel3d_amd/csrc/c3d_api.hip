@@ -1334,8 +1334,14 @@ static int substep_les(c3d_ctx* c, int ns) {
   if ((rc = launch_les_y(c, 1, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
   if ((rc = launch_les_y(c, 2, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
-  if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], nullptr, 0.0))) return rc;
-  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  if (c->ppe_sweeps < 0) {
+    if ((rc = launch_prsrc_fft_x(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr))) return rc;
+    if ((rc = run_ppe_from_spectrum(c, s[C3D_HALOARR]))) return rc;
+  } else {
+    if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], nullptr, 0.0)))
+      return rc;
+    if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  }
   if ((rc = launch_bc_pr(c, s[C3D_HALOARR]))) return rc;
   if ((rc = launch_proj(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr, 1, 0, 0)))
     return rc;
@@ -1363,8 +1369,15 @@ extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   if ((rc = launch_adi(c, 2, ns, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
   // :110-114
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
-  if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], s[C3D_PR], 0.0))) return rc;
-  if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  if (c->ppe_sweeps < 0) {
+    // clcPrSrc fused into the first pass of clcPPE (RealArr1 is not written)
+    if ((rc = launch_prsrc_fft_x(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR]))) return rc;
+    if ((rc = run_ppe_from_spectrum(c, s[C3D_HALOARR]))) return rc;
+  } else {
+    if ((rc = launch_prsrc(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], s[C3D_PR], 0.0)))
+      return rc;
+    if ((rc = run_ppe(c, s[C3D_ARR1], s[C3D_HALOARR]))) return rc;
+  }
   // :115-121, the Neumann ghost of phi applied inside the projection
   if ((rc = launch_proj(c, s[C3D_HALOARR], s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], 1, 1, 1)))
     return rc;
@@ -1537,7 +1550,7 @@ static const char* kStageNames[ST_COUNT] = {
     "bc_ghosts",
     "prsrc", "fft_x_fwd", "fft_z_fwd", "tridiag_y", "fft_z_inv", "fft_x_inv", "projection",
     "vel_update", "pr_update", "diagnostics", "misc", "statistics", "ppe_residual",
-    "sgs_viscosity"};
+    "sgs_viscosity", "prsrc_fft_x"};
 
 extern "C" int c3d_profile_enable(c3d_ctx* c, int on) {
   CHECK_CTX(c);

@@ -152,7 +152,7 @@ enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1, YS_PARTITIONED = 2 };
 enum Stage {
   ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_XZ, ST_ADI_Y, ST_BC, ST_PRSRC,
   ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
-  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_STAT, ST_RESID, ST_SGS, ST_COUNT
+  ST_VELUPD, ST_PRUPD, ST_DIAG, ST_MISC, ST_STAT, ST_RESID, ST_SGS, ST_PRSRC_FFT, ST_COUNT
 };
 
 }  // namespace c3d
@@ -358,6 +358,12 @@ int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale);
 int launch_fft_z(c3d_ctx* c, void* spec, int inverse);
 int launch_tridiag(c3d_ctx* c, void* spec);
 int run_ppe(c3d_ctx* c, void* prsrc, void* phi);
+// the pressure source (with the -nu/2 div pressure correction when pr is
+// given) fused into the x forward pass of the FP64-solve pressure equation;
+// the source lands in the spectral array only, max |div| in divmax1.  Then
+// run_ppe_from_spectrum finishes the solve into phi.
+int launch_prsrc_fft_x(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* pr);
+int run_ppe_from_spectrum(c3d_ctx* c, void* phi);
 // FP32 transforms of the mixed-precision solve
 enum { FFT32_X_FWD_F64, FFT32_X_FWD_F32, FFT32_Z_FWD, FFT32_Z_INV, FFT32_X_INV_SET,
        FFT32_X_INV_ADD };

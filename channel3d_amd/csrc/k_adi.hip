@@ -116,13 +116,15 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       v[q] = T(0);
-      if (rowok && tx + 64 * q < w) v[q] = ADD ? f[row + ix[q]] + add : f[row + ix[q]];
+      if (rowok && tx + 64 * q < w) v[q] = f[row + ix[q]];
     }
   };
+  // the constant-flow-rate shift is added when the row is staged, not when it
+  // is loaded, so the load of the next plane stays in flight
   auto xconv = [&](const T (&v)[2]) -> T {
 #pragma unroll
     for (int q = 0; q < 2; ++q)
-      if (tx + 64 * q < w) sm[ty][tx + 64 * q] = v[q];
+      if (tx + 64 * q < w) sm[ty][tx + 64 * q] = (ADD && rowok) ? v[q] + add : v[q];
     __syncthreads();
     T acc = 0;
     for (int t = 0; t <= 2 * mx; ++t) acc += (T)xt[t] * sm[ty][tx + t];

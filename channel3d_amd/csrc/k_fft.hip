@@ -247,6 +247,88 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// The pressure source fused into the x forward pass: clcPrSrc_FIMP
+// (m_TScheme.f90:246-276; without PCORR clcPrSrcOther, :213-241) evaluated
+// for the two x points of every complex element as the row is staged, so the
+// source never goes through HBM.  Same expressions and order as the stand-
+// alone source kernel, without contraction; the per-block max |div| goes to
+// partials[blockIdx.x].
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DivArgs {
+  const T* ux;
+  const T* uy;
+  const T* uz;
+  T* pr;
+  const T* rdyp;
+  T rdx, rdz, sudtal, xnuhm;
+  double* partials;
+};
+
+template <typename T, int N, bool PCORR>
+__global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
+                                                       typename CT<T>::t* __restrict__ spec,
+                                                       const typename CT<T>::t* __restrict__ tw,
+                                                       const typename CT<T>::t* __restrict__ twr) {
+  using T2 = typename CT<T>::t;
+  constexpr int LINES = xlines(N);
+  constexpr int LDN = N + 1;
+  __shared__ T2 sm[LINES * LDN];
+  __shared__ double wmax[FNT / 64];
+  const int nrows = g.ny * g.nz;
+  const int row0 = blockIdx.x * LINES;
+  const int tid = threadIdx.x;
+  const int nx = g.nx;
+  double dm = 0.0;
+  for (int e = tid; e < LINES * N; e += FNT) {
+    const int l = e / N, m = e - l * N, row = row0 + l;
+    T2 v = mk<T2>(0, 0);
+    if (row < nrows) {
+#pragma clang fp contract(off)
+      const int j0 = row % g.ny, k = row / g.ny;
+      const int jc = j0 + 1, kp = k == g.nz - 1 ? 0 : k + 1;
+      const long long rowc = (long long)nx * jc + g.plane * k;
+      const long long rowkp = (long long)nx * jc + g.plane * kp;
+      const T sucaj = d.rdyp[jc];
+      T val[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * m + h, ip = i == nx - 1 ? 0 : i + 1;
+        const long long c = rowc + i;
+        const T rdiv = (d.ux[rowc + ip] - d.ux[c]) * d.rdx + (d.uy[c + nx] - d.uy[c]) * sucaj +
+                       (d.uz[rowkp + i] - d.uz[c]) * d.rdz;
+        dm = fmax(dm, (double)fabs(rdiv));
+        val[h] = d.sudtal * rdiv;
+        if (PCORR) d.pr[c] = d.pr[c] + d.xnuhm * rdiv;
+      }
+      v = mk<T2>(val[0], val[1]);
+    }
+    sm[l * LDN + m] = v;
+  }
+  // block max of |div|
+  for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
+  if ((tid & 63) == 0) wmax[tid >> 6] = dm;
+  __syncthreads();
+  if (tid == 0) {
+    double r = 0.0;
+    for (int q = 0; q < FNT / 64; ++q) r = fmax(r, wmax[q]);
+    d.partials[blockIdx.x] = r;
+  }
+  stockham<T2, N, LINES, LDN, FNT, -1, 1>(sm, tw);
+  for (int e = tid; e < LINES * (N + 1); e += FNT) {
+    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
+    if (row >= nrows) continue;
+    const T2 zk = sm[l * LDN + (k == N ? 0 : k)];
+    const T2 zn = sm[l * LDN + (k == 0 ? 0 : N - k)];
+    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
+    const T dr = zk.x - zn.x, di = zk.y + zn.y;
+    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
+    const T2 wo = cmul(twr[k], o);
+    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // x inverse: spectral rows -> field rows (jj = 1..ny), times scale
 // ---------------------------------------------------------------------------
 // TOUT: precision of the field written; ACC: add to it (the correction of a
@@ -360,6 +442,44 @@ static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
                        dim3(FNT), 0, c->stream, g, (const T*)in, (T2*)spec, (const T2*)c->tw_x, \
                        (const T2*)c->tw_xr);                                                     \
     break;
+    C3D_FFT_SIZES(X)
+#undef X
+    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+template <typename T>
+static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* pr,
+                       double alpha, int* nblocks) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const int nrows = g.ny * g.nz;
+  DivArgs<T> d;
+  d.ux = (const T*)ux; d.uy = (const T*)uy; d.uz = (const T*)uz; d.pr = (T*)pr;
+  d.rdyp = (c->prec == C3D_FP64 ? (const T*)c->md.rdyp : (const T*)c->mf.rdyp);
+  d.rdx = (T)c->rdx; d.rdz = (T)c->rdz;
+  d.sudtal = (T)(1.0 / alpha);
+  d.xnuhm = (T)(-0.5 * c->p.xnu);
+  d.partials = c->d_partials;
+  switch (g.nk) {
+#define X(n_)                                                                                    \
+  case n_: {                                                                                     \
+    const int nb = (nrows + xlines(n_) - 1) / xlines(n_);                                       \
+    if (nb > c->max_partials) {                                                                  \
+      set_error("prsrc_fft: too many workgroups for the partials buffer");                       \
+      return C3D_ERR_ARG;                                                                        \
+    }                                                                                            \
+    *nblocks = nb;                                                                               \
+    if (pr)                                                                                      \
+      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, true>), dim3(nb), dim3(FNT), 0, c->stream, g,  \
+                         d, (T2*)c->spec, (const T2*)c->tw_x, (const T2*)c->tw_xr);              \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, false>), dim3(nb), dim3(FNT), 0, c->stream, g, \
+                         d, (T2*)c->spec, (const T2*)c->tw_x, (const T2*)c->tw_xr);              \
+    break;                                                                                       \
+  }
     C3D_FFT_SIZES(X)
 #undef X
     default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
@@ -508,6 +628,18 @@ int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec) {
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_FFT_X_FWD, ev, 2.0 * N * c->esize);
   return C3D_OK;
+}
+
+int launch_prsrc_fft_x(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* pr) {
+  hipEvent_t ev;
+  prof_begin(c, ST_PRSRC_FFT, &ev);
+  int nb = 0;
+  int rc = c->prec == C3D_FP64 ? prsrc_fft_t<double>(c, ux, uy, uz, pr, c->pmAlpha, &nb)
+                               : prsrc_fft_t<float>(c, ux, uy, uz, pr, c->pmAlpha, &nb);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_PRSRC_FFT, ev, (pr ? 6.0 : 4.0) * N * c->esize);
+  return launch_max_finalize(c, nb, 0);
 }
 
 int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale) {

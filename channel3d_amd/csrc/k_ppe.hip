@@ -429,6 +429,11 @@ int run_ppe(c3d_ctx* c, void* prsrc, void* phi) {
   int rc;
   if (c->ppe_sweeps >= 0) return run_ppe_mixed(c, prsrc, phi, c->ppe_sweeps);
   if ((rc = launch_fft_x_fwd(c, prsrc, c->spec))) return rc;
+  return run_ppe_from_spectrum(c, phi);
+}
+
+int run_ppe_from_spectrum(c3d_ctx* c, void* phi) {
+  int rc;
   if ((rc = launch_fft_z(c, c->spec, 0))) return rc;
   if ((rc = launch_tridiag(c, c->spec))) return rc;
   if ((rc = launch_fft_z(c, c->spec, 1))) return rc;
