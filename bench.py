@@ -168,10 +168,31 @@ def run_les(case, steps, warmup, label):
                 divmax_after_step=d["divmax2"], stages=stages)
 
 
+def first_substep_source(g, L):
+    """the pressure source of a first substep from the current state, left
+    in ARR1 by the per-routine entries (the fused step hands its source to
+    the x transform without storing it); the velocity is restored after"""
+    saved = {f: g.download(f) for f in (L.UX, L.UY, L.UZ)}
+    g.PMcoeUpdate(1)
+    g.Update_uy_ym()
+    g.clcRhsX(L.UX, L.UY, L.UZ, L.ARR1, L.HISTX, L.PR)
+    g.clcRhsY(L.UX, L.UY, L.UZ, L.ARR2, L.HISTY, L.PR)
+    g.clcRhsZ(L.UX, L.UY, L.UZ, L.HALOARR, L.HISTZ, L.PR)
+    g.clcU1Hat(L.UX, L.ARR1)
+    g.clcU2Hat(L.UY, L.ARR2, L.DUY_YM)
+    g.clcU3Hat(L.UZ, L.HALOARR)
+    g.SetBC_and_UpdateHaloForPrSrc(L.UX, L.UY, L.UZ, L.UY_YM)
+    g.clcPrSrc(L.UX, L.UY, L.UZ, L.ARR1)
+    for f, a in saved.items():
+        g.upload(f, a)
+    del saved
+
+
 def run_precision_sweep(g, L, nsolves, fp64_per_s, nsteps):
     """FP64 pressure solve against FP32 + s refinement sweeps on the same
-    source (the last step's): solves/s and relative max-norm difference of
-    phi from the FP64 solution; then whole steps with the refined solve."""
+    source (ARR1, a first substep's from the current state): solves/s and relative
+    max-norm difference of phi from the FP64 solution; then whole steps with
+    the refined solve."""
     import numpy as np
 
     def interior(a):
@@ -336,7 +357,8 @@ def main(argv=None):
     if not (d1["divmax2"] < 1e-10 and math.isfinite(d1["vmax_x"])):
         log("WARNING: divergence / velocity check failed")
 
-    # Poisson solves/s on the same context (source: the last pressure source)
+    # Poisson solves/s on the same context (source: a first substep's)
+    first_substep_source(g, L)
     g.sync()
     t0 = time.perf_counter()
     for _ in range(a.ppe_solves):

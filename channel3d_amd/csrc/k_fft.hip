@@ -139,26 +139,34 @@ __host__ __device__ constexpr int pick_radix(int m) {
 }
 
 // One Stockham pass of radix R with current sub-transform size NS over
-// LINES lines of length N held in LDS (line stride LDN).
-template <typename T2, int N, int LINES, int LDN, int NT, int S, int R, int NS>
-__device__ __forceinline__ void stockham_pass(T2* sm, const T2* __restrict__ tw) {
+// LINES lines of length N.  The pass reads its R inputs per butterfly
+// through in(line, index) and writes its R outputs through
+// out(line, index, value).  The middle passes use LDS accessors; the first
+// pass reads the field itself and the last pass writes the result itself, so
+// data goes HBM -> registers -> LDS and LDS -> registers -> HBM with no
+// separate staging copy.  LF: butterfly b works on line b % LINES
+// (line-fastest, for transforms whose lines are adjacent in memory) instead
+// of line b / (N / R).  SYNC: the outputs overwrite LDS that the inputs came
+// from, so all reads finish before the first write.
+template <typename T2, int N, int LINES, int NT, int S, int R, int NS, bool LF, bool SYNC,
+          class In, class Out>
+__device__ __forceinline__ void fft_pass(const T2* __restrict__ tw, const In& in, const Out& out) {
   constexpr int NR = N / R;
   constexpr int NB = LINES * NR;
   constexpr int PER = (NB + NT - 1) / NT;
   constexpr int TWS = N / (NS * R);
   const int tid = threadIdx.x;
   T2 v[PER][R];
-  int od[PER];
+  int ln[PER], od[PER];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int gb = tid + p * NT;
-    od[p] = -1;
+    ln[p] = -1;
     if (NB % NT == 0 || gb < NB) {
-      const int line = gb / NR;
-      const int j = gb - line * NR;
-      const int base = line * LDN;
+      const int line = LF ? gb % LINES : gb / NR;
+      const int j = LF ? gb / LINES : gb - line * NR;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[p][r] = sm[base + j + r * NR];
+      for (int r = 0; r < R; ++r) v[p][r] = in(line, j + r * NR);
       const int k = j % NS;
       if (NS > 1) {
 #pragma unroll
@@ -169,27 +177,46 @@ __device__ __forceinline__ void stockham_pass(T2* sm, const T2* __restrict__ tw)
         }
       }
       Dft<R, S, T2>::run(v[p]);
-      od[p] = base + (j / NS) * NS * R + k;
+      ln[p] = line;
+      od[p] = (j / NS) * NS * R + k;
     }
   }
-  __syncthreads();
+  if (SYNC) __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
-    if (od[p] >= 0) {
+    if (ln[p] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) sm[od[p] + r * NS] = v[p][r];
+      for (int r = 0; r < R; ++r) out(ln[p], od[p] + r * NS, v[p][r]);
     }
   }
-  __syncthreads();
 }
 
-template <typename T2, int N, int LINES, int LDN, int NT, int S, int NS>
-__device__ __forceinline__ void stockham(T2* sm, const T2* __restrict__ tw) {
-  if constexpr (NS < N) {
-    constexpr int R = pick_radix(N / NS);
-    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported FFT length");
-    stockham_pass<T2, N, LINES, LDN, NT, S, R, NS>(sm, tw);
-    stockham<T2, N, LINES, LDN, NT, S, NS * R>(sm, tw);
+// The whole transform of length N on LINES lines: first pass from `in`,
+// last pass to `out`, the passes between in LDS (sm, line stride LDN).
+// OUT_LDS: `out` writes sm, so the last pass synchronises like a middle one
+// and the transform ends with a barrier.
+template <typename T2, int N, int LINES, int LDN, int NT, int S, bool LF, bool OUT_LDS, int NS,
+          class In, class Out>
+__device__ __forceinline__ void stockham_io(T2* sm, const T2* __restrict__ tw, const In& in,
+                                            const Out& out) {
+  constexpr int R = pick_radix(N / NS);
+  static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported FFT length");
+  constexpr bool FIRST = NS == 1, LAST = NS * R == N;
+  auto lds_in = [sm](int l, int m) { return sm[l * LDN + m]; };
+  auto lds_out = [sm](int l, int m, const T2& v) { sm[l * LDN + m] = v; };
+  if constexpr (LAST) {
+    if constexpr (FIRST)
+      fft_pass<T2, N, LINES, NT, S, R, NS, LF, false>(tw, in, out);
+    else
+      fft_pass<T2, N, LINES, NT, S, R, NS, LF, OUT_LDS>(tw, lds_in, out);
+    if (OUT_LDS) __syncthreads();
+  } else {
+    if constexpr (FIRST)
+      fft_pass<T2, N, LINES, NT, S, R, NS, LF, false>(tw, in, lds_out);
+    else
+      fft_pass<T2, N, LINES, NT, S, R, NS, false, true>(tw, lds_in, lds_out);
+    __syncthreads();
+    stockham_io<T2, N, LINES, LDN, NT, S, LF, OUT_LDS, NS * R>(sm, tw, in, out);
   }
 }
 
@@ -201,6 +228,39 @@ constexpr int FNT = 512;
 
 __host__ __device__ constexpr int xlines(int N) { return N <= 128 ? 16 : N <= 256 ? 8 : N <= 512 ? 8 : 4; }
 constexpr int ZTK = 8;
+
+// Offsets of the LINES field rows (jj = 1..ny) of a workgroup, -1 past the
+// last row; thread t < LINES fills entry t (a barrier follows).
+__device__ __forceinline__ void row_offsets(const Geom& g, int row0, int lines, long long* rb) {
+  const int t = threadIdx.x;
+  if (t < lines) {
+    const int row = row0 + t;
+    rb[t] = row < g.ny * g.nz ? (long long)g.nx * (row % g.ny + 1) + g.plane * (row / g.ny) : -1;
+  }
+}
+
+// The half spectrum kx = 0..N of each real row from the complex transform Z
+// of its even/odd pairs (in LDS): X_k = (Z_k + conj Z_{N-k})/2
+// + w_k (Z_k - conj Z_{N-k})/(2i).
+template <typename T, int N, int LINES, int LDN, int NT>
+__device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
+                                           const typename CT<T>::t* sm,
+                                           typename CT<T>::t* __restrict__ spec,
+                                           const typename CT<T>::t* __restrict__ twr) {
+  using T2 = typename CT<T>::t;
+  const int nrows = g.ny * g.nz;
+  for (int e = threadIdx.x; e < LINES * (N + 1); e += NT) {
+    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
+    if (row >= nrows) continue;
+    const T2 zk = sm[l * LDN + (k == N ? 0 : k)];
+    const T2 zn = sm[l * LDN + (k == 0 ? 0 : N - k)];
+    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
+    const T dr = zk.x - zn.x, di = zk.y + zn.y;
+    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
+    const T2 wo = cmul(twr[k], o);
+    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // x forward: field rows (jj = 1..ny) -> spectral rows, kx = 0..N (N = nx/2)
@@ -217,42 +277,28 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict
   constexpr int LINES = xlines(N);
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
-  const int nrows = g.ny * g.nz;
+  __shared__ long long rb[LINES];
   const int row0 = blockIdx.x * LINES;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < LINES * N; e += FNT) {
-    const int l = e / N, m = e - l * N, row = row0 + l;
-    T2 v = mk<T2>(0, 0);
-    if (row < nrows) {
-      const int j0 = row % g.ny, k = row / g.ny;
-      const TI2* rp = (const TI2*)(in + (long long)g.nx * (j0 + 1) + g.plane * k);
-      const TI2 w = rp[m];
-      v = mk<T2>((T)w.x, (T)w.y);
-    }
-    sm[l * LDN + m] = v;
-  }
+  row_offsets(g, row0, LINES, rb);
   __syncthreads();
-  stockham<T2, N, LINES, LDN, FNT, -1, 1>(sm, tw);
-  for (int e = tid; e < LINES * (N + 1); e += FNT) {
-    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
-    if (row >= nrows) continue;
-    const T2 zk = sm[l * LDN + (k == N ? 0 : k)];
-    const T2 zn = sm[l * LDN + (k == 0 ? 0 : N - k)];
-    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
-    const T dr = zk.x - zn.x, di = zk.y + zn.y;
-    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
-    const T2 wo = cmul(twr[k], o);
-    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
-  }
+  auto ld = [&](int l, int m) -> T2 {
+    const long long b = rb[l];
+    if (b < 0) return mk<T2>(0, 0);
+    const TI2 w = ((const TI2*)(in + b))[m];
+    return mk<T2>((T)w.x, (T)w.y);
+  };
+  auto st = [&](int l, int m, const T2& v) { sm[l * LDN + m] = v; };
+  stockham_io<T2, N, LINES, LDN, FNT, -1, false, true, 1>(sm, tw, ld, st);
+  x_fwd_post<T, N, LINES, LDN, FNT>(g, row0, sm, spec, twr);
 }
 
 // ---------------------------------------------------------------------------
 // The pressure source fused into the x forward pass: clcPrSrc_FIMP
 // (m_TScheme.f90:246-276; without PCORR clcPrSrcOther, :213-241) evaluated
-// for the two x points of every complex element as the row is staged, so the
-// source never goes through HBM.  Same expressions and order as the stand-
-// alone source kernel, without contraction; the per-block max |div| goes to
-// partials[blockIdx.x].
+// for the two x points of every complex element as the first FFT pass loads
+// it, so the source never goes through HBM.  Same expressions and order as
+// the stand-alone source kernel, without contraction; the per-block max
+// |div| goes to partials[blockIdx.x].
 // ---------------------------------------------------------------------------
 template <typename T>
 struct DivArgs {
@@ -274,65 +320,66 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
   constexpr int LINES = xlines(N);
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
+  __shared__ long long rb[LINES], rkp[LINES];
+  __shared__ T rdy[LINES];
   __shared__ double wmax[FNT / 64];
-  const int nrows = g.ny * g.nz;
   const int row0 = blockIdx.x * LINES;
   const int tid = threadIdx.x;
   const int nx = g.nx;
-  double dm = 0.0;
-  for (int e = tid; e < LINES * N; e += FNT) {
-    const int l = e / N, m = e - l * N, row = row0 + l;
-    T2 v = mk<T2>(0, 0);
-    if (row < nrows) {
-#pragma clang fp contract(off)
-      const int j0 = row % g.ny, k = row / g.ny;
-      const int jc = j0 + 1, kp = k == g.nz - 1 ? 0 : k + 1;
-      const long long rowc = (long long)nx * jc + g.plane * k;
-      const long long rowkp = (long long)nx * jc + g.plane * kp;
-      const T sucaj = d.rdyp[jc];
-      T val[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int i = 2 * m + h, ip = i == nx - 1 ? 0 : i + 1;
-        const long long c = rowc + i;
-        const T rdiv = (d.ux[rowc + ip] - d.ux[c]) * d.rdx + (d.uy[c + nx] - d.uy[c]) * sucaj +
-                       (d.uz[rowkp + i] - d.uz[c]) * d.rdz;
-        dm = fmax(dm, (double)fabs(rdiv));
-        val[h] = d.sudtal * rdiv;
-        if (PCORR) d.pr[c] = d.pr[c] + d.xnuhm * rdiv;
-      }
-      v = mk<T2>(val[0], val[1]);
+  if (tid < LINES) {
+    const int row = row0 + tid;
+    long long o = -1, okp = 0;
+    T sy = T(0);
+    if (row < g.ny * g.nz) {
+      const int jc = row % g.ny + 1, k = row / g.ny;
+      const int kp = k == g.nz - 1 ? 0 : k + 1;
+      o = (long long)nx * jc + g.plane * k;
+      okp = (long long)nx * jc + g.plane * kp;
+      sy = d.rdyp[jc];
     }
-    sm[l * LDN + m] = v;
+    rb[tid] = o;
+    rkp[tid] = okp;
+    rdy[tid] = sy;
   }
-  // block max of |div|
+  __syncthreads();
+  double dm = 0.0;
+  auto ld = [&](int l, int m) -> T2 {
+#pragma clang fp contract(off)
+    const long long rowc = rb[l];
+    if (rowc < 0) return mk<T2>(0, 0);
+    const long long rowkp = rkp[l];
+    const T sucaj = rdy[l];
+    T val[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = 2 * m + h, ip = i == nx - 1 ? 0 : i + 1;
+      const long long c = rowc + i;
+      const T rdiv = (d.ux[rowc + ip] - d.ux[c]) * d.rdx + (d.uy[c + nx] - d.uy[c]) * sucaj +
+                     (d.uz[rowkp + i] - d.uz[c]) * d.rdz;
+      dm = fmax(dm, (double)fabs(rdiv));
+      val[h] = d.sudtal * rdiv;
+      if (PCORR) d.pr[c] = d.pr[c] + d.xnuhm * rdiv;
+    }
+    return mk<T2>(val[0], val[1]);
+  };
+  auto st = [&](int l, int m, const T2& v) { sm[l * LDN + m] = v; };
+  stockham_io<T2, N, LINES, LDN, FNT, -1, false, true, 1>(sm, tw, ld, st);
+  // block max of |div| (the transform's barriers order the wmax writes)
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
   if ((tid & 63) == 0) wmax[tid >> 6] = dm;
+  x_fwd_post<T, N, LINES, LDN, FNT>(g, row0, sm, spec, twr);
   __syncthreads();
   if (tid == 0) {
     double r = 0.0;
     for (int q = 0; q < FNT / 64; ++q) r = fmax(r, wmax[q]);
     d.partials[blockIdx.x] = r;
   }
-  stockham<T2, N, LINES, LDN, FNT, -1, 1>(sm, tw);
-  for (int e = tid; e < LINES * (N + 1); e += FNT) {
-    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
-    if (row >= nrows) continue;
-    const T2 zk = sm[l * LDN + (k == N ? 0 : k)];
-    const T2 zn = sm[l * LDN + (k == 0 ? 0 : N - k)];
-    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
-    const T dr = zk.x - zn.x, di = zk.y + zn.y;
-    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
-    const T2 wo = cmul(twr[k], o);
-    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
-  }
 }
 
 // ---------------------------------------------------------------------------
 // x inverse: spectral rows -> field rows (jj = 1..ny), times scale
+// (ACC: added to the field -- the correction of a refinement sweep)
 // ---------------------------------------------------------------------------
-// TOUT: precision of the field written; ACC: add to it (the correction of a
-// refinement sweep) instead of overwriting it
 template <typename T, int N, typename TOUT = T, bool ACC = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>::t* __restrict__ spec,
                                                    TOUT* __restrict__ out,
@@ -344,47 +391,46 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
   constexpr int LINES = xlines(N);
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
-  const int nrows = g.ny * g.nz;
+  __shared__ long long rb[LINES];
   const int row0 = blockIdx.x * LINES;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < LINES * N; e += FNT) {
-    const int l = e / N, k = e - l * N, row = row0 + l;
-    T2 z = mk<T2>(0, 0);
-    if (row < nrows) {
-      const T2* sp = spec + (long long)g.P * row;
-      T2 xk = sp[k], xn = sp[N - k];
-      if (k == 0) {
-        xk.y = 0;
-        xn.y = 0;
-      }
-      const T2 a = mk<T2>(xk.x + xn.x, xk.y - xn.y);
-      const T2 b = mk<T2>(xk.x - xn.x, xk.y + xn.y);
-      T2 w = twr[k];
-      w.y = -w.y;
-      const T2 wb = cmul(w, b);
-      z = mk<T2>(a.x - wb.y, a.y + wb.x);
-    }
-    sm[l * LDN + k] = z;
-  }
+  const int nrows = g.ny * g.nz;
+  row_offsets(g, row0, LINES, rb);
   __syncthreads();
-  stockham<T2, N, LINES, LDN, FNT, 1, 1>(sm, tw);
-  for (int e = tid; e < LINES * N; e += FNT) {
-    const int l = e / N, m = e - l * N, row = row0 + l;
-    if (row >= nrows) continue;
-    const int j0 = row % g.ny, k = row / g.ny;
-    TO2* rp = (TO2*)(out + (long long)g.nx * (j0 + 1) + g.plane * k);
-    const T2 z = sm[l * LDN + m];
+  // Z_k = (X_k + conj X_{N-k}) + i conj(w_k) (X_k - conj X_{N-k}), the
+  // complex sequence whose inverse transform interleaves the real row
+  auto ld = [&](int l, int k) -> T2 {
+    const int row = row0 + l;
+    if (row >= nrows) return mk<T2>(0, 0);
+    const T2* sp = spec + (long long)g.P * row;
+    T2 xk = sp[k], xn = sp[N - k];
+    if (k == 0) {
+      xk.y = 0;
+      xn.y = 0;
+    }
+    const T2 a = mk<T2>(xk.x + xn.x, xk.y - xn.y);
+    const T2 b = mk<T2>(xk.x - xn.x, xk.y + xn.y);
+    T2 w = twr[k];
+    w.y = -w.y;
+    const T2 wb = cmul(w, b);
+    return mk<T2>(a.x - wb.y, a.y + wb.x);
+  };
+  auto st = [&](int l, int m, const T2& z) {
+    const long long b = rb[l];
+    if (b < 0) return;
+    TO2* rp = (TO2*)(out + b);
     if (ACC) {
       const TO2 o = rp[m];
       rp[m] = mk<TO2>(o.x + (TOUT)z.x * scale, o.y + (TOUT)z.y * scale);
     } else {
       rp[m] = mk<TO2>((TOUT)z.x * scale, (TOUT)z.y * scale);
     }
-  }
+  };
+  stockham_io<T2, N, LINES, LDN, FNT, 1, false, false, 1>(sm, tw, ld, st);
 }
 
 // ---------------------------------------------------------------------------
-// z, in place: TK consecutive kx of one y plane, all kz
+// z, in place: TK consecutive kx of one y plane, all kz.  The first pass
+// reads and the last pass writes TK*16-byte runs (line-fastest butterflies).
 // ---------------------------------------------------------------------------
 template <typename T, int N, int S>
 __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __restrict__ spec,
@@ -394,23 +440,14 @@ __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __rest
   constexpr int LDN = N + 1;
   __shared__ T2 sm[TK * LDN];
   const int kx0 = blockIdx.x * TK;
-  const int j0 = blockIdx.y;
-  const int nkx = g.nk + 1;
+  const int nl = min(TK, g.nk + 1 - kx0);  // lines of this tile
   const long long sk = (long long)g.P * g.ny;
-  T2* base = spec + (long long)g.P * j0;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < TK * N; e += FNT) {
-    const int kz = e / TK, l = e - kz * TK, kx = kx0 + l;
-    T2 v = mk<T2>(0, 0);
-    if (kx < nkx) v = base[sk * kz + kx];
-    sm[l * LDN + kz] = v;
-  }
-  __syncthreads();
-  stockham<T2, N, TK, LDN, FNT, S, 1>(sm, tw);
-  for (int e = tid; e < TK * N; e += FNT) {
-    const int kz = e / TK, l = e - kz * TK, kx = kx0 + l;
-    if (kx < nkx) base[sk * kz + kx] = sm[l * LDN + kz];
-  }
+  T2* base = spec + (long long)g.P * blockIdx.y + kx0;
+  auto ld = [&](int l, int kz) -> T2 { return l < nl ? base[sk * kz + l] : mk<T2>(0, 0); };
+  auto st = [&](int l, int kz, const T2& v) {
+    if (l < nl) base[sk * kz + l] = v;
+  };
+  stockham_io<T2, N, TK, LDN, FNT, S, true, false, 1>(sm, tw, ld, st);
 }
 
 // ===========================================================================

@@ -233,8 +233,11 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
   }
 }
 
+// At least 4 waves per SIMD: the FP64 body fits 128 VGPRs without spilling
+// (136 unconstrained, 3 waves), and at 768x384x576 the 1008 workgroups then
+// run in one round instead of 1.3 -- 0.99 ms per solve instead of 1.21 ms.
 template <typename T, int C>
-__global__ __launch_bounds__(256) void k_tridiag_ck(Geom g, Metrics<T> m,
+__global__ __launch_bounds__(256, 4) void k_tridiag_ck(Geom g, Metrics<T> m,
                                                     typename CT2<T>::t* __restrict__ spec,
                                                     T* __restrict__ ckp) {
   k_tridiag_ck_body<T, C>(g, m, spec, ckp);
