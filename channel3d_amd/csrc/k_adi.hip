@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
   const bool active = rowok && i < nx;
   const int k0 = bz * kchunk, k1 = min(nz, k0 + kchunk);
   const T add = ADD ? (T)sc->dpmdxns : T(0);
-  const int w = 64 + 2 * mx;  // <= 128: two row elements per thread
+  // a staged row holds 64 + 2 mx <= 128 values: two per thread
   // x positions this thread stages (periodic), fixed for the whole march
   int ix[2];
 #pragma unroll
@@ -110,44 +110,54 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
     ix[q] = ii < 0 ? ii + nx : ii;
   }
   auto wrapk = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
-  // the row of plane kk, into registers (issued one plane ahead of its use)
+  // the row of plane kk, into registers.  Both loads are unconditional --
+  // ix[] wraps to valid columns and a row past the last one (ny not a
+  // multiple of 4) reads row ny, whose results are never stored -- so no
+  // branch surrounds them and the only wait is the one before staging.
+  const int jl = rowok ? jc : g.ny;
   auto load = [&](int kk, T (&v)[2]) {
-    const long long row = (long long)nx * jc + g.plane * kk;
+    const long long row = (long long)nx * jl + g.plane * kk;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      v[q] = T(0);
-      if (rowok && tx + 64 * q < w) v[q] = f[row + ix[q]];
-    }
+    for (int q = 0; q < 2; ++q) v[q] = f[row + ix[q]];
   };
-  // the constant-flow-rate shift is added when the row is staged, not when it
-  // is loaded, so the load of the next plane stays in flight
-  auto xconv = [&](const T (&v)[2]) -> T {
+  // One plane: stage the row loaded one step earlier in LDS (plus the
+  // constant-flow-rate shift), reuse its registers for the next plane's load
+  // -- in flight through this plane's x and z convolutions and store -- and
+  // convolve along x.  Row ty of sm belongs to wavefront ty alone (64 x 4
+  // threads, one row per wave) and a wave's LDS accesses complete in issue
+  // order, so no workgroup barrier is needed: the wave barriers only keep the
+  // compiler from moving the reads above the writes or the next plane's
+  // writes above the reads.
+  // The previous plane's result is stored between the staging and the next
+  // load: the wait before the next staging then covers a store issued a
+  // whole step earlier rather than one issued just before it (loads and
+  // stores share one completion counter).
+  T raw[2];
+  int kn = k0 - MZ;            // next plane to convolve; its row is in raw
+  const int klast = k1 + MZ;   // one past the last plane needed
+  T pend = 0;                  // result waiting to be stored at pend_at
+  long long pend_at = -1;
+  load(wrapk(kn), raw);
+  auto step = [&]() -> T {
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-      if (tx + 64 * q < w) sm[ty][tx + 64 * q] = (ADD && rowok) ? v[q] + add : v[q];
-    __syncthreads();
+    for (int q = 0; q < 2; ++q) sm[ty][tx + 64 * q] = ADD ? raw[q] + add : raw[q];
+    if (pend_at >= 0) out[pend_at] = pend;
+    pend_at = -1;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (kn + 1 < klast) load(wrapk(kn + 1), raw);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
     T acc = 0;
     for (int t = 0; t <= 2 * mx; ++t) acc += (T)xt[t] * sm[ty][tx + t];
-    __syncthreads();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    ++kn;
     return acc;
   };
   // ring of x-convolved planes: plane p sits in slot (p - k0 + MZ) mod NZT;
   // unrolling by NZT makes every slot index a compile-time constant.  The
-  // planes are consumed in the order k0-MZ, k0-MZ+1, ..., k1+MZ-1; cur holds
-  // the next one's row, nxt the one after it.
+  // planes are consumed in the order k0-MZ, k0-MZ+1, ..., k1+MZ-1.
   T win[NZT];
-  T cur[2], nxt[2];
-  int kn = k0 - MZ;            // next plane to convolve
-  const int klast = k1 + MZ;   // one past the last plane needed
-  load(wrapk(kn), cur);
-  auto step = [&]() -> T {
-    if (kn + 1 < klast) load(wrapk(kn + 1), nxt);
-    const T r = xconv(cur);
-#pragma unroll
-    for (int q = 0; q < 2; ++q) cur[q] = nxt[q];
-    ++kn;
-    return r;
-  };
 #pragma unroll
   for (int q = 0; q < NZT - 1; ++q) win[q] = step();
   for (int kb = k0; kb < k1; kb += NZT) {
@@ -159,10 +169,12 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
         T acc = 0;
 #pragma unroll
         for (int q = 0; q < NZT; ++q) acc += (T)zt[q] * win[(s + q) % NZT];
-        if (active) out[(long long)nx * jc + g.plane * k + i] = acc;
+        pend = acc;
+        pend_at = active ? (long long)nx * jc + g.plane * k + i : -1;
       }
     }
   }
+  if (pend_at >= 0) out[pend_at] = pend;
 }
 
 // Thomas sweeps along y with shared factors: a(j), vecm(j), arrmn(j) for
