@@ -88,6 +88,36 @@ def run_cpu_baseline(case, ux, uy, uz, threads):
     return t_ppe, t_sub
 
 
+def time_steps(g, steps, warmup, graph):
+    """seconds for `steps` steps after `warmup` untimed ones, no profiling;
+    graph: replay HIP graphs of the step (the warm-up captures them)"""
+    g.set_graph(graph)
+    for _ in range(warmup):
+        g.step()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step()
+    g.sync()
+    t = time.perf_counter() - t0
+    g.set_graph(False)
+    return t
+
+
+def stage_split(g, steps):
+    """per-stage HIP-event timing over `steps` launched steps"""
+    g.profile(True)
+    g.profile_reset()
+    for _ in range(steps):
+        g.step()
+    g.sync()
+    st = g.profile_stats()
+    g.profile(False)
+    return {k: dict(calls=v["calls"], avg_ms=round(v["total_ms"] / v["calls"], 4),
+                    gbs=round(v["bytes_per_call"] / (v["total_ms"] / v["calls"] * 1e-3) / 1e9, 1)
+                    if v["bytes_per_call"] else None) for k, v in st.items()}
+
+
 def run_fourth_order(steps, warmup):
     """the 4th-order step at 192x128x160 on the same GPU: steps/s and the
     per-stage split (inputs resident, HIP events on the library stream)"""
@@ -105,27 +135,16 @@ def run_fourth_order(steps, warmup):
     g.upload(L.UY, to4(uy))
     g.upload(L.UZ, to4(uz))
     g.project()
-    for _ in range(warmup):
-        g.step()
-    g.sync()
-    g.profile(True)
-    g.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.step()
-    g.sync()
-    t = time.perf_counter() - t0
-    st = g.profile_stats()
-    g.profile(False)
+    t = time_steps(g, steps, warmup, False)
+    tg = time_steps(g, steps, max(warmup, 3), True)
+    stages = stage_split(g, steps)
     d = g.diagnostics()
     g.close()
-    stages = {k: dict(calls=v["calls"], avg_ms=round(v["total_ms"] / v["calls"], 4),
-                      gbs=round(v["bytes_per_call"] / (v["total_ms"] / v["calls"] * 1e-3) / 1e9, 1)
-                      if v["bytes_per_call"] else None) for k, v in st.items()}
     return dict(workload=f"4th-order channel RK3 step {nx}x{ny}x{nz}, Re_tau=180, FP64",
                 steps=steps, ms_per_step=round(1e3 * t / steps, 3),
                 steps_per_s=round(steps / t, 2),
                 grid_points_steps_per_s=round(nx * ny * nz * steps / t, 1),
+                graph_ms_per_step=round(1e3 * tg / steps, 3),
                 divmax_after_step=d["divmax2"], stages=stages)
 
 
@@ -142,29 +161,18 @@ def run_les(case, steps, warmup, label):
     g.upload(L.UZ, uz)
     del ux, uy, uz
     g.project()
-    for _ in range(warmup):
-        g.step()
-    g.sync()
-    g.profile(True)
-    g.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.step()
-    g.sync()
-    t = time.perf_counter() - t0
-    st = g.profile_stats()
-    g.profile(False)
+    t = time_steps(g, steps, warmup, False)
+    tg = time_steps(g, steps, max(warmup, 3), True)
+    stages = stage_split(g, steps)
     d = g.diagnostics()
     g.close()
-    stages = {k: dict(calls=v["calls"], avg_ms=round(v["total_ms"] / v["calls"], 4),
-                      gbs=round(v["bytes_per_call"] / (v["total_ms"] / v["calls"] * 1e-3) / 1e9, 1)
-                      if v["bytes_per_call"] else None) for k, v in st.items()}
     model = {2: "WALE", 3: "MTS"}[case["les_type"]]
     return dict(workload=f"{label}: 2nd-order LES channel RK3 step {nx}x{ny}x{nz}, {model}, "
                          "partially implicit scheme, FP64",
                 steps=steps, ms_per_step=round(1e3 * t / steps, 3),
                 steps_per_s=round(steps / t, 2),
                 grid_points_steps_per_s=round(nx * ny * nz * steps / t, 1),
+                graph_ms_per_step=round(1e3 * tg / steps, 3),
                 divmax_after_step=d["divmax2"], stages=stages)
 
 

@@ -183,6 +183,10 @@ class Channel:
         """-1: FP64 pressure solve; s >= 0: FP32 solve + s FP64-residual sweeps"""
         L.check(self.lib.c3d_set_ppe_refine(self._h, int(sweeps)), "c3d_set_ppe_refine")
 
+    def set_graph(self, on=True):
+        """step() replays HIP graphs of the whole step (captured on first use)"""
+        L.check(self.lib.c3d_set_graph(self._h, int(bool(on))), "c3d_set_graph")
+
     def clcSpectra1D(self, jlcs=(1, 2), ux=L.UX, uy=L.UY, uz=L.UZ, pressure=L.PR):
         """one sample of the 4th-order 1-D spectra (clcStat with clcSpectra1D)"""
         L.check(self.lib.c3d_clcSpectra1D(self._h, *[self.ptr(f) for f in (ux, uy, uz, pressure)],

@@ -634,6 +634,15 @@ static size_t layout_elems(const c3d_ctx* c, int layout) {
   return (size_t)c->g.nfield;
 }
 
+// the captured steps bake kernel arguments (dt-derived coefficients, solver
+// forms, LES tables): every setter that changes one drops them
+static void drop_graphs(c3d_ctx* c) {
+  for (auto& x : c->gexec) {
+    if (x) hipGraphExecDestroy(x);
+    x = nullptr;
+  }
+}
+
 extern "C" int c3d_destroy(c3d_ctx* c) {
   if (!c) return C3D_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -642,6 +651,8 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
     hipEventDestroy(q.b);
   }
   for (auto e : c->ev_free) hipEventDestroy(e);
+  for (auto& x : c->gexec)
+    if (x) hipGraphExecDestroy(x);
   for (int i = 0; i < C3D_NSTATE; ++i)
     if (c->state[i]) hipFree(c->state[i]);
   for (void* p : c->user_fields) hipFree(p);
@@ -859,6 +870,7 @@ extern "C" int c3d_set_ppe_refine(c3d_ctx* c, int sweeps) {
   int rc = c3d_sync(c);
   if (rc) return rc;
   if (sweeps >= 0 && (rc = ensure_mixed_tables(c))) return rc;
+  if (sweeps != c->ppe_sweeps) drop_graphs(c);
   c->ppe_sweeps = sweeps;
   return C3D_OK;
 }
@@ -869,6 +881,7 @@ extern "C" int c3d_set_dt(c3d_ctx* c, double dt) {
     return C3D_ERR_ARG;
   }
   C3D_HIP(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->dt = dt;
   return build_adi_tables(c);
 }
@@ -1386,8 +1399,7 @@ extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   return launch_bc_pr(c, s[C3D_PR]);
 }
 
-extern "C" int c3d_step(c3d_ctx* c) {
-  CHECK_CTX(c);
+static int step_body(c3d_ctx* c) {
   int rc;
   void** s = c->state;
   // CalcMaxCFL on the state entering the step (:81); it was computed by the
@@ -1413,6 +1425,59 @@ extern "C" int c3d_step(c3d_ctx* c) {
   c->last_cfl_slot = c->cfl_slot;
   c->cfl_slot = next;
   c->cfl_valid = true;
+  return C3D_OK;
+}
+
+// With graphs on, a step whose CFL is already on the device (every step
+// after the first) is captured once per CFL slot and replayed: the capture
+// runs step_body's host side (the substeps' PMcoeUpdate, SimTime, the slot
+// switch) without executing anything, the launch then executes the step;
+// later steps redo the same host side and launch the cached graph.  Nothing
+// in step_body synchronises, copies or sets memory, so the capture holds
+// kernels only.
+static int step_graph(c3d_ctx* c) {
+  const int slot = c->cfl_slot;
+  if (!c->gexec[slot]) {
+    C3D_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = step_body(c);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "c3d_step: hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&c->gexec[slot], graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      c->gexec[slot] = nullptr;
+      return hip_fail(ei, "c3d_step: hipGraphInstantiate");
+    }
+  } else {
+    // the host side of step_body, with the device work left to the graph
+    for (int ns = 1; ns <= c->iadvance; ++ns) {
+      int rc = c3d_PMcoeUpdate(c, ns);
+      if (rc) return rc;
+    }
+    c->last_cfl_slot = c->cfl_slot;
+    c->cfl_slot ^= 1;
+  }
+  C3D_HIP(hipGraphLaunch(c->gexec[slot], c->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_step(c3d_ctx* c) {
+  CHECK_CTX(c);
+  if (c->graph_on && c->cfl_valid && !c->prof) return step_graph(c);
+  return step_body(c);
+}
+
+extern "C" int c3d_set_graph(c3d_ctx* c, int on) {
+  CHECK_CTX(c);
+  int rc = c3d_sync(c);
+  if (rc) return rc;
+  drop_graphs(c);
+  c->graph_on = on != 0;
   return C3D_OK;
 }
 
@@ -1479,6 +1544,7 @@ static void fill_lesmet(c3d_ctx* c, std::vector<T>& h, LesMet<T>& lm, T* d) {
 
 extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
   CHECK_CTX(c);
+  drop_graphs(c);
   if (c->order != 2) {
     set_error("c3d_set_les: the LES layer belongs to the 2nd-order solver");
     return C3D_ERR_ARG;

@@ -262,6 +262,11 @@ struct c3d_ctx {
   int stat_n = 0;             // nfstime
   bool stat_grad = false;     // 4th order: also the 98 gradient moments
 
+  // c3d_step replayed from HIP graphs, one per CFL slot parity (the slot the
+  // closing diagnostics write alternates); dropped when a baked value changes
+  bool graph_on = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
   // profiling
   bool prof = false;
   struct Pending { int stage; hipEvent_t a, b; };

@@ -241,6 +241,11 @@ module channel3d_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: sweeps
     end function
+    integer(c_int) function c3d_set_graph(ctx, on) bind(C, name="c3d_set_graph")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
 
     ! ---- 4th-order spectra (SpectraOptions, CFD_4th m_FlowCase.f90:644-938)
     integer(c_int) function c3d_clcSpectra1D(ctx, ux, uy, uz, pr, j1, j2) &

@@ -47,7 +47,7 @@ EXPORTS = (
     "c3d_set_ppe_refine", "c3d_clcStat", "c3d_stat_set_grad", "c3d_stat_get_grad",
     "c3d_write_vgrad", "c3d_set_les", "c3d_Clc_SGS_Vis",
     "c3d_clcSpectra1D", "c3d_spectra_get", "c3d_spectra_reset", "c3d_write_spectra",
-    "c3d_clcSpectra2D", "c3d_spectra2d_get", "c3d_write_spectra2d",
+    "c3d_clcSpectra2D", "c3d_spectra2d_get", "c3d_write_spectra2d", "c3d_set_graph",
 )
 
 
@@ -145,6 +145,7 @@ def load(path: str | None = None):
         "c3d_write_stat": ([_VP, ctypes.c_char_p, I, I], I),
         "c3d_append_prgrad": ([_VP, ctypes.c_char_p, I], I),
         "c3d_set_ppe_refine": ([_VP, I], I),
+        "c3d_set_graph": ([_VP, I], I),
         "c3d_clcStat": ([_VP] * 5, I),
         "c3d_stat_set_grad": ([_VP, I], I),
         "c3d_stat_get_grad": ([_VP, _DP], I),

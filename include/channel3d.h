@@ -203,6 +203,14 @@ int c3d_substep(c3d_ctx* ctx, int ns);
 /* one time step: CFL, all substeps, divergence / vmax checks
  * (m_ChannelSystem.f90:73-130); no host sync */
 int c3d_step(c3d_ctx* ctx);
+/* on != 0: c3d_step replays HIP graphs of the whole step (no counterpart
+ * in the reference, which has no device; a launch-count optimisation for
+ * small grids).  Each graph is captured on the first step that needs it,
+ * from a step whose CFL is already on the device; the graphs are dropped
+ * by c3d_set_dt, c3d_set_ppe_refine, c3d_set_les and c3d_set_graph, and
+ * not used while profiling.  Results are bit for bit those of the launches
+ * it replaces.  Synchronous. */
+int c3d_set_graph(c3d_ctx* ctx, int on);
 /* diagnostics of the last step, synchronous:
  * out[0]=cfl*dt, [1]=divmax before projection (last substep), [2]=divmax
  * after, [3..5]=vmax, [6]=PrGradNow, [7]=PrGradAver, [8]=SimTime */
