@@ -1,0 +1,791 @@
+// Learned subgrid-stress closure: samples from a DNS state and a small MLP
+// trained on them with bf16 MFMA.  No counterpart in the reference, which
+// has WALE and MTS only (src/CFD_2nd/m_LESModels.f90:53-58).
+//
+//  - Samples: the DNS velocity of an FP64 2nd-order context, centred in its
+//    cells (face averages), box-filtered over fx x fy x fz cell blocks (y
+//    weighted by cell height) onto the LES grid; per LES cell the inputs are
+//    the 9 filtered velocity gradients over |S| = sqrt(2 S:S) and the wall
+//    distance over the half height, padded to 16; the targets are the SGS
+//    stresses tau_ab = bar(u_a u_b) - bar(u_a) bar(u_b) over Delta^2 |S|^2.
+//    Inputs and targets are clipped to +-10.  FP64 arithmetic, FP32 storage.
+//  - Network: 10 -> 64 -> 64 -> 64 -> 6, SiLU, linear output, mean square
+//    error.  One wavefront runs 16 samples through all layers with
+//    v_mfma_f32_16x16x16_bf16 in the transposed form Z^T = W^T H^T: the
+//    accumulator layout of one layer's output (rows 4*(lane/16)+i, column
+//    lane%16) is exactly the B-operand layout of the next layer, so the
+//    activations never leave registers.  The backward pass forms dH^T =
+//    W dZ^T the same way; the weight gradients dW^T = dZ^T H need both
+//    operands transposed (16x16 tiles through LDS) and accumulate in LDS.
+//    Operands are bf16 (round to nearest even), accumulation FP32; master
+//    weights and Adam moments FP32.
+#include "c3d_internal.h"
+
+#include <algorithm>
+#include <random>
+
+namespace c3d {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = 10, NPAD = 16, NH = 64, NOUT = 6;
+// flat parameter layout: W1 (16 x 64), b1, W2 (64 x 64), b2, W3, b3, W4 (64 x 6), b4
+constexpr int OFF_W[4] = {0, 1088, 5248, 9408};
+constexpr int OFF_B[4] = {1024, 5184, 9344, 9792};
+constexpr int NPARAM = 9798;
+constexpr int IND[4] = {NPAD, NH, NH, NH};
+constexpr int OUTD[4] = {NH, NH, NH, NOUT};
+// bf16 A fragments: forward (W^T tiles) per layer (o, kt), backward (W tiles)
+// per layer 2..4 (it, ot)
+constexpr int OT[4] = {4, 4, 4, 1}, KT[4] = {1, 4, 4, 4};
+constexpr int FWD_OFF[4] = {0, 4, 20, 36};
+constexpr int BWD_OFF[4] = {-1, 0, 16, 32};
+constexpr int NFWD = 40, NBWD = 36;
+constexpr float CLIPV = 10.0f;
+constexpr int GRAD_WG_MAX = 512;
+
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  unsigned int b = __float_as_uint(f);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (unsigned short)(b >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short h) {
+  return __uint_as_float((unsigned int)h << 16);
+}
+__device__ __forceinline__ s4v pack4(f4v v) {
+  s4v r;
+  r.x = (short)to_bf16(v.x);
+  r.y = (short)to_bf16(v.y);
+  r.z = (short)to_bf16(v.z);
+  r.w = (short)to_bf16(v.w);
+  return r;
+}
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ f4v mfma(s4v a, s4v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void wave_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// ------------------------------------------------------------- samples --
+// one thread per LES cell: the block means of u, v, w and their six products
+__global__ void k_cl_filter(Geom g, const double* __restrict__ ux, const double* __restrict__ uy,
+                            const double* __restrict__ uz, const double* __restrict__ yp,
+                            int fx, int fy, int fz, double* __restrict__ fb) {
+  const int nI = g.nx / fx, nJ = g.ny / fy, nK = g.nz / fz;
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int I = (int)(c % nI), J = (int)((c / nI) % nJ), K = (int)(c / ((long long)nI * nJ));
+  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double h = 0.0;
+  for (int b = 0; b < fy; ++b) {
+    const int j = J * fy + b + 1;  // reference row
+    const double dy = yp[j + 1] - yp[j];
+    h += dy;
+    double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int cz = 0; cz < fz; ++cz) {
+      const int k = K * fz + cz, kp = k == g.nz - 1 ? 0 : k + 1;
+      for (int a = 0; a < fx; ++a) {
+        const int i = I * fx + a, ip = i == g.nx - 1 ? 0 : i + 1;
+        const double u = 0.5 * (ux[fidx(g, i, j, k)] + ux[fidx(g, ip, j, k)]);
+        const double v = 0.5 * (uy[fidx(g, i, j, k)] + uy[fidx(g, i, j + 1, k)]);
+        const double w = 0.5 * (uz[fidx(g, i, j, k)] + uz[fidx(g, i, j, kp)]);
+        r[0] += u; r[1] += v; r[2] += w;
+        r[3] += u * u; r[4] += v * v; r[5] += w * w;
+        r[6] += u * v; r[7] += u * w; r[8] += v * w;
+      }
+    }
+    for (int q = 0; q < 9; ++q) s[q] += r[q] * dy;
+  }
+  const double inv = 1.0 / ((double)(fx * fz) * h);
+  for (int q = 0; q < 9; ++q) fb[q * nl + c] = s[q] * inv;
+}
+
+// one thread per LES cell: gradients of the filtered velocity, inputs,
+// targets and the stress scale Delta^2 |S|^2
+__global__ void k_cl_samples(int nI, int nJ, int nK, const double* __restrict__ fb,
+                             const double* __restrict__ yc, const double* __restrict__ dY,
+                             double dX, double dZ, double yly, float* __restrict__ X,
+                             float* __restrict__ Tg, float* __restrict__ scale) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int I = (int)(c % nI), J = (int)((c / nI) % nJ), K = (int)(c / ((long long)nI * nJ));
+  const long long sI = 1, sJ = nI, sK = (long long)nI * nJ;
+  const long long ip = c + (I == nI - 1 ? -(long long)(nI - 1) : 1) * sI;
+  const long long im = c + (I == 0 ? (long long)(nI - 1) : -1) * sI;
+  const long long kp = c + (K == nK - 1 ? -(long long)(nK - 1) : 1) * sK;
+  const long long km = c + (K == 0 ? (long long)(nK - 1) : -1) * sK;
+  long long jp, jm;
+  double dyd;
+  if (J == 0) {
+    jp = c + sJ; jm = c; dyd = yc[1] - yc[0];
+  } else if (J == nJ - 1) {
+    jp = c; jm = c - sJ; dyd = yc[nJ - 1] - yc[nJ - 2];
+  } else {
+    jp = c + sJ; jm = c - sJ; dyd = yc[J + 1] - yc[J - 1];
+  }
+  double gr[3][3];
+  for (int a = 0; a < 3; ++a) {
+    const double* f = fb + a * nl;
+    gr[a][0] = (f[ip] - f[im]) / (2.0 * dX);
+    gr[a][1] = (f[jp] - f[jm]) / dyd;
+    gr[a][2] = (f[kp] - f[km]) / (2.0 * dZ);
+  }
+  double s2 = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double sab = 0.5 * (gr[a][b] + gr[b][a]);
+      s2 += sab * sab;
+    }
+  const double sn = sqrt(2.0 * s2);
+  const double inv = 1.0 / fmax(sn, 1e-30);
+  float* x = X + c * NPAD;
+  int q = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) x[q++] = (float)fmin(fmax(gr[a][b] * inv, -10.0), 10.0);
+  x[9] = (float)(fmin(yc[J], yly - yc[J]) / (0.5 * yly));
+  for (int r = 10; r < NPAD; ++r) x[r] = 0.0f;
+  const double d2 = pow(dX * dY[J] * dZ, 2.0 / 3.0);
+  const double sc = d2 * sn * sn;
+  const double sinv = 1.0 / fmax(sc, 1e-30);
+  const double* u0 = fb;
+  const double* u1 = fb + nl;
+  const double* u2 = fb + 2 * nl;
+  const double tau[6] = {fb[3 * nl + c] - u0[c] * u0[c], fb[4 * nl + c] - u1[c] * u1[c],
+                         fb[5 * nl + c] - u2[c] * u2[c], fb[6 * nl + c] - u0[c] * u1[c],
+                         fb[7 * nl + c] - u0[c] * u2[c], fb[8 * nl + c] - u1[c] * u2[c]};
+  for (int t = 0; t < 6; ++t) Tg[c * NOUT + t] = (float)fmin(fmax(tau[t] * sinv, -10.0), 10.0);
+  scale[c] = (float)sc;
+}
+
+// ------------------------------------------------------------ network --
+// bf16 A fragments from the FP32 masters: lane l of a forward fragment
+// (layer, o, kt) holds W[16kt + 4(l/16) + j][16o + l%16]; of a backward
+// fragment (layer, it, ot) W[16it + l%16][16ot + 4(l/16) + j]
+__global__ void k_cl_pack(const float* __restrict__ w, s4v* __restrict__ fwd,
+                          s4v* __restrict__ bwd) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  const int q = lane >> 4, r = lane & 15;
+  if (f < NFWD) {
+    int l = 3;
+    while (l > 0 && f < FWD_OFF[l]) --l;
+    const int loc = f - FWD_OFF[l], o = loc / KT[l], kt = loc % KT[l];
+    s4v v;
+    for (int j = 0; j < 4; ++j) {
+      const int in = 16 * kt + 4 * q + j, out = 16 * o + r;
+      const float x = (in < IND[l] && out < OUTD[l]) ? w[OFF_W[l] + in * OUTD[l] + out] : 0.0f;
+      v[j] = (short)to_bf16(x);
+    }
+    fwd[f * 64 + lane] = v;
+  } else {
+    const int fb = f - NFWD;
+    int l = 3;
+    while (l > 1 && fb < BWD_OFF[l]) --l;
+    const int loc = fb - BWD_OFF[l], it = loc / OT[l], ot = loc % OT[l];
+    s4v v;
+    for (int j = 0; j < 4; ++j) {
+      const int in = 16 * it + r, out = 16 * ot + 4 * q + j;
+      const float x = out < OUTD[l] ? w[OFF_W[l] + in * OUTD[l] + out] : 0.0f;
+      v[j] = (short)to_bf16(x);
+    }
+    bwd[fb * 64 + lane] = v;
+  }
+}
+
+struct Fwd {
+  s4v h0, h1[4], h2[4], h3[4];
+  f4v z1[4], z2[4], z3[4], y;
+};
+
+// forward of the 16 samples s0 + lane%16 (rows past n are zero inputs)
+__device__ __forceinline__ void mlp_forward(const float* __restrict__ X, long long s, bool ok,
+                                            const s4v* __restrict__ fwd,
+                                            const float* __restrict__ w, Fwd& F) {
+  const int lane = threadIdx.x & 63, q = lane >> 4;
+  f4v xv = {0.f, 0.f, 0.f, 0.f};
+  if (ok) xv = *(const f4v*)(X + s * NPAD + 4 * q);
+  F.h0 = pack4(xv);
+  const f4v zero = {0.f, 0.f, 0.f, 0.f};
+  for (int o = 0; o < 4; ++o) {
+    f4v a = mfma(fwd[(FWD_OFF[0] + o) * 64 + lane], F.h0, zero);
+    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[0] + 16 * o + 4 * q + i];
+    F.z1[o] = a;
+    f4v h;
+    for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
+    F.h1[o] = pack4(h);
+  }
+  for (int o = 0; o < 4; ++o) {
+    f4v a = zero;
+    for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[1] + o * 4 + kt) * 64 + lane], F.h1[kt], a);
+    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[1] + 16 * o + 4 * q + i];
+    F.z2[o] = a;
+    f4v h;
+    for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
+    F.h2[o] = pack4(h);
+  }
+  for (int o = 0; o < 4; ++o) {
+    f4v a = zero;
+    for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[2] + o * 4 + kt) * 64 + lane], F.h2[kt], a);
+    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[2] + 16 * o + 4 * q + i];
+    F.z3[o] = a;
+    f4v h;
+    for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
+    F.h3[o] = pack4(h);
+  }
+  f4v a = zero;
+  for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[3] + kt) * 64 + lane], F.h3[kt], a);
+  for (int i = 0; i < 4; ++i) {
+    const int out = 4 * q + i;
+    a[i] += out < NOUT ? w[OFF_B[3] + out] : 0.0f;
+  }
+  F.y = a;
+}
+
+// one wavefront per workgroup
+__global__ __launch_bounds__(64) void k_cl_forward(const float* __restrict__ X, long long n,
+                                                   const s4v* __restrict__ fwd,
+                                                   const float* __restrict__ w,
+                                                   float* __restrict__ Y) {
+  const int lane = threadIdx.x, q = lane >> 4;
+  const long long ntile = (n + 15) / 16;
+  for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const long long s = t * 16 + (lane & 15);
+    const bool ok = s < n;
+    Fwd F;
+    mlp_forward(X, s, ok, fwd, w, F);
+    for (int i = 0; i < 4; ++i) {
+      const int out = 4 * q + i;
+      if (ok && out < NOUT) Y[s * NOUT + out] = F.y[i];
+    }
+  }
+}
+
+// D-layout tile (row 4q+i, column lane%16) -> A/B layout of its transpose:
+// returns T[row = lane%16][col = 4q + j] as bf16x4
+__device__ __forceinline__ s4v transpose_tile(s4v v, unsigned short (*tr)[20]) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  for (int i = 0; i < 4; ++i) tr[4 * q + i][r] = (unsigned short)v[i];
+  wave_sync();
+  s4v o;
+  for (int j = 0; j < 4; ++j) o[j] = (short)tr[r][4 * q + j];
+  wave_sync();
+  return o;
+}
+
+// sum over the 16 lanes of a quarter (samples), result in every lane
+__device__ __forceinline__ float quarter_sum(float v) {
+  for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Forward + backward over samples base + (idx ? idx[t] : t), t < n; the
+// gradient of the mean square error over ntot samples.  Per workgroup
+// partial sums of every parameter's gradient and of the squared error.
+__global__ __launch_bounds__(64) void k_cl_grad(const float* __restrict__ X,
+                                                const float* __restrict__ Tg, long long base,
+                                                const int* __restrict__ idx, long long n,
+                                                double ntot, const s4v* __restrict__ fwd,
+                                                const s4v* __restrict__ bwd,
+                                                const float* __restrict__ w,
+                                                float* __restrict__ partial,
+                                                double* __restrict__ lpart) {
+  __shared__ float gW[40][64][4];     // dW^T tiles in accumulator layout
+  __shared__ float gb[3 * NH + NOUT];
+  __shared__ unsigned short tr[16][20];
+  const int lane = threadIdx.x, q = lane >> 4, r = lane & 15;
+  for (int e = lane; e < 40 * 256; e += 64) (&gW[0][0][0])[e] = 0.0f;
+  for (int e = lane; e < 3 * NH + NOUT; e += 64) gb[e] = 0.0f;
+  wave_sync();
+  // tile index of the dW^T accumulators: layer l, (ot, it)
+  auto gidx = [](int l, int ot, int it) {
+    const int off[4] = {0, 4, 20, 36};
+    const int nit[4] = {1, 4, 4, 4};
+    return off[l] + ot * nit[l] + it;
+  };
+  const float coef = (float)(2.0 / (ntot * NOUT));
+  double lsum = 0.0;
+  const long long ntile = (n + 15) / 16;
+  const f4v zero = {0.f, 0.f, 0.f, 0.f};
+  for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const long long tl = t * 16 + r;
+    const bool ok = tl < n;
+    const long long s = ok ? base + (idx ? (long long)idx[tl] : tl) : 0;
+    Fwd F;
+    mlp_forward(X, s, ok, fwd, w, F);
+    // output error
+    f4v dz4;
+    for (int i = 0; i < 4; ++i) {
+      const int out = 4 * q + i;
+      float d = 0.0f;
+      if (ok && out < NOUT) d = F.y[i] - Tg[s * NOUT + out];
+      lsum += (double)d * (double)d;
+      dz4[i] = coef * d;
+    }
+    const s4v dz4b = pack4(dz4);
+    // layer 4: dW4^T[out x in] += dz4^T H3, db4, dH3^T = W4 dz4^T
+    {
+      const s4v A = transpose_tile(dz4b, tr);
+      for (int it = 0; it < 4; ++it) {
+        const s4v B = transpose_tile(F.h3[it], tr);
+        const int gi = gidx(3, 0, it);
+        f4v acc = *(f4v*)&gW[gi][lane][0];
+        *(f4v*)&gW[gi][lane][0] = mfma(A, B, acc);
+      }
+      for (int i = 0; i < 4; ++i) {
+        const float v = quarter_sum(dz4[i]);
+        if (r == 0 && 4 * q + i < NOUT) gb[3 * NH + 4 * q + i] += v;
+      }
+    }
+    s4v dzb[4];
+    f4v dz[4];
+    for (int it = 0; it < 4; ++it) {
+      const f4v dh = mfma(bwd[(BWD_OFF[3] + it) * 64 + lane], dz4b, zero);
+      for (int i = 0; i < 4; ++i) dz[it][i] = dh[i] * silu_grad(F.z3[it][i]);
+      dzb[it] = pack4(dz[it]);
+    }
+    // layers 3 and 2: dW^T += dz^T H_in, db, dH_in^T = W dz^T
+    for (int l = 2; l >= 1; --l) {
+      const s4v* hin = l == 2 ? F.h2 : F.h1;
+      s4v At[4];
+      for (int ot = 0; ot < 4; ++ot) At[ot] = transpose_tile(dzb[ot], tr);
+      for (int it = 0; it < 4; ++it) {
+        const s4v B = transpose_tile(hin[it], tr);
+        for (int ot = 0; ot < 4; ++ot) {
+          const int gi = gidx(l, ot, it);
+          f4v acc = *(f4v*)&gW[gi][lane][0];
+          *(f4v*)&gW[gi][lane][0] = mfma(At[ot], B, acc);
+        }
+      }
+      for (int ot = 0; ot < 4; ++ot)
+        for (int i = 0; i < 4; ++i) {
+          const float v = quarter_sum(dz[ot][i]);
+          if (r == 0) gb[l * NH + 16 * ot + 4 * q + i] += v;
+        }
+      const f4v* zin = l == 2 ? F.z2 : F.z1;
+      s4v nb[4];
+      f4v nd[4];
+      for (int it = 0; it < 4; ++it) {
+        f4v dh = zero;
+        for (int ot = 0; ot < 4; ++ot) dh = mfma(bwd[(BWD_OFF[l] + it * 4 + ot) * 64 + lane], dzb[ot], dh);
+        for (int i = 0; i < 4; ++i) nd[it][i] = dh[i] * silu_grad(zin[it][i]);
+        nb[it] = pack4(nd[it]);
+      }
+      for (int it = 0; it < 4; ++it) {
+        dz[it] = nd[it];
+        dzb[it] = nb[it];
+      }
+    }
+    // layer 1: dW1^T += dz1^T X, db1
+    {
+      const s4v B = transpose_tile(F.h0, tr);
+      for (int ot = 0; ot < 4; ++ot) {
+        const s4v A = transpose_tile(dzb[ot], tr);
+        const int gi = gidx(0, ot, 0);
+        f4v acc = *(f4v*)&gW[gi][lane][0];
+        *(f4v*)&gW[gi][lane][0] = mfma(A, B, acc);
+      }
+      for (int ot = 0; ot < 4; ++ot)
+        for (int i = 0; i < 4; ++i) {
+          const float v = quarter_sum(dz[ot][i]);
+          if (r == 0) gb[16 * ot + 4 * q + i] += v;
+        }
+    }
+  }
+  wave_sync();
+  // accumulators -> flat layout: tile (l, ot, it), lane holds
+  // dW^T[out = 16ot + 4q + i][in = 16it + lane%16]
+  float* pp = partial + (long long)blockIdx.x * NPARAM;
+  for (int l = 0; l < 4; ++l)
+    for (int ot = 0; ot < OT[l]; ++ot)
+      for (int it = 0; it < KT[l]; ++it) {
+        const int gi = gidx(l, ot, it);
+        for (int i = 0; i < 4; ++i) {
+          const int out = 16 * ot + 4 * q + i, in = 16 * it + r;
+          if (out < OUTD[l] && in < IND[l]) pp[OFF_W[l] + in * OUTD[l] + out] = gW[gi][lane][i];
+        }
+      }
+  for (int l = 0; l < 4; ++l)
+    for (int o = lane; o < OUTD[l]; o += 64) pp[OFF_B[l] + o] = gb[l * NH + o];
+  for (int o = 32; o > 0; o >>= 1) lsum += __shfl_down(lsum, o, 64);
+  if (lane == 0) lpart[blockIdx.x] = lsum;
+}
+
+// grad[p] = sum of the workgroup partials in workgroup order (FP64 sums);
+// loss = sum of squared errors / (ntot * 6)
+__global__ void k_cl_reduce(const float* __restrict__ partial, int nwg,
+                            const double* __restrict__ lpart, double ntot,
+                            float* __restrict__ grad, float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NPARAM) {
+    double a = 0.0;
+    for (int b = 0; b < nwg; ++b) a += (double)partial[(long long)b * NPARAM + p];
+    grad[p] = (float)a;
+  }
+  if (p == 0 && loss) {
+    double a = 0.0;
+    for (int b = 0; b < nwg; ++b) a += lpart[b];
+    *loss = (float)(a / (ntot * NOUT));
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 masters; rows 10..15 of W1 stay 0
+__global__ void k_cl_adam(float* __restrict__ w, const float* __restrict__ g,
+                          float* __restrict__ m, float* __restrict__ v, float lr, float c1,
+                          float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  if (p < OFF_B[0] && p / NH >= NIN) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gp = g[p];
+  const float mp = b1 * m[p] + (1.0f - b1) * gp;
+  const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  const float mh = mp / c1, vh = vp / c2;
+  w[p] = w[p] - lr * mh / (sqrtf(vh) + eps);
+}
+
+}  // namespace c3d
+
+using namespace c3d;
+
+struct c3d_closure {
+  hipStream_t stream = nullptr;
+  long long cap = 0, n = 0;        // dataset capacity and size (samples)
+  float* X = nullptr;              // cap x 16
+  float* T = nullptr;              // cap x 6
+  float* S = nullptr;              // cap
+  float* w = nullptr;              // NPARAM masters
+  float* m = nullptr;
+  float* v = nullptr;
+  float* g = nullptr;
+  float* loss = nullptr;           // per training step
+  int loss_cap = 0;
+  s4v* fwd = nullptr;
+  s4v* bwd = nullptr;
+  float* partial = nullptr;        // GRAD_WG_MAX x NPARAM
+  double* lpart = nullptr;
+  int* perm = nullptr;
+  long long perm_cap = 0;
+  double* fb = nullptr;            // filter scratch
+  long long fb_cap = 0;
+  float* Y = nullptr;              // predictions
+  long long y_cap = 0;
+  int adam_t = 0;
+};
+
+namespace {
+
+int cl_alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return hip_fail(e, "c3d_closure: hipMalloc");
+  return C3D_OK;
+}
+
+int cl_pack(c3d_closure* cl) {
+  hipLaunchKernelGGL(k_cl_pack, dim3(NFWD + NBWD), dim3(64), 0, cl->stream, cl->w, cl->fwd,
+                     cl->bwd);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int cl_range(c3d_closure* cl, long long first, long long count, const char* what) {
+  if (first < 0 || count < 1 || first + count > cl->n) {
+    set_error(std::string(what) + ": sample range outside the data set");
+    return C3D_ERR_ARG;
+  }
+  return C3D_OK;
+}
+
+// gradient over samples first + (idx ? idx[t] : t), t < count, into cl->g;
+// loss into *dloss (device)
+int cl_grad(c3d_closure* cl, long long first, const int* idx, long long count, float* dloss) {
+  const long long ntile = (count + 15) / 16;
+  const int nwg = (int)std::min<long long>(GRAD_WG_MAX, ntile);
+  hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64), 0, cl->stream, cl->X, cl->T, first, idx,
+                     count, (double)count, cl->fwd, cl->bwd, cl->w, cl->partial, cl->lpart);
+  C3D_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream,
+                     cl->partial, nwg, cl->lpart, (double)count, cl->g, dloss);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+}  // namespace
+
+extern "C" int c3d_closure_destroy(c3d_closure* cl);
+
+extern "C" int c3d_closure_nparam(void) { return NPARAM; }
+
+extern "C" int c3d_closure_create(long long max_samples, c3d_closure** out) {
+  if (!out || max_samples < 16) {
+    set_error("c3d_closure_create: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  *out = nullptr;
+  c3d_closure* cl = new c3d_closure();
+  int rc = C3D_OK;
+  auto fail = [&](int r) {
+    c3d_closure_destroy(cl);
+    return r;
+  };
+  hipError_t e = hipStreamCreate(&cl->stream);
+  if (e != hipSuccess) return fail(hip_fail(e, "c3d_closure_create: hipStreamCreate"));
+  cl->cap = max_samples;
+  if ((rc = cl_alloc((void**)&cl->X, sizeof(float) * NPAD * max_samples))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->T, sizeof(float) * NOUT * max_samples))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->S, sizeof(float) * max_samples))) return fail(rc);
+  for (float** p : {&cl->w, &cl->m, &cl->v, &cl->g})
+    if ((rc = cl_alloc((void**)p, sizeof(float) * NPARAM))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->fwd, sizeof(s4v) * 64 * NFWD))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->bwd, sizeof(s4v) * 64 * NBWD))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->partial, sizeof(float) * NPARAM * GRAD_WG_MAX))) return fail(rc);
+  if ((rc = cl_alloc((void**)&cl->lpart, sizeof(double) * GRAD_WG_MAX))) return fail(rc);
+  for (float* p : {cl->w, cl->m, cl->v, cl->g}) {
+    e = hipMemset(p, 0, sizeof(float) * NPARAM);
+    if (e != hipSuccess) return fail(hip_fail(e, "c3d_closure_create: hipMemset"));
+  }
+  if ((rc = cl_pack(cl))) return fail(rc);
+  e = hipStreamSynchronize(cl->stream);
+  if (e != hipSuccess) return fail(hip_fail(e, "c3d_closure_create"));
+  *out = cl;
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_destroy(c3d_closure* cl) {
+  if (!cl) return C3D_OK;
+  if (cl->stream) hipStreamSynchronize(cl->stream);
+  void* bufs[] = {cl->X, cl->T, cl->S, cl->w, cl->m, cl->v, cl->g, cl->loss, cl->fwd, cl->bwd,
+                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  if (cl->stream) hipStreamDestroy(cl->stream);
+  delete cl;
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_set_params(c3d_closure* cl, const float* host) {
+  if (!cl || !host) return C3D_ERR_ARG;
+  C3D_HIP(hipMemcpyAsync(cl->w, host, sizeof(float) * NPARAM, hipMemcpyHostToDevice, cl->stream));
+  // a new network starts a new optimisation
+  for (float* p : {cl->m, cl->v}) {
+    hipError_t e = hipMemsetAsync(p, 0, sizeof(float) * NPARAM, cl->stream);
+    if (e != hipSuccess) return hip_fail(e, "c3d_closure_set_params");
+  }
+  cl->adam_t = 0;
+  int rc = cl_pack(cl);
+  if (rc) return rc;
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_get_params(c3d_closure* cl, float* host) {
+  if (!cl || !host) return C3D_ERR_ARG;
+  C3D_HIP(hipMemcpyAsync(host, cl->w, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, cl->stream));
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_size(c3d_closure* cl, long long* n) {
+  if (!cl || !n) return C3D_ERR_ARG;
+  *n = cl->n;
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_add_samples(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int fz,
+                                       long long* first) {
+  if (!cl || !c) {
+    set_error("c3d_closure_add_samples: null argument");
+    return C3D_ERR_ARG;
+  }
+  if (c->prec != C3D_FP64 || c->order != 2) {
+    set_error("c3d_closure_add_samples: needs an FP64 2nd-order (staggered) context");
+    return C3D_ERR_ARG;
+  }
+  const Geom& g = c->g;
+  if (fx < 1 || fy < 1 || fz < 1 || g.nx % fx || g.ny % fy || g.nz % fz || g.ny / fy < 2) {
+    set_error("c3d_closure_add_samples: the filter blocks must tile the grid (>= 2 LES rows)");
+    return C3D_ERR_ARG;
+  }
+  const int nI = g.nx / fx, nJ = g.ny / fy, nK = g.nz / fz;
+  const long long nl = (long long)nI * nJ * nK;
+  if (cl->n + nl > cl->cap) {
+    set_error("c3d_closure_add_samples: data set full");
+    return C3D_ERR_ARG;
+  }
+  int rc;
+  if (cl->fb_cap < 9 * nl) {
+    if (cl->fb) hipFree(cl->fb);
+    cl->fb = nullptr;
+    if ((rc = cl_alloc((void**)&cl->fb, sizeof(double) * 9 * nl))) return rc;
+    cl->fb_cap = 9 * nl;
+  }
+  // LES rows: heights and centres from the DNS nodes yp(1..ny+1)
+  const auto& yp = c->met[C3D_M_YP];
+  std::vector<double> host(2 * nJ + g.ny + 3);
+  double* yc = host.data();
+  double* dY = yc + nJ;
+  double* ypd = dY + nJ;
+  for (int J = 0; J < nJ; ++J) {
+    const double lo = yp[J * fy + 1], hi = yp[(J + 1) * fy + 1];
+    dY[J] = hi - lo;
+    yc[J] = 0.5 * (hi + lo);
+  }
+  for (int j = 0; j < g.ny + 3; ++j) ypd[j] = yp[j];
+  double* dev = nullptr;
+  if ((rc = cl_alloc((void**)&dev, sizeof(double) * host.size()))) return rc;
+  // the context's stream must have finished writing the state
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  hipError_t e = hipMemcpyAsync(dev, host.data(), sizeof(double) * host.size(),
+                                hipMemcpyHostToDevice, cl->stream);
+  if (e == hipSuccess) {
+    const int blk = 256;
+    const unsigned grid = (unsigned)((nl + blk - 1) / blk);
+    hipLaunchKernelGGL(k_cl_filter, dim3(grid), dim3(blk), 0, cl->stream, g,
+                       (const double*)c->state[C3D_UX], (const double*)c->state[C3D_UY],
+                       (const double*)c->state[C3D_UZ], (const double*)(dev + 2 * nJ), fx, fy,
+                       fz, cl->fb);
+    hipLaunchKernelGGL(k_cl_samples, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)cl->fb, (const double*)dev, (const double*)(dev + nJ),
+                       fx * c->dx, fz * c->dz, c->p.yly, cl->X + cl->n * NPAD,
+                       cl->T + cl->n * NOUT, cl->S + cl->n);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
+  }
+  hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "c3d_closure_add_samples");
+  if (first) *first = cl->n;
+  cl->n += nl;
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_get_samples(c3d_closure* cl, long long first, long long count,
+                                       float* X, float* T, float* scale) {
+  if (!cl) return C3D_ERR_ARG;
+  int rc = cl_range(cl, first, count, "c3d_closure_get_samples");
+  if (rc) return rc;
+  if (X) C3D_HIP(hipMemcpyAsync(X, cl->X + first * NPAD, sizeof(float) * NPAD * count,
+                                hipMemcpyDeviceToHost, cl->stream));
+  if (T) C3D_HIP(hipMemcpyAsync(T, cl->T + first * NOUT, sizeof(float) * NOUT * count,
+                                hipMemcpyDeviceToHost, cl->stream));
+  if (scale) C3D_HIP(hipMemcpyAsync(scale, cl->S + first, sizeof(float) * count,
+                                    hipMemcpyDeviceToHost, cl->stream));
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_predict(c3d_closure* cl, long long first, long long count, float* Y) {
+  if (!cl || !Y) return C3D_ERR_ARG;
+  int rc = cl_range(cl, first, count, "c3d_closure_predict");
+  if (rc) return rc;
+  if (cl->y_cap < count) {
+    if (cl->Y) hipFree(cl->Y);
+    cl->Y = nullptr;
+    if ((rc = cl_alloc((void**)&cl->Y, sizeof(float) * NOUT * count))) return rc;
+    cl->y_cap = count;
+  }
+  const long long ntile = (count + 15) / 16;
+  const int nwg = (int)std::min<long long>(4096, ntile);
+  hipLaunchKernelGGL(k_cl_forward, dim3(nwg), dim3(64), 0, cl->stream, cl->X + first * NPAD,
+                     count, cl->fwd, cl->w, cl->Y);
+  C3D_HIP(hipGetLastError());
+  C3D_HIP(hipMemcpyAsync(Y, cl->Y, sizeof(float) * NOUT * count, hipMemcpyDeviceToHost,
+                         cl->stream));
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_grad(c3d_closure* cl, long long first, long long count, float* grad,
+                                float* loss) {
+  if (!cl) return C3D_ERR_ARG;
+  int rc = cl_range(cl, first, count, "c3d_closure_grad");
+  if (rc) return rc;
+  if (!cl->loss) {
+    if ((rc = cl_alloc((void**)&cl->loss, sizeof(float) * 1024))) return rc;
+    cl->loss_cap = 1024;
+  }
+  if ((rc = cl_grad(cl, first, nullptr, count, cl->loss))) return rc;
+  if (grad) C3D_HIP(hipMemcpyAsync(grad, cl->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost,
+                                   cl->stream));
+  if (loss) C3D_HIP(hipMemcpyAsync(loss, cl->loss, sizeof(float), hipMemcpyDeviceToHost,
+                                   cl->stream));
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+extern "C" int c3d_closure_train(c3d_closure* cl, long long first, long long count, int batch,
+                                 int steps, float lr, unsigned seed, float* losses) {
+  if (!cl || batch < 16 || steps < 1 || !(lr > 0)) {
+    set_error("c3d_closure_train: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  int rc = cl_range(cl, first, count, "c3d_closure_train");
+  if (rc) return rc;
+  if (batch > count) batch = (int)count;
+  if (cl->loss_cap < steps) {
+    if (cl->loss) hipFree(cl->loss);
+    cl->loss = nullptr;
+    if ((rc = cl_alloc((void**)&cl->loss, sizeof(float) * steps))) return rc;
+    cl->loss_cap = steps;
+  }
+  if (cl->perm_cap < count) {
+    if (cl->perm) hipFree(cl->perm);
+    cl->perm = nullptr;
+    if ((rc = cl_alloc((void**)&cl->perm, sizeof(int) * count))) return rc;
+    cl->perm_cap = count;
+  }
+  std::vector<int> perm(count);
+  std::mt19937 gen(seed);
+  long long pos = count;  // reshuffle on the first step
+  for (int st = 0; st < steps; ++st) {
+    if (pos + batch > count) {
+      for (long long t = 0; t < count; ++t) perm[t] = (int)t;
+      std::shuffle(perm.begin(), perm.end(), gen);
+      // the previous epoch's steps read the old order: finish them first
+      C3D_HIP(hipStreamSynchronize(cl->stream));
+      C3D_HIP(hipMemcpyAsync(cl->perm, perm.data(), sizeof(int) * count,
+                             hipMemcpyHostToDevice, cl->stream));
+      pos = 0;
+    }
+    if ((rc = cl_grad(cl, first, cl->perm + pos, batch, cl->loss + st))) return rc;
+    ++cl->adam_t;
+    const float c1 = 1.0f - std::pow(0.9f, (float)cl->adam_t);
+    const float c2 = 1.0f - std::pow(0.999f, (float)cl->adam_t);
+    hipLaunchKernelGGL(k_cl_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream, cl->w,
+                       cl->g, cl->m, cl->v, lr, c1, c2);
+    C3D_HIP(hipGetLastError());
+    if ((rc = cl_pack(cl))) return rc;
+    pos += batch;
+  }
+  if (losses)
+    C3D_HIP(hipMemcpyAsync(losses, cl->loss, sizeof(float) * steps, hipMemcpyDeviceToHost,
+                           cl->stream));
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+// one Adam step with a given gradient (host), for checking the optimiser
+extern "C" int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float lr) {
+  if (!cl || !grad || !(lr > 0)) return C3D_ERR_ARG;
+  C3D_HIP(hipMemcpyAsync(cl->g, grad, sizeof(float) * NPARAM, hipMemcpyHostToDevice, cl->stream));
+  ++cl->adam_t;
+  const float c1 = 1.0f - std::pow(0.9f, (float)cl->adam_t);
+  const float c2 = 1.0f - std::pow(0.999f, (float)cl->adam_t);
+  hipLaunchKernelGGL(k_cl_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream, cl->w,
+                     cl->g, cl->m, cl->v, lr, c1, c2);
+  C3D_HIP(hipGetLastError());
+  int rc = cl_pack(cl);
+  if (rc) return rc;
+  C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}

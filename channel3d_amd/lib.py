@@ -48,6 +48,10 @@ EXPORTS = (
     "c3d_write_vgrad", "c3d_set_les", "c3d_Clc_SGS_Vis",
     "c3d_clcSpectra1D", "c3d_spectra_get", "c3d_spectra_reset", "c3d_write_spectra",
     "c3d_clcSpectra2D", "c3d_spectra2d_get", "c3d_write_spectra2d", "c3d_set_graph",
+    "c3d_closure_nparam", "c3d_closure_create", "c3d_closure_destroy", "c3d_closure_set_params",
+    "c3d_closure_get_params", "c3d_closure_size", "c3d_closure_add_samples",
+    "c3d_closure_get_samples", "c3d_closure_predict", "c3d_closure_grad", "c3d_closure_train",
+    "c3d_closure_adam_step",
 )
 
 
@@ -77,6 +81,7 @@ _lib = None
 _VP = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
 
 
 def load(path: str | None = None):
@@ -146,6 +151,19 @@ def load(path: str | None = None):
         "c3d_append_prgrad": ([_VP, ctypes.c_char_p, I], I),
         "c3d_set_ppe_refine": ([_VP, I], I),
         "c3d_set_graph": ([_VP, I], I),
+        "c3d_closure_nparam": ([], I),
+        "c3d_closure_create": ([ctypes.c_longlong, ctypes.POINTER(_VP)], I),
+        "c3d_closure_destroy": ([_VP], I),
+        "c3d_closure_set_params": ([_VP, _FP], I),
+        "c3d_closure_get_params": ([_VP, _FP], I),
+        "c3d_closure_size": ([_VP, ctypes.POINTER(ctypes.c_longlong)], I),
+        "c3d_closure_add_samples": ([_VP, _VP, I, I, I, ctypes.POINTER(ctypes.c_longlong)], I),
+        "c3d_closure_get_samples": ([_VP, ctypes.c_longlong, ctypes.c_longlong, _FP, _FP, _FP], I),
+        "c3d_closure_predict": ([_VP, ctypes.c_longlong, ctypes.c_longlong, _FP], I),
+        "c3d_closure_grad": ([_VP, ctypes.c_longlong, ctypes.c_longlong, _FP, _FP], I),
+        "c3d_closure_train": ([_VP, ctypes.c_longlong, ctypes.c_longlong, I, I, ctypes.c_float,
+                               ctypes.c_uint, _FP], I),
+        "c3d_closure_adam_step": ([_VP, _FP, ctypes.c_float], I),
         "c3d_clcStat": ([_VP] * 5, I),
         "c3d_stat_set_grad": ([_VP, I], I),
         "c3d_stat_get_grad": ([_VP, _DP], I),

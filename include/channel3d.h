@@ -355,6 +355,40 @@ int c3d_stat_reset(c3d_ctx* ctx);
 int c3d_write_stat(c3d_ctx* ctx, const char* path, int itime, int ivstats);
 int c3d_append_prgrad(c3d_ctx* ctx, const char* path, int itime);
 
+/* ------------------------------------------- learned subgrid closure -- */
+/* No counterpart in the reference (its LES models are WALE and MTS,
+ * m_LESModels.f90:53-58).  A data set of LES samples filtered from DNS
+ * states and a 10-64-64-64-6 SiLU network trained on them with bf16 MFMA
+ * (FP32 accumulation, FP32 masters, Adam).  Samples: the context's velocity
+ * (FP64, 2nd order) centred in its cells and box-filtered over fx x fy x fz
+ * cell blocks; inputs X (16 floats: the 9 filtered gradients over |S|, the
+ * wall distance over the half height, zeros), targets T (6 floats:
+ * tau_11 22 33 12 13 23 over Delta^2 |S|^2), scale Delta^2 |S|^2, LES cells
+ * x fastest.  Parameters (c3d_closure_nparam floats): W1 (16 x 64, rows 10..
+ * 15 zero), b1, W2 (64 x 64), b2, W3, b3, W4 (64 x 6), b4, W[in][out].
+ * All calls synchronous; the closure has its own stream. */
+typedef struct c3d_closure c3d_closure;
+int c3d_closure_nparam(void);
+int c3d_closure_create(long long max_samples, c3d_closure** out);
+int c3d_closure_destroy(c3d_closure* cl);
+int c3d_closure_set_params(c3d_closure* cl, const float* params);
+int c3d_closure_get_params(c3d_closure* cl, float* params);
+int c3d_closure_size(c3d_closure* cl, long long* n);
+int c3d_closure_add_samples(c3d_closure* cl, c3d_ctx* dns, int fx, int fy, int fz,
+                            long long* first);
+int c3d_closure_get_samples(c3d_closure* cl, long long first, long long count, float* X,
+                            float* T, float* scale);
+/* network outputs (normalised stresses, 6 per sample) */
+int c3d_closure_predict(c3d_closure* cl, long long first, long long count, float* Y);
+/* gradient of the mean square error over a sample range, and the loss */
+int c3d_closure_grad(c3d_closure* cl, long long first, long long count, float* grad,
+                     float* loss);
+/* Adam over shuffled mini-batches of a sample range; loss per step */
+int c3d_closure_train(c3d_closure* cl, long long first, long long count, int batch, int steps,
+                      float lr, unsigned seed, float* losses);
+/* one Adam step with a given gradient */
+int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float lr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,172 @@
+"""The learned subgrid closure on the GPU (c3d_closure_*) against its
+oracle (oracle/closure.py): samples filtered from a DNS state, the bf16
+MFMA network's outputs and gradients, the Adam step, and a training run
+whose network beats the dynamic Smagorinsky model a priori on held-out
+filtered DNS.  The network computes with bf16 operands: a 1-ulp bf16
+difference of an activation (0.4 %) moves later values, so the network
+bars are a small maximum relative error plus a tight bound on the 99th
+percentile, both written in each test.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DNS = dict(nxc=64, nyc=32, nzc=48, xlx=2 * np.pi, yly=2.0, zlz=np.pi, xnu=1.0 / 2000.0,
+           dt=0.004, gamma=1.5, ubulk=1.0)
+
+
+def _dns(case, seed, amp=0.3):
+    from channel3d_amd import Channel, synth, lib as L
+    g = Channel(case)
+    ux, uy, uz = synth.laminar_noise(g.nxc, g.nyc, g.nzc, g.yp, yly=case["yly"], ubulk=1.0,
+                                     amp=amp, seed=seed)
+    for f, v in ((L.UX, ux), (L.UY, uy), (L.UZ, uz)):
+        g.upload(f, v)
+    g.project()
+    return g
+
+
+def _oracle_samples(g, case):
+    from oracle import closure as C
+    from channel3d_amd import lib as L
+    ux, uy, uz = (g.download(f) for f in (L.UX, L.UY, L.UZ))
+    X, T, S = C.samples(ux, uy, uz, g.yp, case["xlx"] / case["nxc"], case["zlz"] / case["nzc"],
+                        case["yly"])
+    return X, T, S, (ux, uy, uz)
+
+
+def test_samples_match_oracle():
+    from channel3d_amd.closure import Closure
+    g = _dns(DNS, seed=4)
+    for _ in range(2):
+        g.step()
+    cl = Closure(1 << 16)
+    first, n = cl.add_samples(g)
+    assert first == 0 and n == (64 // 4) * (32 // 2) * (48 // 4)
+    X, T, S = cl.samples(first, n)
+    Xo, To, So, _ = _oracle_samples(g, DNS)
+    assert np.allclose(X, Xo, rtol=1e-6, atol=1e-6 * np.abs(Xo).max())
+    assert np.allclose(T, To, rtol=1e-6, atol=1e-6 * np.abs(To).max())
+    assert np.allclose(S, So, rtol=1e-6, atol=0)
+    # a second state appends after the first
+    g.step()
+    first2, n2 = cl.add_samples(g)
+    assert first2 == n and n2 == n and cl.size() == 2 * n
+    cl.close()
+    g.close()
+
+
+def _net_data(seed=4):
+    from channel3d_amd.closure import Closure
+    g = _dns(DNS, seed=seed)
+    g.step()
+    cl = Closure(1 << 16)
+    first, n = cl.add_samples(g)
+    g.close()
+    return cl, first, n
+
+
+def test_forward_matches_oracle():
+    from oracle import closure as C
+    cl, first, n = _net_data()
+    p = C.init_params(3)
+    cl.set_params(p)
+    X, _, _ = cl.samples(first, n)
+    Y = cl.predict(first, n)
+    Yo, _ = C.forward(p, X, emulate=True)
+    scale = np.abs(Yo).max()
+    err = np.abs(Y - Yo) / scale
+    # bars: maximum 1e-2, 99th percentile 1e-5 of the largest output
+    assert err.max() < 1e-2 and np.quantile(err, 0.99) < 1e-5, (err.max(), np.quantile(err, 0.99))
+    # and the bf16 network is the FP64 network to bf16 accuracy
+    Y64, _ = C.forward({k: v.astype(np.float64) for k, v in p.items()}, X.astype(np.float64),
+                       emulate=False)
+    assert np.abs(Y - Y64).max() / np.abs(Y64).max() < 3e-2
+    assert cl.get_params()["W2"].tobytes() == p["W2"].tobytes()
+    cl.close()
+
+
+def test_gradient_matches_oracle():
+    from oracle import closure as C
+    cl, first, n = _net_data(seed=5)
+    p = C.init_params(8)
+    cl.set_params(p)
+    X, T, _ = cl.samples(first, n)
+    g, loss = cl.grad(first, n)
+    go = C.backward(p, X, T, emulate=True)
+    lo = C.loss(p, X, T, emulate=True)
+    assert abs(loss - lo) <= 1e-5 * lo
+    # each block's gradient: relative 2-norm error below 1e-2 (bf16 operands
+    # of the products), and the GPU's W1 rows 10..15 stay exactly zero
+    for k in go:
+        e = np.linalg.norm(g[k] - go[k]) / np.linalg.norm(go[k])
+        assert e < 1e-2, (k, e)
+    assert not g["W1"][10:].any()
+    cl.close()
+
+
+def test_adam_step_matches_oracle():
+    from oracle import closure as C
+    cl, first, n = _net_data(seed=6)
+    p = C.init_params(1)
+    cl.set_params(p)
+    rng = np.random.default_rng(2)
+    state = {}
+    q = {k: v.copy() for k, v in p.items()}
+    for _ in range(3):
+        grads = {k: rng.standard_normal(v.shape).astype(np.float32) * 1e-2 for k, v in p.items()}
+        grads["W1"][10:] = 0.0
+        cl.adam_step(grads, 1e-3)
+        C.adam(q, grads, state, lr=1e-3)
+    got = cl.get_params()
+    for k in q:
+        assert np.allclose(got[k], q[k], rtol=1e-5, atol=1e-7), k
+    cl.close()
+
+
+def test_training_beats_dynamic_smagorinsky_a_priori():
+    """train on four DNS snapshots, judge on two later ones: correlation of
+    the predicted with the filtered-DNS stresses (tau_12 and the deviatoric
+    tensor) against the dynamic Smagorinsky model's"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from channel3d_amd import lib as L
+    g = _dns(DNS, seed=9)
+    cl = Closure(1 << 18)
+    snaps = []
+    for s in range(6):
+        for _ in range(3):
+            g.step()
+        first, n = cl.add_samples(g)
+        snaps.append((first, n, [g.download(f) for f in (L.UX, L.UY, L.UZ)]))
+    g.close()
+    cl.set_params(C.init_params(0))
+    ntrain = snaps[4][0]
+    losses = cl.train(0, ntrain, batch=2048, steps=600, lr=2e-3, seed=1)
+    assert losses[-50:].mean() < 0.5 * losses[:10].mean()
+    dx, dz = DNS["xlx"] / DNS["nxc"], DNS["zlz"] / DNS["nzc"]
+    nI, nJ, nK = DNS["nxc"] // 4, DNS["nyc"] // 2, DNS["nzc"] // 4
+    dY, yc = C.les_grid(_yp(), 2)
+    true, mlp, dsm = [], [], []
+    for first, n, (ux, uy, uz) in snaps[4:]:
+        ub, tau = C.filtered(ux, uy, uz, _yp(), 4, 2, 4)
+        _, _, S = cl.samples(first, n)
+        Y = cl.predict(first, n)
+        that = [(Y[:, q] * S).reshape((nI, nJ, nK), order="F") for q in range(6)]
+        tdsm, _ = C.dynamic_smagorinsky(ub, 4 * dx, dY, 4 * dz, yc)
+        true.append(C.deviatoric(tau))
+        mlp.append(C.deviatoric(that))
+        dsm.append(tdsm)
+    r12 = [C.correlation([t[3] for t in m], [t[3] for t in true]) for m in (mlp, dsm)]
+    rdev = [C.correlation([x for t in m for x in t], [x for t in true for x in t])
+            for m in (mlp, dsm)]
+    print(f"a priori: tau_12 corr mlp {r12[0]:.3f} dsm {r12[1]:.3f}; "
+          f"deviatoric corr mlp {rdev[0]:.3f} dsm {rdev[1]:.3f}")
+    assert r12[0] > r12[1] and rdev[0] > rdev[1]
+    cl.close()
+
+
+def _yp():
+    from channel3d_amd.channel import channel_yp
+    return channel_yp(DNS["nyc"], DNS["yly"], DNS["gamma"])
