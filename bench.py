@@ -148,6 +148,93 @@ def run_fourth_order(steps, warmup):
                 divmax_after_step=d["divmax2"], stages=stages)
 
 
+CLOSURE_DNS = dict(nxc=384, nyc=192, nzc=288, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
+                   xnu=1.4428e-4, dt=0.002, gamma=1.5, ubulk=1.0)
+# MFMA multiply-adds per sample of one training step (padded tiles): forward
+# 16x64 + 2 x 64x64 + 64x16, input gradients 2 x 64x64 + 64x16, weight
+# gradients as the forward
+CLOSURE_MACS = (1024 + 8192 + 1024) + (8192 + 1024) + (1024 + 8192 + 1024)
+
+
+def _corr(a, b):
+    a = np.concatenate([np.ravel(x) for x in a])
+    b = np.concatenate([np.ravel(x) for x in b])
+    a = a - a.mean()
+    b = b - b.mean()
+    return float((a * b).sum() / np.sqrt((a * a).sum() * (b * b).sum()))
+
+
+def run_closure(train_steps, batch):
+    """the learned closure at the 384x192x288 DNS / 96x96x72 LES shape: six
+    DNS states 20 steps apart from smooth random modes, samples filtered on
+    the GPU, training on four (bf16 MFMA, Adam), a-priori correlation of the
+    predicted with the filtered-DNS stress on the other two"""
+    from channel3d_amd import Channel, synth, lib as L
+    from channel3d_amd.closure import Closure, unflatten
+    c = CLOSURE_DNS
+    g = Channel(c)
+    ux, uy, uz = synth.smooth_modes(c["nxc"], c["nyc"], c["nzc"], g.yp, c["xlx"], c["yly"],
+                                    c["zlz"], seed=0)
+    for f, v in ((L.UX, ux), (L.UY, uy), (L.UZ, uz)):
+        g.upload(f, v)
+    del ux, uy, uz
+    g.project()
+    nI, nJ, nK = c["nxc"] // 4, c["nyc"] // 2, c["nzc"] // 4
+    cl = Closure(6 * nI * nJ * nK)
+    held = []
+    t_samples = 0.0
+    for s in range(6):
+        for _ in range(20):
+            g.step()
+        g.sync()
+        t0 = time.perf_counter()
+        first, n = cl.add_samples(g)
+        t_samples += time.perf_counter() - t0
+        if s >= 4:
+            held.append((first, n))
+    g.close()
+    ntrain = held[0][0]
+    # initial weights: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)), seeded
+    rng = np.random.default_rng(0)
+    v = []
+    for fan, shape in ((10, (16, 64)), (10, (64,)), (64, (64, 64)), (64, (64,)), (64, (64, 64)),
+                       (64, (64,)), (64, (64, 6)), (64, (6,))):
+        w = rng.uniform(-1 / np.sqrt(fan), 1 / np.sqrt(fan), size=shape).astype(np.float32)
+        v.append(w)
+    v[0][10:] = 0.0
+    cl.set_params(unflatten(np.concatenate([w.ravel() for w in v])))
+    cl.train(0, ntrain, batch=batch, steps=5, lr=1e-3, seed=0)   # warm-up
+    t0 = time.perf_counter()
+    losses = cl.train(0, ntrain, batch=batch, steps=train_steps, lr=1e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    sps = train_steps * batch / t_train
+    res = {"mlp": [], "true": []}
+    for first, n in held:
+        _, T, S = cl.samples(first, n)
+        Y = cl.predict(first, n)
+        for key, arr in (("mlp", Y), ("true", T)):
+            t = [arr[:, q].astype(np.float64) * S for q in range(6)]
+            tr = (t[0] + t[1] + t[2]) / 3.0
+            res[key].append([t[0] - tr, t[1] - tr, t[2] - tr, t[3], t[4], t[5]])
+    r12 = _corr([d[3] for d in res["mlp"]], [d[3] for d in res["true"]])
+    rdev = _corr([x for d in res["mlp"] for x in d], [x for d in res["true"] for x in d])
+    cl.close()
+    return dict(workload=f"learned SGS closure: DNS {c['nxc']}x{c['nyc']}x{c['nzc']} "
+                         f"(Re_tau=395) box-filtered 4x2x4 to LES {nI}x{nJ}x{nK}; MLP "
+                         "10-64-64-64-6 SiLU, bf16 MFMA, FP32 accumulation, Adam",
+                samples_train=int(ntrain), samples_held_out=int(sum(n for _, n in held)),
+                sample_extraction_ms_per_state=round(1e3 * t_samples / 6, 3),
+                batch=batch, train_steps=train_steps,
+                train_samples_per_s=round(sps, 1),
+                train_mfma_tflops=round(2 * CLOSURE_MACS * sps / 1e12, 3),
+                loss_first=float(losses[:10].mean()), loss_last=float(losses[-10:].mean()),
+                a_priori_corr_tau12_held_out=round(r12, 4),
+                a_priori_corr_deviatoric_held_out=round(rdev, 4),
+                note="targets: tau / (Delta^2 |S|^2) of the filtered DNS; inputs and targets "
+                     "clipped to +-10; fields: laminar flow plus smooth random modes, "
+                     "projected, 20-120 steps evolved")
+
+
 def run_les(case, steps, warmup, label):
     """the LES step (WALE or MTS, partially implicit scheme) on the same GPU:
     steps/s and the per-stage split (inputs resident, HIP events on the
@@ -286,6 +373,8 @@ def main(argv=None):
                     help="skip the 4th-order 192x128x160 line reported beside the main one")
     ap.add_argument("--no-les", action="store_true",
                     help="skip the LES step measurements (96x96x72 and 384x192x288)")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the learned-closure training measurement")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32 + refinement pressure-solve sweep")
     ap.add_argument("--probe", choices=["copy"], default=None,
@@ -428,6 +517,12 @@ def main(argv=None):
             log(f"LES step {key}")
             les[key] = run_les(case_l, max(a.steps, 20), max(a.warmup, 2), lbl)
             log(f"{key}: {les[key]['ms_per_step']} ms/step")
+    closure = None
+    if not a.no_closure and not a.small:
+        log("learned closure 384x192x288 -> 96x96x72")
+        closure = run_closure(400, 16384)
+        log(f"closure: {closure['train_samples_per_s']} samples/s, a-priori tau_12 corr "
+            f"{closure['a_priori_corr_tau12_held_out']}")
 
     out = {
         "metric": METRIC,
@@ -455,6 +550,7 @@ def main(argv=None):
         "divmax_after_step": d1["divmax2"],
         "fourth_order": fourth,
         "les": les,
+        "closure": closure,
         "precision_sweep": precision,
     }
     print(json.dumps(out), flush=True)

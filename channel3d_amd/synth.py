@@ -83,3 +83,31 @@ def random_halo(nxc, nyc, nzc, seed, field, amp=1.0):
 
 def random_interior(nxc, nyc, nzc, seed, field, amp=1.0):
     return amp * uniform_pm1(seed, field, (nxc, nyc, nzc))
+
+
+def smooth_modes(nxc, nyc, nzc, yp, xlx, yly, zlz, seed=0, nmodes=48, kmax=6, amp=0.25,
+                 ubulk=1.0):
+    """laminar profile plus random smooth modes on the staggered grid: x and
+    z wavenumbers 1..kmax, wall-normal sines sin(m pi y / yly), m = 1..4,
+    vanishing at the walls, amplitudes amp N(0,1)/sqrt(nmodes); halo arrays
+    (halo layers zero), to be projected.  A field with resolved structure at
+    and above a few cells' scale (the learned closure's training data)."""
+    rng = np.random.default_rng(seed)
+    xf = np.arange(nxc) * xlx / nxc
+    xc = xf + 0.5 * xlx / nxc
+    zf = np.arange(nzc) * zlz / nzc
+    zc = zf + 0.5 * zlz / nzc
+    yn = np.asarray(yp[1:nyc + 1])
+    yc = 0.5 * (np.asarray(yp[1:nyc + 1]) + np.asarray(yp[2:nyc + 2]))
+    ux, uy, uz = laminar_noise(nxc, nyc, nzc, yp, yly=yly, ubulk=ubulk, amp=0.0, seed=seed)
+    for comp, (x, y, z) in zip((ux, uy, uz), ((xf, yc, zc), (xc, yn, zc), (xc, yc, zf))):
+        f = np.zeros((nxc, nyc, nzc))
+        for _ in range(nmodes):
+            kx, kz, m = rng.integers(1, kmax + 1), rng.integers(1, kmax + 1), rng.integers(1, 5)
+            a = amp * ubulk * rng.standard_normal() / np.sqrt(nmodes)
+            px, pz = rng.uniform(0, 2 * np.pi, 2)
+            f += a * (np.sin(2 * np.pi * kx * x / xlx + px)[:, None, None]
+                      * np.sin(m * np.pi * y / yly)[None, :, None]
+                      * np.sin(2 * np.pi * kz * z / zlz + pz)[None, None, :])
+        comp[1:nxc + 1, 1:nyc + 1, 1:nzc + 1] += f
+    return ux, uy, uz

@@ -27,34 +27,14 @@ def _dns(case, seed, amp=0.3):
     return g
 
 
-def _smooth_dns(case, seed, nmodes=48, kmax=6, amp=0.25):
-    """laminar profile plus random smooth modes (x, z wavenumbers <= kmax,
-    wall-normal sine profiles vanishing at the walls), projected: resolved
-    structure at and above the filter scale, whose subgrid stress the
-    filtered gradients determine (white noise would leave nothing to learn)"""
+def _smooth_dns(case, seed):
+    """laminar flow plus random smooth modes, projected: resolved structure
+    at and above the filter scale, whose subgrid stress the filtered
+    gradients determine (white noise would leave nothing to learn)"""
     from channel3d_amd import Channel, synth, lib as L
     g = Channel(case)
-    nx, ny, nz = g.nxc, g.nyc, g.nzc
-    rng = np.random.default_rng(seed)
-    Lx, Ly, Lz = case["xlx"], case["yly"], case["zlz"]
-    xf = np.arange(nx) * Lx / nx
-    xc = xf + 0.5 * Lx / nx
-    zf = np.arange(nz) * Lz / nz
-    zc = zf + 0.5 * Lz / nz
-    yn = np.asarray(g.yp[1:ny + 1])
-    yc = 0.5 * (g.yp[1:ny + 1] + g.yp[2:ny + 2])
-    grids = ((xf, yc, zc), (xc, yn, zc), (xc, yc, zf))
-    ux, uy, uz = synth.laminar_noise(nx, ny, nz, g.yp, yly=Ly, ubulk=1.0, amp=0.0, seed=seed)
-    for comp, (x, y, z) in zip((ux, uy, uz), grids):
-        f = np.zeros((nx, ny, nz))
-        for _ in range(nmodes):
-            kx, kz, m = rng.integers(1, kmax + 1), rng.integers(1, kmax + 1), rng.integers(1, 5)
-            a = amp * rng.standard_normal() / np.sqrt(nmodes)
-            px, pz = rng.uniform(0, 2 * np.pi, 2)
-            f += a * (np.sin(2 * np.pi * kx * x / Lx + px)[:, None, None]
-                      * np.sin(m * np.pi * y / Ly)[None, :, None]
-                      * np.sin(2 * np.pi * kz * z / Lz + pz)[None, None, :])
-        comp[1:nx + 1, 1:ny + 1, 1:nz + 1] += f
+    ux, uy, uz = synth.smooth_modes(g.nxc, g.nyc, g.nzc, g.yp, case["xlx"], case["yly"],
+                                    case["zlz"], seed=seed)
     for f, v in ((L.UX, ux), (L.UY, uy), (L.UZ, uz)):
         g.upload(f, v)
     g.project()
