@@ -105,3 +105,19 @@ class Closure:
         v = np.ascontiguousarray(flatten(grads))
         L.check(self.lib.c3d_closure_adam_step(self._h, _fp(v), float(lr)),
                 "c3d_closure_adam_step")
+
+    def dsm(self, channel, fx=4, fy=2, fz=4):
+        """dynamic Smagorinsky on the LES grid of the channel's state:
+        (model stress (6, nI, nJ, nK), filtered-DNS stress, Cs^2 per row)"""
+        nI, nJ, nK = channel.nxc // fx, channel.nyc // fy, channel.nzc // fz
+        n = nI * nJ * nK
+        tm = np.zeros(6 * n)
+        tt = np.zeros(6 * n)
+        cs2 = np.zeros(nJ)
+        L.check(self.lib.c3d_closure_dsm(self._h, channel._h, int(fx), int(fy), int(fz),
+                                         tm.ctypes.data_as(L._DP), tt.ctypes.data_as(L._DP),
+                                         cs2.ctypes.data_as(L._DP)), "c3d_closure_dsm")
+        def fields(a):
+            return np.stack([a[q * n:(q + 1) * n].reshape((nI, nJ, nK), order="F")
+                             for q in range(6)])
+        return fields(tm), fields(tt), cs2

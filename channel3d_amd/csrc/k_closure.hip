@@ -455,6 +455,172 @@ __global__ void k_cl_adam(float* __restrict__ w, const float* __restrict__ g,
   w[p] = w[p] - lr * mh / (sqrtf(vh) + eps);
 }
 
+// ------------------------------------------------ dynamic Smagorinsky --
+// Germano, Piomelli, Moin & Cabot (1991) with Lilly's (1992) least squares
+// over x-z planes, on the LES grid of the samples: test filter (1/4, 1/2,
+// 1/4) in x then z, width ratio 2.  The fallback model and the closure's
+// a-priori comparator.
+struct LesNb {
+  long long ip, im, kp, km, jp, jm;
+  double dyd;
+};
+__device__ __forceinline__ LesNb les_nb(long long c, int I, int J, int K, int nI, int nJ, int nK,
+                                        const double* yc) {
+  LesNb n;
+  const long long sJ = nI, sK = (long long)nI * nJ;
+  n.ip = c + (I == nI - 1 ? -(long long)(nI - 1) : 1);
+  n.im = c + (I == 0 ? (long long)(nI - 1) : -1);
+  n.kp = c + (K == nK - 1 ? -(long long)(nK - 1) : 1) * sK;
+  n.km = c + (K == 0 ? (long long)(nK - 1) : -1) * sK;
+  if (J == 0) {
+    n.jp = c + sJ; n.jm = c; n.dyd = yc[1] - yc[0];
+  } else if (J == nJ - 1) {
+    n.jp = c; n.jm = c - sJ; n.dyd = yc[nJ - 1] - yc[nJ - 2];
+  } else {
+    n.jp = c + sJ; n.jm = c - sJ; n.dyd = yc[J + 1] - yc[J - 1];
+  }
+  return n;
+}
+__device__ __forceinline__ void les_grad(const double* const* f, const LesNb& n, double dX,
+                                         double dZ, double (&S)[6], double& sn) {
+  double gr[3][3];
+  for (int a = 0; a < 3; ++a) {
+    gr[a][0] = (f[a][n.ip] - f[a][n.im]) / (2.0 * dX);
+    gr[a][1] = (f[a][n.jp] - f[a][n.jm]) / n.dyd;
+    gr[a][2] = (f[a][n.kp] - f[a][n.km]) / (2.0 * dZ);
+  }
+  double s2 = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double sab = 0.5 * (gr[a][b] + gr[b][a]);
+      s2 += sab * sab;
+    }
+  sn = sqrt(2.0 * s2);
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) S[q] = 0.5 * (gr[pa[q]][pb[q]] + gr[pb[q]][pa[q]]);
+}
+// (1/4, 1/2, 1/4) in x, then in z, of f at cell (I, K) of row J
+__device__ __forceinline__ double tfilt(const double* f, int I, int J, int K, int nI, int nJ,
+                                        int nK) {
+  auto at = [&](int i, int k) {
+    i = i < 0 ? i + nI : (i >= nI ? i - nI : i);
+    k = k < 0 ? k + nK : (k >= nK ? k - nK : k);
+    return f[i + (long long)nI * (J + (long long)nJ * k)];
+  };
+  auto xf = [&](int k) { return 0.25 * at(I - 1, k) + 0.5 * at(I, k) + 0.25 * at(I + 1, k); };
+  return 0.25 * xf(K - 1) + 0.5 * xf(K) + 0.25 * xf(K + 1);
+}
+
+// pass 1: S (6), |S| and |S| S (6) per LES cell; test-filtered velocity (3)
+// and products (6).  w: 22 fields of nl doubles.
+__global__ void k_dsm_a(int nI, int nJ, int nK, const double* __restrict__ fb,
+                        const double* __restrict__ yc, double dX, double dZ,
+                        double* __restrict__ w) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int I = (int)(c % nI), J = (int)((c / nI) % nJ), K = (int)(c / ((long long)nI * nJ));
+  const LesNb n = les_nb(c, I, J, K, nI, nJ, nK, yc);
+  const double* u[3] = {fb, fb + nl, fb + 2 * nl};
+  double S[6], sn;
+  les_grad(u, n, dX, dZ, S, sn);
+  for (int q = 0; q < 6; ++q) {
+    w[q * nl + c] = S[q];
+    w[(7 + q) * nl + c] = sn * S[q];
+  }
+  w[6 * nl + c] = sn;
+}
+// pass 2 (after pass 1 everywhere): test-filtered velocity and products
+__global__ void k_dsm_t(int nI, int nJ, int nK, const double* __restrict__ fb,
+                        double* __restrict__ w) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int I = (int)(c % nI), J = (int)((c / nI) % nJ), K = (int)(c / ((long long)nI * nJ));
+  for (int a = 0; a < 3; ++a) w[(13 + a) * nl + c] = tfilt(fb + a * nl, I, J, K, nI, nJ, nK);
+}
+// products of the filtered velocity, then their test filter (two passes:
+// the products first into w[16..21], then filtered in place by k_dsm_tp)
+__global__ void k_dsm_p(int nI, int nJ, int nK, const double* __restrict__ fb,
+                        double* __restrict__ pr) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) pr[q * nl + c] = fb[pa[q] * nl + c] * fb[pb[q] * nl + c];
+}
+// pass 3: L^d : M and M : M per cell (weights 1 on the diagonal, 2 off)
+__global__ void k_dsm_b(int nI, int nJ, int nK, const double* __restrict__ w,
+                        const double* __restrict__ pr, const double* __restrict__ yc,
+                        const double* __restrict__ dY, double dX, double dZ,
+                        double* __restrict__ lm, double* __restrict__ mm) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int I = (int)(c % nI), J = (int)((c / nI) % nJ), K = (int)(c / ((long long)nI * nJ));
+  const LesNb n = les_nb(c, I, J, K, nI, nJ, nK, yc);
+  const double* tu[3] = {w + 13 * nl, w + 14 * nl, w + 15 * nl};
+  double tS[6], tsn;
+  les_grad(tu, n, dX, dZ, tS, tsn);
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  double L[6];
+  for (int q = 0; q < 6; ++q)
+    L[q] = tfilt(pr + q * nl, I, J, K, nI, nJ, nK) - tu[pa[q]][c] * tu[pb[q]][c];
+  const double tr = (L[0] + L[1] + L[2]) / 3.0;
+  for (int a = 0; a < 3; ++a) L[a] = L[a] - tr;
+  const double d2 = pow(dX * dY[J] * dZ, 2.0 / 3.0);
+  double num = 0.0, den = 0.0;
+  for (int q = 0; q < 6; ++q) {
+    const double wt = q < 3 ? 1.0 : 2.0;
+    const double M =
+        -2.0 * d2 * (2.0 * 2.0 * tsn * tS[q] - tfilt(w + (7 + q) * nl, I, J, K, nI, nJ, nK));
+    num = num + wt * L[q] * M;
+    den = den + wt * M * M;
+  }
+  lm[c] = num;
+  mm[c] = den;
+}
+// pass 4: one workgroup per row J: Cs^2 = sum(L:M) / sum(M:M) (fixed order)
+__global__ void k_dsm_plane(int nI, int nJ, int nK, const double* __restrict__ lm,
+                            const double* __restrict__ mm, double* __restrict__ cs2) {
+  __shared__ double sh[2][256];
+  const int J = blockIdx.x, t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (long long e = t; e < (long long)nI * nK; e += 256) {
+    const int I = (int)(e % nI), K = (int)(e / nI);
+    const long long c = I + (long long)nI * (J + (long long)nJ * K);
+    a += lm[c];
+    b += mm[c];
+  }
+  sh[0][t] = a;
+  sh[1][t] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      sh[0][t] += sh[0][t + h];
+      sh[1][t] += sh[1][t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) cs2[J] = sh[0][0] / fmax(sh[1][0], 1e-300);
+}
+// pass 5: model stress -2 Cs^2 Delta^2 |S| S and the filtered-DNS stress
+__global__ void k_dsm_tau(int nI, int nJ, int nK, const double* __restrict__ w,
+                          const double* __restrict__ fb, const double* __restrict__ cs2,
+                          const double* __restrict__ dY, double dX, double dZ,
+                          double* __restrict__ tm, double* __restrict__ tt) {
+  const long long nl = (long long)nI * nJ * nK;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int J = (int)((c / nI) % nJ);
+  const double d2 = pow(dX * dY[J] * dZ, 2.0 / 3.0);
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) {
+    tm[q * nl + c] = -2.0 * cs2[J] * d2 * w[6 * nl + c] * w[q * nl + c];
+    tt[q * nl + c] = fb[(3 + q) * nl + c] - fb[pa[q] * nl + c] * fb[pb[q] * nl + c];
+  }
+}
+
 }  // namespace c3d
 
 using namespace c3d;
@@ -601,6 +767,10 @@ extern "C" int c3d_closure_size(c3d_closure* cl, long long* n) {
   *n = cl->n;
   return C3D_OK;
 }
+
+// dynamic Smagorinsky on the LES grid of a DNS state (forward declaration)
+extern "C" int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int fz,
+                               double* tau_model, double* tau_true, double* cs2);
 
 extern "C" int c3d_closure_add_samples(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int fz,
                                        long long* first) {
@@ -787,5 +957,90 @@ extern "C" int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float l
   int rc = cl_pack(cl);
   if (rc) return rc;
   C3D_HIP(hipStreamSynchronize(cl->stream));
+  return C3D_OK;
+}
+
+// The dynamic Smagorinsky model on the LES grid of the context's state:
+// tau_model (6 x nl, component-major, deviatoric by construction), the
+// filtered-DNS stress tau_true (6 x nl) and Cs^2 per LES row (nJ); any
+// output may be null.  Order of components: 11 22 33 12 13 23; cells x
+// fastest.
+extern "C" int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int fz,
+                               double* tau_model, double* tau_true, double* cs2) {
+  if (!cl || !c) {
+    set_error("c3d_closure_dsm: null argument");
+    return C3D_ERR_ARG;
+  }
+  if (c->prec != C3D_FP64 || c->order != 2) {
+    set_error("c3d_closure_dsm: needs an FP64 2nd-order (staggered) context");
+    return C3D_ERR_ARG;
+  }
+  const Geom& g = c->g;
+  if (fx < 1 || fy < 1 || fz < 1 || g.nx % fx || g.ny % fy || g.nz % fz || g.ny / fy < 2) {
+    set_error("c3d_closure_dsm: the filter blocks must tile the grid (>= 2 LES rows)");
+    return C3D_ERR_ARG;
+  }
+  const int nI = g.nx / fx, nJ = g.ny / fy, nK = g.nz / fz;
+  const long long nl = (long long)nI * nJ * nK;
+  const auto& yp = c->met[C3D_M_YP];
+  std::vector<double> host(2 * nJ + g.ny + 3);
+  for (int J = 0; J < nJ; ++J) {
+    const double lo = yp[J * fy + 1], hi = yp[(J + 1) * fy + 1];
+    host[nJ + J] = hi - lo;
+    host[J] = 0.5 * (hi + lo);
+  }
+  for (int j = 0; j < g.ny + 3; ++j) host[2 * nJ + j] = yp[j];
+  // device: yc, dY, yp | fb (9 nl) | w (22 nl) | pr (6 nl) | lm, mm | cs2 | tm, tt (12 nl)
+  const size_t nd = host.size() + 9 * nl + 22 * nl + 6 * nl + 2 * nl + nJ + 12 * nl;
+  double* dev = nullptr;
+  int rc = cl_alloc((void**)&dev, sizeof(double) * nd);
+  if (rc) return rc;
+  double* dyc = dev;
+  double* ddY = dyc + nJ;
+  double* dyp = ddY + nJ;
+  double* fb = dev + host.size();
+  double* w = fb + 9 * nl;
+  double* pr = w + 22 * nl;
+  double* lm = pr + 6 * nl;
+  double* mm = lm + nl;
+  double* dcs = mm + nl;
+  double* tm = dcs + nJ;
+  double* tt = tm + 6 * nl;
+  const double dX = fx * c->dx, dZ = fz * c->dz;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice,
+                       cl->stream);
+  if (e == hipSuccess) {
+    const int blk = 256;
+    const unsigned grid = (unsigned)((nl + blk - 1) / blk);
+    hipLaunchKernelGGL(k_cl_filter, dim3(grid), dim3(blk), 0, cl->stream, g,
+                       (const double*)c->state[C3D_UX], (const double*)c->state[C3D_UY],
+                       (const double*)c->state[C3D_UZ], (const double*)dyp, fx, fy, fz, fb);
+    hipLaunchKernelGGL(k_dsm_a, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)fb, (const double*)dyc, dX, dZ, w);
+    hipLaunchKernelGGL(k_dsm_t, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)fb, w);
+    hipLaunchKernelGGL(k_dsm_p, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)fb, pr);
+    hipLaunchKernelGGL(k_dsm_b, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)w, (const double*)pr, (const double*)dyc,
+                       (const double*)ddY, dX, dZ, lm, mm);
+    hipLaunchKernelGGL(k_dsm_plane, dim3(nJ), dim3(256), 0, cl->stream, nI, nJ, nK,
+                       (const double*)lm, (const double*)mm, dcs);
+    hipLaunchKernelGGL(k_dsm_tau, dim3(grid), dim3(blk), 0, cl->stream, nI, nJ, nK,
+                       (const double*)w, (const double*)fb, (const double*)dcs,
+                       (const double*)ddY, dX, dZ, tm, tt);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && tau_model)
+    e = hipMemcpyAsync(tau_model, tm, sizeof(double) * 6 * nl, hipMemcpyDeviceToHost, cl->stream);
+  if (e == hipSuccess && tau_true)
+    e = hipMemcpyAsync(tau_true, tt, sizeof(double) * 6 * nl, hipMemcpyDeviceToHost, cl->stream);
+  if (e == hipSuccess && cs2)
+    e = hipMemcpyAsync(cs2, dcs, sizeof(double) * nJ, hipMemcpyDeviceToHost, cl->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
+  hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "c3d_closure_dsm");
   return C3D_OK;
 }

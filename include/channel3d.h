@@ -388,6 +388,15 @@ int c3d_closure_train(c3d_closure* cl, long long first, long long count, int bat
                       float lr, unsigned seed, float* losses);
 /* one Adam step with a given gradient */
 int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float lr);
+/* Dynamic Smagorinsky (Germano et al. 1991, Lilly 1992: Cs^2 per LES row
+ * from x-z plane sums; test filter (1/4, 1/2, 1/4) in x and z, width ratio
+ * 2) on the LES grid of a DNS state filtered like the samples: the model
+ * stress -2 Cs^2 Delta^2 |S| S (6 x nLES, component-major, 11 22 33 12 13
+ * 23, cells x fastest), the filtered-DNS stress (same layout) and Cs^2
+ * (nLES rows); FP64; any output may be null.  The closure's fallback and
+ * its a-priori comparator. */
+int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* dns, int fx, int fy, int fz, double* tau_model,
+                    double* tau_true, double* cs2);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,30 @@ def test_adam_step_matches_oracle():
     cl.close()
 
 
+def test_dsm_matches_oracle():
+    """the GPU dynamic Smagorinsky model against the oracle's on the same
+    filtered state: Cs^2 per row, model and filtered-DNS stresses"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from channel3d_amd import lib as L
+    g = _smooth_dns(DNS, seed=2)
+    for _ in range(5):
+        g.step()
+    cl = Closure(1 << 12)
+    tm, tt, cs2 = cl.dsm(g)
+    ux, uy, uz = (g.download(f) for f in (L.UX, L.UY, L.UZ))
+    g.close()
+    ub, tau = C.filtered(ux, uy, uz, _yp(), 4, 2, 4)
+    dY, yc = C.les_grid(_yp(), 2)
+    dx, dz = DNS["xlx"] / DNS["nxc"], DNS["zlz"] / DNS["nzc"]
+    to, co = C.dynamic_smagorinsky(ub, 4 * dx, dY, 4 * dz, yc)
+    assert np.allclose(cs2, co, rtol=1e-10, atol=1e-14 * np.abs(co).max())
+    for q in range(6):
+        assert np.allclose(tm[q], to[q], rtol=1e-9, atol=1e-12 * np.abs(to[q]).max()), q
+        assert np.allclose(tt[q], tau[q], rtol=1e-9, atol=1e-12 * np.abs(tau[q]).max()), q
+    cl.close()
+
+
 def test_training_beats_dynamic_smagorinsky_a_priori():
     """train on four DNS snapshots, judge on two later ones: correlation of
     the predicted with the filtered-DNS stresses (tau_12 and the deviatoric
@@ -153,7 +177,8 @@ def test_training_beats_dynamic_smagorinsky_a_priori():
         for _ in range(20):
             g.step()
         first, n = cl.add_samples(g)
-        snaps.append((first, n, [g.download(f) for f in (L.UX, L.UY, L.UZ)]))
+        dsm = cl.dsm(g) if s >= 4 else None
+        snaps.append((first, n, [g.download(f) for f in (L.UX, L.UY, L.UZ)], dsm))
     g.close()
     cl.set_params(C.init_params(0))
     ntrain = snaps[4][0]
@@ -164,15 +189,14 @@ def test_training_beats_dynamic_smagorinsky_a_priori():
     nI, nJ, nK = DNS["nxc"] // 4, DNS["nyc"] // 2, DNS["nzc"] // 4
     dY, yc = C.les_grid(_yp(), 2)
     true, mlp, dsm = [], [], []
-    for first, n, (ux, uy, uz) in snaps[4:]:
+    for first, n, (ux, uy, uz), (tm, tt, _) in snaps[4:]:
         ub, tau = C.filtered(ux, uy, uz, _yp(), 4, 2, 4)
         _, _, S = cl.samples(first, n)
         Y = cl.predict(first, n)
         that = [(Y[:, q] * S).reshape((nI, nJ, nK), order="F") for q in range(6)]
-        tdsm, _ = C.dynamic_smagorinsky(ub, 4 * dx, dY, 4 * dz, yc)
         true.append(C.deviatoric(tau))
         mlp.append(C.deviatoric(that))
-        dsm.append(tdsm)
+        dsm.append(list(tm))     # the GPU model (its parity: test_dsm_matches_oracle)
     r12 = [C.correlation([t[3] for t in m], [t[3] for t in true]) for m in (mlp, dsm)]
     rdev = [C.correlation([x for t in m for x in t], [x for t in true for x in t])
             for m in (mlp, dsm)]
