@@ -234,6 +234,13 @@ class Channel:
             L.check(self.lib.c3d_state_field(self._h, L.NUT, ctypes.byref(v)), "c3d_state_field")
             self._state[L.NUT] = v
 
+    def set_les_closure(self, closure):
+        """the learned closure (channel3d_amd.closure.Closure) as this LES
+        context's SGS model; None restores WALE / MTS"""
+        L.check(self.lib.c3d_set_les_closure(self._h, closure._h if closure else None),
+                "c3d_set_les_closure")
+        self._les_closure = closure   # kept alive while in use
+
     def Clc_SGS_Vis(self, ux=L.UX, uy=L.UY, uz=L.UZ, nut=L.NUT):
         self._call("Clc_SGS_Vis", ux, uy, uz, nut)
 

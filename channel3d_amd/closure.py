@@ -106,6 +106,13 @@ class Closure:
         L.check(self.lib.c3d_closure_adam_step(self._h, _fp(v), float(lr)),
                 "c3d_closure_adam_step")
 
+    def fallbacks(self):
+        """cells of the last LES viscosity evaluation that fell back to
+        dynamic Smagorinsky"""
+        n = ctypes.c_longlong()
+        L.check(self.lib.c3d_closure_fallbacks(self._h, ctypes.byref(n)), "c3d_closure_fallbacks")
+        return n.value
+
     def dsm(self, channel, fx=4, fy=2, fz=4):
         """dynamic Smagorinsky on the LES grid of the channel's state:
         (model stress (6, nI, nJ, nK), filtered-DNS stress, Cs^2 per row)"""

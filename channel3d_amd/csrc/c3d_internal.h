@@ -240,6 +240,8 @@ struct c3d_ctx {
   // its weights, the metric block; the eddy viscosity is state[C3D_NUT]
   int les_type = 0;
   int filter_type = 0;
+  // learned closure replacing the WALE / MTS viscosity (c3d_set_les_closure)
+  struct c3d_closure* les_closure = nullptr;
   double les_fcoe[9] = {};
   void* d_lesmet = nullptr;
   c3d::LesMet<double> lmd{};
@@ -343,6 +345,9 @@ int launch_div4(c3d_ctx* c, const void* ux, const void* uy, const void* uz);
 // ---- launchers (k_les.hip, LES layer of the 2nd-order path) ----
 // eddy viscosity of the context's model into nut, wall ghosts included
 int launch_sgs(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* nut);
+// nut from the learned closure, dynamic Smagorinsky where it backscatters
+int closure_nut(struct c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy,
+                const void* uz, void* nut);
 int launch_rhs_les(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
                    const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
                    int want_bulk);

@@ -621,12 +621,58 @@ __global__ void k_dsm_tau(int nI, int nJ, int nK, const double* __restrict__ w,
   }
 }
 
+__global__ void k_cl_zero_int(int* p) { p[threadIdx.x] = 0; }
+
+// nut of an LES cell from the network's stress: the least-squares eddy
+// viscosity nu = -(tau^d : S) / (2 S : S) of its deviatoric prediction; where
+// that is negative (backscatter, which an eddy-viscosity step cannot carry)
+// or not finite, the dynamic Smagorinsky value max(Cs^2, 0) Delta^2 |S|.
+// nut = xnu + nu, with the wall ghosts of SetBC_and_UpdateHalo_SGS
+// (m_LESModels.f90:375-428).  LES cell c = (i, j-1, k) of the context.
+__global__ void k_cl_nut(Geom g, int nl, const float* __restrict__ Y,
+                         const float* __restrict__ scale, const double* __restrict__ w,
+                         const double* __restrict__ cs2, const double* __restrict__ dY,
+                         double dX, double dZ, double xnu, int bcym, int bcyp,
+                         double* __restrict__ nut, int* __restrict__ nfallback) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  const int i = (int)(c % g.nx), J = (int)((c / g.nx) % g.ny), k = (int)(c / ((long long)g.nx * g.ny));
+  double t[6];
+  for (int q = 0; q < 6; ++q) t[q] = (double)Y[c * NOUT + q] * (double)scale[c];
+  const double tr = (t[0] + t[1] + t[2]) / 3.0;
+  for (int q = 0; q < 3; ++q) t[q] -= tr;
+  double ts = 0.0, ss = 0.0;
+  for (int q = 0; q < 6; ++q) {
+    const double wt = q < 3 ? 1.0 : 2.0;
+    const double S = w[q * (long long)nl + c];
+    ts += wt * t[q] * S;
+    ss += wt * S * S;
+  }
+  double nu = -ts / (2.0 * ss);
+  if (!(nu >= 0.0) || !isfinite(nu)) {
+    const double sn = w[6 * (long long)nl + c];
+    nu = fmax(cs2[J], 0.0) * pow(dX * dY[J] * dZ, 2.0 / 3.0) * sn;
+    atomicAdd(nfallback, 1);
+  }
+  const double v = xnu + nu;
+  const int jc = J + 1;
+  nut[fidx(g, i, jc, k)] = v;
+  if (jc == 1) nut[fidx(g, i, 0, k)] = bcym == C3D_BC_NOSLIP ? 2.0 * xnu - v : v;
+  if (jc == g.ny) nut[fidx(g, i, g.ny + 1, k)] = bcyp == C3D_BC_NOSLIP ? 2.0 * xnu - v : v;
+}
+
 }  // namespace c3d
 
 using namespace c3d;
 
 struct c3d_closure {
   hipStream_t stream = nullptr;
+  // scratch of the LES viscosity (closure_nut), sized on first use
+  double* nut_d = nullptr;
+  float* nut_f = nullptr;
+  int* nut_fb = nullptr;
+  long long nut_cap = 0;
+  long long nfallback = 0;
   long long cap = 0, n = 0;        // dataset capacity and size (samples)
   float* X = nullptr;              // cap x 16
   float* T = nullptr;              // cap x 6
@@ -732,7 +778,8 @@ extern "C" int c3d_closure_destroy(c3d_closure* cl) {
   if (!cl) return C3D_OK;
   if (cl->stream) hipStreamSynchronize(cl->stream);
   void* bufs[] = {cl->X, cl->T, cl->S, cl->w, cl->m, cl->v, cl->g, cl->loss, cl->fwd, cl->bwd,
-                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y};
+                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y, cl->nut_d, cl->nut_f,
+                  cl->nut_fb};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (cl->stream) hipStreamDestroy(cl->stream);
@@ -1042,5 +1089,101 @@ extern "C" int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int 
   if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
   hipFree(dev);
   if (e != hipSuccess) return hip_fail(e, "c3d_closure_dsm");
+  return C3D_OK;
+}
+
+// The learned closure as the SGS viscosity of an LES context: features of
+// every LES cell from the context's own velocity (the grid is the filter:
+// blocks of one cell), the network's stress, its eddy viscosity, dynamic
+// Smagorinsky where it backscatters.  Runs on the context's stream.
+int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy,
+                     const void* uz, void* nut) {
+  const Geom& g = c->g;
+  const int nI = g.nx, nJ = g.ny, nK = g.nz;
+  const long long nl = (long long)nI * nJ * nK;
+  int rc;
+  if (cl->nut_cap < nl) {
+    for (void* p : {(void*)cl->nut_d, (void*)cl->nut_f, (void*)cl->nut_fb})
+      if (p) hipFree(p);
+    cl->nut_d = nullptr;
+    cl->nut_f = nullptr;
+    cl->nut_fb = nullptr;
+    const long long nd = 2 * nJ + g.ny + 3 + 9 * nl + 22 * nl + 6 * nl + 2 * nl + nJ;
+    if ((rc = cl_alloc((void**)&cl->nut_d, sizeof(double) * nd))) return rc;
+    if ((rc = cl_alloc((void**)&cl->nut_f, sizeof(float) * (NPAD + NOUT + 1 + NOUT) * nl)))
+      return rc;
+    if ((rc = cl_alloc((void**)&cl->nut_fb, sizeof(int)))) return rc;
+    // the LES rows are the context's cells: heights and centres never change
+    const auto& yp = c->met[C3D_M_YP];
+    std::vector<double> host(2 * nJ + g.ny + 3);
+    for (int J = 0; J < nJ; ++J) {
+      host[nJ + J] = yp[J + 2] - yp[J + 1];
+      host[J] = 0.5 * (yp[J + 2] + yp[J + 1]);
+    }
+    for (int j = 0; j < g.ny + 3; ++j) host[2 * nJ + j] = yp[j];
+    C3D_HIP(hipMemcpy(cl->nut_d, host.data(), sizeof(double) * host.size(),
+                      hipMemcpyHostToDevice));
+    cl->nut_cap = nl;
+  }
+  double* dyc = cl->nut_d;
+  double* ddY = dyc + nJ;
+  double* dyp = ddY + nJ;
+  double* fb = dyp + g.ny + 3;
+  double* w = fb + 9 * nl;
+  double* pr = w + 22 * nl;
+  double* lm = pr + 6 * nl;
+  double* mm = lm + nl;
+  double* dcs = mm + nl;
+  float* X = cl->nut_f;
+  float* T = X + NPAD * nl;
+  float* S = T + NOUT * nl;
+  float* Y = S + nl;
+  const double dX = c->dx, dZ = c->dz;
+  hipStream_t st = c->stream;
+  const int blk = 256;
+  const unsigned grid = (unsigned)((nl + blk - 1) / blk);
+  hipLaunchKernelGGL(k_cl_filter, dim3(grid), dim3(blk), 0, st, g, (const double*)ux,
+                     (const double*)uy, (const double*)uz, (const double*)dyp, 1, 1, 1, fb);
+  hipLaunchKernelGGL(k_dsm_a, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)fb,
+                     (const double*)dyc, dX, dZ, w);
+  hipLaunchKernelGGL(k_dsm_t, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)fb, w);
+  hipLaunchKernelGGL(k_dsm_p, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)fb, pr);
+  hipLaunchKernelGGL(k_dsm_b, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)w,
+                     (const double*)pr, (const double*)dyc, (const double*)ddY, dX, dZ, lm, mm);
+  hipLaunchKernelGGL(k_dsm_plane, dim3(nJ), dim3(256), 0, st, nI, nJ, nK, (const double*)lm,
+                     (const double*)mm, dcs);
+  hipLaunchKernelGGL(k_cl_samples, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)fb,
+                     (const double*)dyc, (const double*)ddY, dX, dZ, c->p.yly, X, T, S);
+  const long long ntile = (nl + 15) / 16;
+  hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(4096, ntile)), dim3(64), 0,
+                     st, (const float*)X, nl, cl->fwd, cl->w, Y);
+  hipLaunchKernelGGL(k_cl_zero_int, dim3(1), dim3(1), 0, st, cl->nut_fb);
+  hipLaunchKernelGGL(k_cl_nut, dim3(grid), dim3(blk), 0, st, g, (int)nl, (const float*)Y,
+                     (const float*)S, (const double*)w, (const double*)dcs, (const double*)ddY,
+                     dX, dZ, c->p.xnu, c->p.bc[2], c->p.bc[3], (double*)nut, cl->nut_fb);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+extern "C" int c3d_set_les_closure(c3d_ctx* c, c3d_closure* cl) {
+  if (!c) return C3D_ERR_ARG;
+  if (cl && (!c->les_type || c->prec != C3D_FP64)) {
+    set_error("c3d_set_les_closure: needs an FP64 context with LES on (c3d_set_les)");
+    return C3D_ERR_ARG;
+  }
+  C3D_HIP(hipStreamSynchronize(c->stream));
+  c->les_closure = cl;
+  return C3D_OK;
+}
+
+// cells that fell back to dynamic Smagorinsky in the last closure_nut
+extern "C" int c3d_closure_fallbacks(c3d_closure* cl, long long* n) {
+  if (!cl || !n) return C3D_ERR_ARG;
+  int h = 0;
+  if (cl->nut_fb) {
+    C3D_HIP(hipDeviceSynchronize());
+    C3D_HIP(hipMemcpy(&h, cl->nut_fb, sizeof(int), hipMemcpyDeviceToHost));
+  }
+  *n = h;
   return C3D_OK;
 }

@@ -507,8 +507,9 @@ static int sgs_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, voi
 int launch_sgs(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* nut) {
   hipEvent_t ev;
   prof_begin(c, ST_SGS, &ev);
-  int rc = c->prec == C3D_FP64 ? sgs_t<double>(c, ux, uy, uz, nut)
-                               : sgs_t<float>(c, ux, uy, uz, nut);
+  int rc = c->les_closure ? closure_nut(c->les_closure, c, ux, uy, uz, nut)
+           : c->prec == C3D_FP64 ? sgs_t<double>(c, ux, uy, uz, nut)
+                                 : sgs_t<float>(c, ux, uy, uz, nut);
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_SGS, ev, 4.0 * N * c->esize);

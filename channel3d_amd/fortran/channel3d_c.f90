@@ -241,6 +241,10 @@ module channel3d_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: sweeps
     end function
+    integer(c_int) function c3d_set_les_closure(ctx, cl) bind(C, name="c3d_set_les_closure")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, cl
+    end function
     integer(c_int) function c3d_set_graph(ctx, on) bind(C, name="c3d_set_graph")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -51,7 +51,7 @@ EXPORTS = (
     "c3d_closure_nparam", "c3d_closure_create", "c3d_closure_destroy", "c3d_closure_set_params",
     "c3d_closure_get_params", "c3d_closure_size", "c3d_closure_add_samples",
     "c3d_closure_get_samples", "c3d_closure_predict", "c3d_closure_grad", "c3d_closure_train",
-    "c3d_closure_adam_step", "c3d_closure_dsm",
+    "c3d_closure_adam_step", "c3d_closure_dsm", "c3d_set_les_closure", "c3d_closure_fallbacks",
 )
 
 
@@ -165,6 +165,8 @@ def load(path: str | None = None):
                                ctypes.c_uint, _FP], I),
         "c3d_closure_adam_step": ([_VP, _FP, ctypes.c_float], I),
         "c3d_closure_dsm": ([_VP, _VP, I, I, I, _DP, _DP, _DP], I),
+        "c3d_set_les_closure": ([_VP, _VP], I),
+        "c3d_closure_fallbacks": ([_VP, ctypes.POINTER(ctypes.c_longlong)], I),
         "c3d_clcStat": ([_VP] * 5, I),
         "c3d_stat_set_grad": ([_VP, I], I),
         "c3d_stat_get_grad": ([_VP, _DP], I),

@@ -397,6 +397,15 @@ int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float lr);
  * its a-priori comparator. */
 int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* dns, int fx, int fy, int fz, double* tau_model,
                     double* tau_true, double* cs2);
+/* The closure as the SGS model of an FP64 LES context (LES already on with
+ * c3d_set_les; null restores WALE / MTS): c3d_Clc_SGS_Vis and c3d_step then
+ * compute nut = xnu + nu per cell from the network's stress at the context's
+ * own grid (its cells are the filter), nu = -(tau^d : S) / (2 S : S), and
+ * the dynamic Smagorinsky max(Cs^2, 0) Delta^2 |S| where that is negative
+ * (backscatter) or not finite; c3d_closure_fallbacks reads how many cells
+ * fell back in the last evaluation.  The closure must outlive its use. */
+int c3d_set_les_closure(c3d_ctx* ctx, c3d_closure* cl);
+int c3d_closure_fallbacks(c3d_closure* cl, long long* n);
 
 #ifdef __cplusplus
 }
