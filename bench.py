@@ -223,6 +223,26 @@ def run_closure(train_steps, batch):
                                for q in range(6)]))
         res["dsm"].append(list(tm))
         res["true"].append(dev(list(tt)))
+    # the trained network as the SGS model of the 96x96x72 LES step
+    from tests import cases as TC
+    lc = TC.LES96
+    h = Channel(lc)
+    ux, uy, uz = synth.smooth_modes(lc["nxc"], lc["nyc"], lc["nzc"], h.yp, lc["xlx"], lc["yly"],
+                                    lc["zlz"], seed=1)
+    for f, v in ((L.UX, ux), (L.UY, uy), (L.UZ, uz)):
+        h.upload(f, v)
+    h.project()
+    t_wale = time_steps(h, 20, 3, False)
+    h.set_les_closure(cl)
+    t_learn = time_steps(h, 20, 3, False)
+    nfb = cl.fallbacks()
+    dl = h.diagnostics()
+    h.set_les_closure(None)
+    h.close()
+    les_learned = dict(ms_per_step=round(1e3 * t_learn / 20, 3),
+                       wale_ms_per_step=round(1e3 * t_wale / 20, 3),
+                       fallback_fraction=round(nfb / (lc["nxc"] * lc["nyc"] * lc["nzc"]), 4),
+                       divmax_after_step=dl["divmax2"])
     corr = {}
     for key in ("mlp", "dsm"):
         corr[key] = dict(
@@ -241,6 +261,7 @@ def run_closure(train_steps, batch):
                 train_mfma_tflops=round(2 * CLOSURE_MACS * sps / 1e12, 3),
                 loss_first=float(losses[:10].mean()), loss_last=float(losses[-10:].mean()),
                 a_priori_corr_held_out=corr,
+                les96_with_closure=les_learned,
                 note="targets: tau / (Delta^2 |S|^2) of the filtered DNS; inputs and targets "
                      "clipped to +-10; fields: laminar flow plus smooth random modes, "
                      "projected, 20-120 steps evolved; correlations of the deviatoric "
