@@ -420,21 +420,27 @@ __global__ __launch_bounds__(64) void k_cl_grad(const float* __restrict__ X,
   if (lane == 0) lpart[blockIdx.x] = lsum;
 }
 
-// grad[p] = sum of the workgroup partials in workgroup order (FP64 sums);
-// loss = sum of squared errors / (ntot * 6)
-__global__ void k_cl_reduce(const float* __restrict__ partial, int nwg,
-                            const double* __restrict__ lpart, double ntot,
-                            float* __restrict__ grad, float* __restrict__ loss) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < NPARAM) {
-    double a = 0.0;
-    for (int b = 0; b < nwg; ++b) a += (double)partial[(long long)b * NPARAM + p];
-    grad[p] = (float)a;
-  }
-  if (p == 0 && loss) {
-    double a = 0.0;
-    for (int b = 0; b < nwg; ++b) a += lpart[b];
-    *loss = (float)(a / (ntot * NOUT));
+// grad[p] = sum of the workgroup partials (FP64, a fixed order: four
+// interleaved runs over the workgroups, then their sum); 64 parameters per
+// workgroup so every partial row is read in 256-byte runs.  loss = sum of
+// squared errors / (ntot * 6).
+__global__ __launch_bounds__(256) void k_cl_reduce(const float* __restrict__ partial, int nwg,
+                                                   const double* __restrict__ lpart, double ntot,
+                                                   float* __restrict__ grad,
+                                                   float* __restrict__ loss) {
+  __shared__ double sh[4][64];
+  const int t = threadIdx.x, pl = t & 63, bg = t >> 6;
+  const int p = blockIdx.x * 64 + pl;
+  double a = 0.0;
+  if (p < NPARAM)
+    for (int b = bg; b < nwg; b += 4) a += (double)partial[(long long)b * NPARAM + p];
+  sh[bg][pl] = a;
+  __syncthreads();
+  if (bg == 0 && p < NPARAM) grad[p] = (float)(((sh[0][pl] + sh[1][pl]) + sh[2][pl]) + sh[3][pl]);
+  if (blockIdx.x == 0 && t == 0 && loss) {
+    double s2 = 0.0;
+    for (int b = 0; b < nwg; ++b) s2 += lpart[b];
+    *loss = (float)(s2 / (ntot * NOUT));
   }
 }
 
@@ -649,11 +655,15 @@ __global__ void k_cl_nut(Geom g, int nl, const float* __restrict__ Y,
     ss += wt * S * S;
   }
   double nu = -ts / (2.0 * ss);
-  if (!(nu >= 0.0) || !isfinite(nu)) {
+  const bool back = !(nu >= 0.0) || !isfinite(nu);
+  if (back) {
     const double sn = w[6 * (long long)nl + c];
     nu = fmax(cs2[J], 0.0) * pow(dX * dY[J] * dZ, 2.0 / 3.0) * sn;
-    atomicAdd(nfallback, 1);
   }
+  // one counter update per wavefront
+  const unsigned long long m = __ballot(back);
+  if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)__ballot(1)) - 1) && m)
+    atomicAdd(nfallback, (int)__popcll(m));
   const double v = xnu + nu;
   const int jc = J + 1;
   nut[fidx(g, i, jc, k)] = v;
@@ -727,7 +737,7 @@ int cl_grad(c3d_closure* cl, long long first, const int* idx, long long count, f
   hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64), 0, cl->stream, cl->X, cl->T, first, idx,
                      count, (double)count, cl->fwd, cl->bwd, cl->w, cl->partial, cl->lpart);
   C3D_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream,
+  hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 63) / 64), dim3(256), 0, cl->stream,
                      cl->partial, nwg, cl->lpart, (double)count, cl->g, dloss);
   C3D_HIP(hipGetLastError());
   return C3D_OK;
