@@ -420,23 +420,29 @@ __global__ __launch_bounds__(64) void k_cl_grad(const float* __restrict__ X,
   if (lane == 0) lpart[blockIdx.x] = lsum;
 }
 
-// grad[p] = sum of the workgroup partials (FP64, a fixed order: four
-// interleaved runs over the workgroups, then their sum); 64 parameters per
-// workgroup so every partial row is read in 256-byte runs.  loss = sum of
-// squared errors / (ntot * 6).
-__global__ __launch_bounds__(256) void k_cl_reduce(const float* __restrict__ partial, int nwg,
-                                                   const double* __restrict__ lpart, double ntot,
-                                                   float* __restrict__ grad,
-                                                   float* __restrict__ loss) {
-  __shared__ double sh[4][64];
+// grad[p] = sum of the workgroup partials (FP64, a fixed order: sixteen
+// interleaved runs over the workgroups, each with four independent partial
+// sums so that several loads are in flight, then their sum); 64 parameters
+// per workgroup of 1024 threads, so every partial row is read in 256-byte
+// runs.  loss = sum of squared errors / (ntot * 6).
+__global__ __launch_bounds__(1024) void k_cl_reduce(const float* __restrict__ partial, int nwg,
+                                                    const double* __restrict__ lpart,
+                                                    double ntot, float* __restrict__ grad,
+                                                    float* __restrict__ loss) {
+  __shared__ double sh[16][64];
   const int t = threadIdx.x, pl = t & 63, bg = t >> 6;
   const int p = blockIdx.x * 64 + pl;
-  double a = 0.0;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < NPARAM)
-    for (int b = bg; b < nwg; b += 4) a += (double)partial[(long long)b * NPARAM + p];
-  sh[bg][pl] = a;
+    for (int b = bg, r = 0; b < nwg; b += 16, r = (r + 1) & 3)
+      a[r] += (double)partial[(long long)b * NPARAM + p];
+  sh[bg][pl] = (a[0] + a[1]) + (a[2] + a[3]);
   __syncthreads();
-  if (bg == 0 && p < NPARAM) grad[p] = (float)(((sh[0][pl] + sh[1][pl]) + sh[2][pl]) + sh[3][pl]);
+  if (bg == 0 && p < NPARAM) {
+    double s = 0.0;
+    for (int q = 0; q < 16; ++q) s += sh[q][pl];
+    grad[p] = (float)s;
+  }
   if (blockIdx.x == 0 && t == 0 && loss) {
     double s2 = 0.0;
     for (int b = 0; b < nwg; ++b) s2 += lpart[b];
@@ -737,7 +743,7 @@ int cl_grad(c3d_closure* cl, long long first, const int* idx, long long count, f
   hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64), 0, cl->stream, cl->X, cl->T, first, idx,
                      count, (double)count, cl->fwd, cl->bwd, cl->w, cl->partial, cl->lpart);
   C3D_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 63) / 64), dim3(256), 0, cl->stream,
+  hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 63) / 64), dim3(1024), 0, cl->stream,
                      cl->partial, nwg, cl->lpart, (double)count, cl->g, dloss);
   C3D_HIP(hipGetLastError());
   return C3D_OK;
