@@ -1112,11 +1112,13 @@ extern "C" int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int 
 // every LES cell from the context's own velocity (the grid is the filter:
 // blocks of one cell), the network's stress, its eddy viscosity, dynamic
 // Smagorinsky where it backscatters.  Runs on the context's stream.
-int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy,
-                     const void* uz, void* nut) {
+// scratch of closure_nut for the context's grid, allocated when the closure
+// is attached so that the steps themselves never allocate (a step may be
+// captured into a graph)
+static int nut_prepare(c3d_closure* cl, c3d_ctx* c) {
   const Geom& g = c->g;
-  const int nI = g.nx, nJ = g.ny, nK = g.nz;
-  const long long nl = (long long)nI * nJ * nK;
+  const int nJ = g.ny;
+  const long long nl = (long long)g.nx * g.ny * g.nz;
   int rc;
   if (cl->nut_cap < nl) {
     for (void* p : {(void*)cl->nut_d, (void*)cl->nut_f, (void*)cl->nut_fb})
@@ -1140,6 +1142,18 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
     C3D_HIP(hipMemcpy(cl->nut_d, host.data(), sizeof(double) * host.size(),
                       hipMemcpyHostToDevice));
     cl->nut_cap = nl;
+  }
+  return C3D_OK;
+}
+
+int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy,
+                     const void* uz, void* nut) {
+  const Geom& g = c->g;
+  const int nI = g.nx, nJ = g.ny, nK = g.nz;
+  const long long nl = (long long)nI * nJ * nK;
+  if (cl->nut_cap < nl) {
+    set_error("closure_nut: closure not prepared for this context (c3d_set_les_closure)");
+    return C3D_ERR_ARG;
   }
   double* dyc = cl->nut_d;
   double* ddY = dyc + nJ;
@@ -1188,6 +1202,15 @@ extern "C" int c3d_set_les_closure(c3d_ctx* c, c3d_closure* cl) {
     return C3D_ERR_ARG;
   }
   C3D_HIP(hipStreamSynchronize(c->stream));
+  if (cl) {
+    int rc = nut_prepare(cl, c);
+    if (rc) return rc;
+  }
+  // graphs of c3d_step baked the previous SGS model's kernels
+  for (auto& x : c->gexec) {
+    if (x) hipGraphExecDestroy(x);
+    x = nullptr;
+  }
   c->les_closure = cl;
   return C3D_OK;
 }
