@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
   auto step = [&]() -> T {
 #pragma unroll
     for (int q = 0; q < 2; ++q) sm[ty][tx + 64 * q] = ADD ? raw[q] + add : raw[q];
-    if (pend_at >= 0) out[pend_at] = pend;
+    if (pend_at >= 0) __builtin_nontemporal_store(pend, &out[pend_at]);  // written once
     pend_at = -1;
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (kn + 1 < klast) load(wrapk(kn + 1), raw);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
       }
     }
   }
-  if (pend_at >= 0) out[pend_at] = pend;
+  if (pend_at >= 0) __builtin_nontemporal_store(pend, &out[pend_at]);
 }
 
 // Thomas sweeps along y with shared factors: a(j), vecm(j), arrmn(j) for
@@ -346,13 +346,13 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      uv[q] = j <= ny ? u[base + sj * j] : T(0);
+      uv[q] = j <= ny ? __builtin_nontemporal_load(&u[base + sj * j]) : T(0);  // once
     }
 #pragma unroll
     for (int q = C - 1; q >= 0; --q) {
       const int j = lo + q;
       if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
-      if (j <= ny) u[base + sj * j] = uv[q] + x;
+      if (j <= ny) __builtin_nontemporal_store(uv[q] + x, &u[base + sj * j]);
     }
   }
 }

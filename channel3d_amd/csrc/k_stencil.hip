@@ -161,8 +161,13 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T gradp = (p_c - pr[rowc + im]) * cf.rdx;
         fix_add(s1, (double)(u_c * s1coe));
         const T conv = -h11 - h12 - h13 + cf.gx;
-        rx[c] = cf.pmGamma * conv + cf.pmTheta * hx[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
-        hx[c] = conv;
+        // written and read once per step: non-temporal, so they do not evict
+        // the neighbour rows the next tiles read from L2
+        const T hold = __builtin_nontemporal_load(&hx[c]);
+        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+                                        T(2) * cf.pmBeta * dcq,
+                                    &rx[c]);
+        __builtin_nontemporal_store(conv, &hx[c]);
       }
       if (MASK & 2) {
         const T h21 = ((I1 * ux[rowm + ip] + I2 * u_xp) * (v_xp + v_c) -
@@ -177,8 +182,13 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T dcq = d11 + d22 + d33;
         const T gradp = (p_c - pr[rowm + i]) * sucac;
         const T conv = -h21 - h22 - h23 + cf.gy;
-        ry[c] = cf.pmGamma * conv + cf.pmTheta * hy[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
-        hy[c] = conv;
+        // written and read once per step: non-temporal, so they do not evict
+        // the neighbour rows the next tiles read from L2
+        const T hold = __builtin_nontemporal_load(&hy[c]);
+        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+                                        T(2) * cf.pmBeta * dcq,
+                                    &ry[c]);
+        __builtin_nontemporal_store(conv, &hy[c]);
       }
       if (MASK & 4) {
         const T h31 = ((u_xp + uxp_m) * (w_xp + w_c) - (u_c + u_m) * (w_c + w_xm)) * qdx1;
@@ -191,8 +201,13 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T dcq = d11 + d22 + d33;
         const T gradp = (p_c - p_m) * cf.rdz;
         const T conv = -h31 - h32 - h33 + cf.gz;
-        rz[c] = cf.pmGamma * conv + cf.pmTheta * hz[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
-        hz[c] = conv;
+        // written and read once per step: non-temporal, so they do not evict
+        // the neighbour rows the next tiles read from L2
+        const T hold = __builtin_nontemporal_load(&hz[c]);
+        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+                                        T(2) * cf.pmBeta * dcq,
+                                    &rz[c]);
+        __builtin_nontemporal_store(conv, &hz[c]);
       }
       // roll
       u_m = u_c; v_m = v_c; w_m = w_c; p_m = p_c;
@@ -395,11 +410,15 @@ __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2
       const T phx = phi[rowc + im];
       const T phy = (GHOST && jc == 1) ? locphi : phi[c - g.nx];
       const T phz = phi[(long long)g.nx * jc + g.plane * km + i];
-      ux[c] = ux[c] - (locphi - phx) * rdxEta2;
-      uy[c] = uy[c] - (locphi - phy) * sucacEta2;
-      uz[c] = uz[c] - (locphi - phz) * rdzEta2;
+      // u, v, w and p are read and written once: non-temporal
+      __builtin_nontemporal_store(__builtin_nontemporal_load(&ux[c]) - (locphi - phx) * rdxEta2,
+                                  &ux[c]);
+      __builtin_nontemporal_store(__builtin_nontemporal_load(&uy[c]) - (locphi - phy) * sucacEta2,
+                                  &uy[c]);
+      __builtin_nontemporal_store(__builtin_nontemporal_load(&uz[c]) - (locphi - phz) * rdzEta2,
+                                  &uz[c]);
     }
-    if (PRS) pr[c] = pr[c] + locphi;
+    if (PRS) __builtin_nontemporal_store(__builtin_nontemporal_load(&pr[c]) + locphi, &pr[c]);
   }
 }
 
