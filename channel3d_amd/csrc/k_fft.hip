@@ -258,7 +258,9 @@ __device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
     const T dr = zk.x - zn.x, di = zk.y + zn.y;
     const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
     const T2 wo = cmul(twr[k], o);
-    spec[(long long)g.P * row + k] = mk<T2>(er + wo.x, ei + wo.y);
+    T2* d = &spec[(long long)g.P * row + k];  // written once: non-temporal
+    __builtin_nontemporal_store(er + wo.x, &d->x);
+    __builtin_nontemporal_store(ei + wo.y, &d->y);
   }
 }
 
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
       const TO2 o = rp[m];
       rp[m] = mk<TO2>(o.x + (TOUT)z.x * scale, o.y + (TOUT)z.y * scale);
     } else {
-      rp[m] = mk<TO2>((TOUT)z.x * scale, (TOUT)z.y * scale);
+      __builtin_nontemporal_store((TOUT)z.x * scale, &rp[m].x);  // written once
+      __builtin_nontemporal_store((TOUT)z.y * scale, &rp[m].y);
     }
   };
   stockham_io<T2, N, LINES, LDN, FNT, 1, false, false, 1>(sm, tw, ld, st);
@@ -445,7 +448,11 @@ __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __rest
   T2* base = spec + (long long)g.P * blockIdx.y + kx0;
   auto ld = [&](int l, int kz) -> T2 { return l < nl ? base[sk * kz + l] : mk<T2>(0, 0); };
   auto st = [&](int l, int kz, const T2& v) {
-    if (l < nl) base[sk * kz + l] = v;
+    if (l < nl) {  // written once: non-temporal
+      T2* d = &base[sk * kz + l];
+      __builtin_nontemporal_store(v.x, &d->x);
+      __builtin_nontemporal_store(v.y, &d->y);
+    }
   };
   stockham_io<T2, N, TK, LDN, FNT, S, true, false, 1>(sm, tw, ld, st);
 }

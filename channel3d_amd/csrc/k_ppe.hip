@@ -226,7 +226,8 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
           x.x = y[q].x - a1 * x.x;
           x.y = y[q].y - a1 * x.y;
         }
-        spec[base + P * j0] = x;
+        __builtin_nontemporal_store(x.x, &spec[base + P * j0].x);  // written once
+        __builtin_nontemporal_store(x.y, &spec[base + P * j0].y);
       }
     }
     arnext = ar[0];
