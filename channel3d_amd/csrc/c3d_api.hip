@@ -731,6 +731,9 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   // 8-wide kx tiles of the z transforms are whole cache lines
   g.P = ((g.nk + 1) + 7) / 8 * 8;
   g.nspec = (long long)g.P * g.ny * g.nz;
+  // non-temporal once-touched words above 64 MiB per field (the L2s hold
+  // 32 MiB, the last-level cache 256 MiB: smaller grids stay cached)
+  g.nt = (double)g.nfield * (p->precision == C3D_FP64 ? 8 : 4) >= 64.0 * 1024 * 1024;
   compute_metrics(c, yp);
   // m_Parameters.f90:135-150
   if (p->ischeme == 1) {

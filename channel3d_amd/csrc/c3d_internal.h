@@ -33,7 +33,18 @@ struct Geom {
   int nk;             // nx/2 (complex length of the x transforms)
   int P;              // spectral kx pitch (complex elements)
   long long nspec;    // P * ny * nz
+  int nt = 0;         // fields far larger than the caches: the kernels that read
+                      // or write words once take their non-temporal variant
 };
+
+// A word a kernel reads or writes exactly once: non-temporal in the kernel
+// variants for grids whose fields stream through the caches anyway (it then
+// leaves the neighbour rows other workgroups still need in L2), ordinary in
+// the variants for grids that fit in the caches (where the next kernel would
+// otherwise miss).  NTS is the kernel's compile-time choice (a run-time
+// select of the two forms would be merged into one ordinary access).
+#define C3D_LD_ONCE(p) (NTS ? __builtin_nontemporal_load(p) : *(p))
+#define C3D_ST_ONCE(v, p) (NTS ? __builtin_nontemporal_store((v), (p)) : (void)(*(p) = (v)))
 
 __host__ __device__ inline long long fidx(const Geom& g, int i, int jj, int k) {
   return (long long)i + (long long)g.nx * ((long long)jj + (long long)g.nyh * (long long)k);

@@ -20,6 +20,8 @@
 // back substitution.
 #include "c3d_internal.h"
 
+#include <type_traits>
+
 namespace c3d {
 
 constexpr int ABX = 64, ABY = 4, AKCH = 8;
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(ABX* ABY) void k_conv_z(Geom g, const double* __res
 // lead-in), every output element written once.
 constexpr int MAXMX = 32;
 
-template <typename T, int MZ, bool ADD>
+template <typename T, int MZ, bool ADD, bool NTS = false>
 __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restrict__ xt, int mx,
                                                  const double* __restrict__ zt,
                                                  const T* __restrict__ f, T* __restrict__ out,
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
   auto step = [&]() -> T {
 #pragma unroll
     for (int q = 0; q < 2; ++q) sm[ty][tx + 64 * q] = ADD ? raw[q] + add : raw[q];
-    if (pend_at >= 0) __builtin_nontemporal_store(pend, &out[pend_at]);  // written once
+    if (pend_at >= 0) C3D_ST_ONCE(pend, &out[pend_at]);  // written once
     pend_at = -1;
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (kn + 1 < klast) load(wrapk(kn + 1), raw);
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
       }
     }
   }
-  if (pend_at >= 0) __builtin_nontemporal_store(pend, &out[pend_at]);
+  if (pend_at >= 0) C3D_ST_ONCE(pend, &out[pend_at]);
 }
 
 // Thomas sweeps along y with shared factors: a(j), vecm(j), arrmn(j) for
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 // the right-hand side is read twice, u once, u written once (4 words per
 // point instead of 5); the results equal k_adi_y's bit for bit.
 // ---------------------------------------------------------------------------
-template <typename T, int C, bool WALLROW, bool SHIFT>
+template <typename T, int C, bool WALLROW, bool SHIFT, bool NTS = false>
 __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
                                                   const double* __restrict__ tvecm,
                                                   const double* __restrict__ tarr,
@@ -346,13 +348,13 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      uv[q] = j <= ny ? __builtin_nontemporal_load(&u[base + sj * j]) : T(0);  // once
+      uv[q] = j <= ny ? C3D_LD_ONCE(&u[base + sj * j]) : T(0);  // once
     }
 #pragma unroll
     for (int q = C - 1; q >= 0; --q) {
       const int j = lo + q;
       if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
-      if (j <= ny) __builtin_nontemporal_store(uv[q] + x, &u[base + sj * j]);
+      if (j <= ny) C3D_ST_ONCE(uv[q] + x, &u[base + sj * j]);
     }
   }
 }
@@ -473,15 +475,20 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
   constexpr int CSEG = 32;
   const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * 256 * sizeof(T);
   if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
-    if (comp == 1)
-      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, true, false>), gy, by, ck_bytes, c->stream, g,
-                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc);
-    else if (shift)
-      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, true>), gy, by, ck_bytes, c->stream, g,
-                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
-    else
-      hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, false>), gy, by, ck_bytes, c->stream, g,
-                         yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+    auto launch = [&](auto ntc) {
+      constexpr bool NT_ = decltype(ntc)::value;
+      if (comp == 1)
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, true, false, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc);
+      else if (shift)
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, true, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+      else
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, false, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+    };
+    if (g.nt) launch(std::true_type{});
+    else launch(std::false_type{});
     C3D_HIP(hipGetLastError());
     return C3D_OK;
   }
@@ -528,9 +535,14 @@ static bool launch_xz_fused(c3d_ctx* c, const c3d_ctx::AdiTab& tb, const void* r
   switch (mz) {
 #define XZ(M)                                                                                  \
   case M:                                                                                      \
-    hipLaunchKernelGGL((k_conv_xz<T, M, ADD>), grid, blk, 0, c->stream, g, tb.d_xtaps, mx,     \
-                       tb.d_ztaps, (const T*)rhs, (T*)c->tmpA, (const DevScalars*)c->d_sc,     \
-                       kchunk, gxb, gyb);                                                      \
+    if (g.nt)                                                                                  \
+      hipLaunchKernelGGL((k_conv_xz<T, M, ADD, true>), grid, blk, 0, c->stream, g, tb.d_xtaps, \
+                         mx, tb.d_ztaps, (const T*)rhs, (T*)c->tmpA,                           \
+                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb);                        \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_conv_xz<T, M, ADD, false>), grid, blk, 0, c->stream, g,            \
+                         tb.d_xtaps, mx, tb.d_ztaps, (const T*)rhs, (T*)c->tmpA,               \
+                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb);                        \
     break;
     XZ(0) XZ(1) XZ(2) XZ(3) XZ(4) XZ(5) XZ(6) XZ(7) XZ(8) XZ(9) XZ(10) XZ(11) XZ(12)
 #undef XZ

@@ -242,7 +242,7 @@ __device__ __forceinline__ void row_offsets(const Geom& g, int row0, int lines, 
 // The half spectrum kx = 0..N of each real row from the complex transform Z
 // of its even/odd pairs (in LDS): X_k = (Z_k + conj Z_{N-k})/2
 // + w_k (Z_k - conj Z_{N-k})/(2i).
-template <typename T, int N, int LINES, int LDN, int NT>
+template <typename T, int N, int LINES, int LDN, int NT, bool NTS>
 __device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
                                            const typename CT<T>::t* sm,
                                            typename CT<T>::t* __restrict__ spec,
@@ -259,8 +259,8 @@ __device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
     const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
     const T2 wo = cmul(twr[k], o);
     T2* d = &spec[(long long)g.P * row + k];  // written once: non-temporal
-    __builtin_nontemporal_store(er + wo.x, &d->x);
-    __builtin_nontemporal_store(ei + wo.y, &d->y);
+    C3D_ST_ONCE(er + wo.x, &d->x);
+    C3D_ST_ONCE(ei + wo.y, &d->y);
   }
 }
 
@@ -269,7 +269,7 @@ __device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
 // ---------------------------------------------------------------------------
 // TIN: precision of the field read (the mixed-precision solve transforms an
 // FP64 right-hand side in FP32)
-template <typename T, int N, typename TIN = T>
+template <typename T, int N, typename TIN = T, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict__ in,
                                                    typename CT<T>::t* __restrict__ spec,
                                                    const typename CT<T>::t* __restrict__ tw,
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict
   };
   auto st = [&](int l, int m, const T2& v) { sm[l * LDN + m] = v; };
   stockham_io<T2, N, LINES, LDN, FNT, -1, false, true, 1>(sm, tw, ld, st);
-  x_fwd_post<T, N, LINES, LDN, FNT>(g, row0, sm, spec, twr);
+  x_fwd_post<T, N, LINES, LDN, FNT, NTS>(g, row0, sm, spec, twr);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,7 +313,7 @@ struct DivArgs {
   double* partials;
 };
 
-template <typename T, int N, bool PCORR>
+template <typename T, int N, bool PCORR, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
                                                        typename CT<T>::t* __restrict__ spec,
                                                        const typename CT<T>::t* __restrict__ tw,
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
   // block max of |div| (the transform's barriers order the wmax writes)
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
   if ((tid & 63) == 0) wmax[tid >> 6] = dm;
-  x_fwd_post<T, N, LINES, LDN, FNT>(g, row0, sm, spec, twr);
+  x_fwd_post<T, N, LINES, LDN, FNT, NTS>(g, row0, sm, spec, twr);
   __syncthreads();
   if (tid == 0) {
     double r = 0.0;
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
 // x inverse: spectral rows -> field rows (jj = 1..ny), times scale
 // (ACC: added to the field -- the correction of a refinement sweep)
 // ---------------------------------------------------------------------------
-template <typename T, int N, typename TOUT = T, bool ACC = false>
+template <typename T, int N, typename TOUT = T, bool ACC = false, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>::t* __restrict__ spec,
                                                    TOUT* __restrict__ out,
                                                    const typename CT<T>::t* __restrict__ tw,
@@ -424,8 +424,8 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
       const TO2 o = rp[m];
       rp[m] = mk<TO2>(o.x + (TOUT)z.x * scale, o.y + (TOUT)z.y * scale);
     } else {
-      __builtin_nontemporal_store((TOUT)z.x * scale, &rp[m].x);  // written once
-      __builtin_nontemporal_store((TOUT)z.y * scale, &rp[m].y);
+      C3D_ST_ONCE((TOUT)z.x * scale, &rp[m].x);  // written once
+      C3D_ST_ONCE((TOUT)z.y * scale, &rp[m].y);
     }
   };
   stockham_io<T2, N, LINES, LDN, FNT, 1, false, false, 1>(sm, tw, ld, st);
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
 // z, in place: TK consecutive kx of one y plane, all kz.  The first pass
 // reads and the last pass writes TK*16-byte runs (line-fastest butterflies).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S>
+template <typename T, int N, int S, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __restrict__ spec,
                                                const typename CT<T>::t* __restrict__ tw) {
   using T2 = typename CT<T>::t;
@@ -450,8 +450,8 @@ __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __rest
   auto st = [&](int l, int kz, const T2& v) {
     if (l < nl) {  // written once: non-temporal
       T2* d = &base[sk * kz + l];
-      __builtin_nontemporal_store(v.x, &d->x);
-      __builtin_nontemporal_store(v.y, &d->y);
+      C3D_ST_ONCE(v.x, &d->x);
+      C3D_ST_ONCE(v.y, &d->y);
     }
   };
   stockham_io<T2, N, TK, LDN, FNT, S, true, false, 1>(sm, tw, ld, st);
@@ -474,7 +474,7 @@ bool fft_size_supported(int n) {
   }
 }
 
-template <typename T>
+template <typename T, bool NTS>
 static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
@@ -482,7 +482,7 @@ static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
-    hipLaunchKernelGGL((k_fft_x_fwd<T, n_>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
+    hipLaunchKernelGGL((k_fft_x_fwd<T, n_, T, NTS>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
                        dim3(FNT), 0, c->stream, g, (const T*)in, (T2*)spec, (const T2*)c->tw_x, \
                        (const T2*)c->tw_xr);                                                     \
     break;
@@ -494,7 +494,7 @@ static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
   return C3D_OK;
 }
 
-template <typename T>
+template <typename T, bool NTS>
 static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void* pr,
                        double alpha, int* nblocks) {
   using T2 = typename CT<T>::t;
@@ -517,10 +517,10 @@ static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* u
     }                                                                                            \
     *nblocks = nb;                                                                               \
     if (pr)                                                                                      \
-      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, true>), dim3(nb), dim3(FNT), 0, c->stream, g,  \
+      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, true, NTS>), dim3(nb), dim3(FNT), 0, c->stream, g,  \
                          d, (T2*)c->spec, (const T2*)c->tw_x, (const T2*)c->tw_xr);              \
     else                                                                                         \
-      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, false>), dim3(nb), dim3(FNT), 0, c->stream, g, \
+      hipLaunchKernelGGL((k_fft_x_fwd_div<T, n_, false, NTS>), dim3(nb), dim3(FNT), 0, c->stream, g, \
                          d, (T2*)c->spec, (const T2*)c->tw_x, (const T2*)c->tw_xr);              \
     break;                                                                                       \
   }
@@ -532,7 +532,7 @@ static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* u
   return C3D_OK;
 }
 
-template <typename T>
+template <typename T, bool NTS>
 static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
@@ -540,7 +540,7 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
-    hipLaunchKernelGGL((k_fft_x_inv<T, n_>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
+    hipLaunchKernelGGL((k_fft_x_inv<T, n_, T, false, NTS>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
                        dim3(FNT), 0, c->stream, g, (const T2*)spec, (T*)out, (const T2*)c->tw_x, \
                        (const T2*)c->tw_xr, (T)scale);                                           \
     break;
@@ -552,7 +552,7 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
   return C3D_OK;
 }
 
-template <typename T>
+template <typename T, bool NTS>
 static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
@@ -561,10 +561,10 @@ static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
     if (inverse)                                                                               \
-      hipLaunchKernelGGL((k_fft_z<T, n_, 1>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,    \
+      hipLaunchKernelGGL((k_fft_z<T, n_, 1, NTS>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,    \
                          (const T2*)c->tw_z);                                                  \
     else                                                                                       \
-      hipLaunchKernelGGL((k_fft_z<T, n_, -1>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,   \
+      hipLaunchKernelGGL((k_fft_z<T, n_, -1, NTS>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,   \
                          (const T2*)c->tw_z);                                                  \
     break;
     C3D_FFT_SIZES(X)
@@ -667,7 +667,7 @@ int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, 
 int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec) {
   hipEvent_t ev;
   prof_begin(c, ST_FFT_X_FWD, &ev);
-  int rc = c->prec == C3D_FP64 ? fft_x_fwd_t<double>(c, in, spec) : fft_x_fwd_t<float>(c, in, spec);
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_fwd_t<double, true>(c, in, spec) : fft_x_fwd_t<double, false>(c, in, spec)) : (c->g.nt ? fft_x_fwd_t<float, true>(c, in, spec) : fft_x_fwd_t<float, false>(c, in, spec));
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_FFT_X_FWD, ev, 2.0 * N * c->esize);
@@ -678,8 +678,8 @@ int launch_prsrc_fft_x(c3d_ctx* c, const void* ux, const void* uy, const void* u
   hipEvent_t ev;
   prof_begin(c, ST_PRSRC_FFT, &ev);
   int nb = 0;
-  int rc = c->prec == C3D_FP64 ? prsrc_fft_t<double>(c, ux, uy, uz, pr, c->pmAlpha, &nb)
-                               : prsrc_fft_t<float>(c, ux, uy, uz, pr, c->pmAlpha, &nb);
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? prsrc_fft_t<double, true>(c, ux, uy, uz, pr, c->pmAlpha, &nb) : prsrc_fft_t<double, false>(c, ux, uy, uz, pr, c->pmAlpha, &nb))
+                               : (c->g.nt ? prsrc_fft_t<float, true>(c, ux, uy, uz, pr, c->pmAlpha, &nb) : prsrc_fft_t<float, false>(c, ux, uy, uz, pr, c->pmAlpha, &nb));
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_PRSRC_FFT, ev, (pr ? 6.0 : 4.0) * N * c->esize);
@@ -689,8 +689,8 @@ int launch_prsrc_fft_x(c3d_ctx* c, const void* ux, const void* uy, const void* u
 int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale) {
   hipEvent_t ev;
   prof_begin(c, ST_FFT_X_INV, &ev);
-  int rc = c->prec == C3D_FP64 ? fft_x_inv_t<double>(c, spec, out, scale)
-                               : fft_x_inv_t<float>(c, spec, out, scale);
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_inv_t<double, true>(c, spec, out, scale) : fft_x_inv_t<double, false>(c, spec, out, scale))
+                               : (c->g.nt ? fft_x_inv_t<float, true>(c, spec, out, scale) : fft_x_inv_t<float, false>(c, spec, out, scale));
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_FFT_X_INV, ev, 2.0 * N * c->esize);
@@ -701,7 +701,7 @@ int launch_fft_z(c3d_ctx* c, void* spec, int inverse) {
   hipEvent_t ev;
   const int st = inverse ? ST_FFT_Z_INV : ST_FFT_Z_FWD;
   prof_begin(c, st, &ev);
-  int rc = c->prec == C3D_FP64 ? fft_z_t<double>(c, spec, inverse) : fft_z_t<float>(c, spec, inverse);
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_z_t<double, true>(c, spec, inverse) : fft_z_t<double, false>(c, spec, inverse)) : (c->g.nt ? fft_z_t<float, true>(c, spec, inverse) : fft_z_t<float, false>(c, spec, inverse));
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, st, ev, 2.0 * N * c->esize);

@@ -15,6 +15,10 @@
 
 namespace c3d {
 
+// these kernels run mostly on grids that fit in the caches: ordinary
+// accesses for the once-touched words
+constexpr bool NTS = false;
+
 namespace {
 
 constexpr int LBX = 64, LBY = 4, LKCH = 8;
@@ -290,8 +294,10 @@ __global__ __launch_bounds__(LBX* LBY) void k_rhs_les(Geom g, Metrics<T> m, LesC
         const T gradp1 = (A(pr, i, jc, k) - A(pr, im, jc, k)) * rdx;
         fix_add(s1, (double)(A(ux, i, jc, k) * s1coe));
         const T convEd1 = -h11 - h12 - h13 + sgs1 + sgs2 + sgs3 + dcq13 + q.gx;
-        rx[c] = q.pmGamma * convEd1 + q.pmTheta * hx[c] - q.pmAlpha * gradp1 + twoBT * d22q1;
-        hx[c] = convEd1;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hx[c]);
+        C3D_ST_ONCE(q.pmGamma * convEd1 + q.pmTheta * hold_ - q.pmAlpha * gradp1 + twoBT * d22q1, &rx[c]);
+        C3D_ST_ONCE(convEd1, &hx[c]);
       }
       if (MASK & 2) {
         const T h21 = ((A(ux, ip, jc, k) + A(ux, ip, jm, k)) * (A(uy, ip, jc, k) + A(uy, i, jc, k)) -
@@ -318,8 +324,10 @@ __global__ __launch_bounds__(LBX* LBY) void k_rhs_les(Geom g, Metrics<T> m, LesC
         const T dcq13 = d11q2 + d33q2;
         const T gradp2 = (A(pr, i, jc, k) - A(pr, i, jm, k)) * sucac;
         const T convEd2 = -h21 - h22 - h23 + sgs1 + sgs2 + sgs3 + dcq13 + q.gy;
-        ry[c] = q.pmGamma * convEd2 + q.pmTheta * hy[c] - q.pmAlpha * gradp2 + twoBT * d22q2;
-        hy[c] = convEd2;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hy[c]);
+        C3D_ST_ONCE(q.pmGamma * convEd2 + q.pmTheta * hold_ - q.pmAlpha * gradp2 + twoBT * d22q2, &ry[c]);
+        C3D_ST_ONCE(convEd2, &hy[c]);
       }
       if (MASK & 4) {
         const T h31 = ((A(ux, ip, jc, k) + A(ux, ip, jc, km)) * (A(uz, ip, jc, k) + A(uz, i, jc, k)) -
@@ -346,8 +354,10 @@ __global__ __launch_bounds__(LBX* LBY) void k_rhs_les(Geom g, Metrics<T> m, LesC
         const T dcq13 = d11q3 + d33q3;
         const T gradp3 = (A(pr, i, jc, k) - A(pr, i, jc, km)) * rdz;
         const T convEd3 = -h31 - h32 - h33 + sgs1 + sgs2 + sgs3 + dcq13 + q.gz;
-        rz[c] = q.pmGamma * convEd3 + q.pmTheta * hz[c] - q.pmAlpha * gradp3 + twoBT * d22q3;
-        hz[c] = convEd3;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hz[c]);
+        C3D_ST_ONCE(q.pmGamma * convEd3 + q.pmTheta * hold_ - q.pmAlpha * gradp3 + twoBT * d22q3, &rz[c]);
+        C3D_ST_ONCE(convEd3, &hz[c]);
       }
     }
   }

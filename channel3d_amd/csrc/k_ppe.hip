@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_tridiag(Geom g, Metrics<T> m,
 // complex element: read, read again, write (48 B in FP64) instead of 80 B;
 // results equal k_tridiag's bit for bit.
 // ---------------------------------------------------------------------------
-template <typename T, int C>
+template <typename T, int C, bool NTS = false>
 __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T>& m,
                                                   typename CT2<T>::t* __restrict__ spec,
                                                   T* __restrict__ ckp) {
@@ -226,8 +226,8 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
           x.x = y[q].x - a1 * x.x;
           x.y = y[q].y - a1 * x.y;
         }
-        __builtin_nontemporal_store(x.x, &spec[base + P * j0].x);  // written once
-        __builtin_nontemporal_store(x.y, &spec[base + P * j0].y);
+        C3D_ST_ONCE(x.x, &spec[base + P * j0].x);  // written once
+        C3D_ST_ONCE(x.y, &spec[base + P * j0].y);
       }
     }
     arnext = ar[0];
@@ -237,11 +237,11 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
 // At least 4 waves per SIMD: the FP64 body fits 128 VGPRs without spilling
 // (136 unconstrained, 3 waves), and at 768x384x576 the 1008 workgroups then
 // run in one round instead of 1.3 -- 0.99 ms per solve instead of 1.21 ms.
-template <typename T, int C>
+template <typename T, int C, bool NTS = false>
 __global__ __launch_bounds__(256, 4) void k_tridiag_ck(Geom g, Metrics<T> m,
                                                     typename CT2<T>::t* __restrict__ spec,
                                                     T* __restrict__ ckp) {
-  k_tridiag_ck_body<T, C>(g, m, spec, ckp);
+  k_tridiag_ck_body<T, C, NTS>(g, m, spec, ckp);
 }
 
 template <typename T>
@@ -254,8 +254,12 @@ static int tridiag_t(c3d_ctx* c, void* spec) {
   // checkpoints: 3 reals per line and segment, in the spectral scratch
   const long long need = 3LL * ((g.ny + CSEG - 1) / CSEG) * g.P * g.nz;
   if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
-    hipLaunchKernelGGL((k_tridiag_ck<T, CSEG>), grid, blk, 0, c->stream, g, m,
-                       (typename CT2<T>::t*)spec, (T*)c->sarr);
+    if (g.nt)
+      hipLaunchKernelGGL((k_tridiag_ck<T, CSEG, true>), grid, blk, 0, c->stream, g, m,
+                         (typename CT2<T>::t*)spec, (T*)c->sarr);
+    else
+      hipLaunchKernelGGL((k_tridiag_ck<T, CSEG, false>), grid, blk, 0, c->stream, g, m,
+                         (typename CT2<T>::t*)spec, (T*)c->sarr);
   } else {
     hipLaunchKernelGGL(k_tridiag<T>, grid, blk, 0, c->stream, g, m,
                        (typename CT2<T>::t*)spec, (T*)c->sarr);

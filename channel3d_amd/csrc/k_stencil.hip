@@ -93,7 +93,7 @@ __device__ __forceinline__ void block_sum_fix(long long* v, long long (*sh)[4]) 
 // y neighbours of the current plane come from the tile's other threads
 // through the caches.  Expressions keep the reference's operation order.
 // ---------------------------------------------------------------------------
-template <typename T, int MASK, int BYR>
+template <typename T, int MASK, int BYR, bool NTS = false>
 __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
                                                 const T* __restrict__ ux, const T* __restrict__ uy,
                                                 const T* __restrict__ uz, const T* __restrict__ pr,
@@ -163,11 +163,11 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T conv = -h11 - h12 - h13 + cf.gx;
         // written and read once per step: non-temporal, so they do not evict
         // the neighbour rows the next tiles read from L2
-        const T hold = __builtin_nontemporal_load(&hx[c]);
-        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+        const T hold = C3D_LD_ONCE(&hx[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
                                         T(2) * cf.pmBeta * dcq,
                                     &rx[c]);
-        __builtin_nontemporal_store(conv, &hx[c]);
+        C3D_ST_ONCE(conv, &hx[c]);
       }
       if (MASK & 2) {
         const T h21 = ((I1 * ux[rowm + ip] + I2 * u_xp) * (v_xp + v_c) -
@@ -184,11 +184,11 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T conv = -h21 - h22 - h23 + cf.gy;
         // written and read once per step: non-temporal, so they do not evict
         // the neighbour rows the next tiles read from L2
-        const T hold = __builtin_nontemporal_load(&hy[c]);
-        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+        const T hold = C3D_LD_ONCE(&hy[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
                                         T(2) * cf.pmBeta * dcq,
                                     &ry[c]);
-        __builtin_nontemporal_store(conv, &hy[c]);
+        C3D_ST_ONCE(conv, &hy[c]);
       }
       if (MASK & 4) {
         const T h31 = ((u_xp + uxp_m) * (w_xp + w_c) - (u_c + u_m) * (w_c + w_xm)) * qdx1;
@@ -203,11 +203,11 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T conv = -h31 - h32 - h33 + cf.gz;
         // written and read once per step: non-temporal, so they do not evict
         // the neighbour rows the next tiles read from L2
-        const T hold = __builtin_nontemporal_load(&hz[c]);
-        __builtin_nontemporal_store(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
+        const T hold = C3D_LD_ONCE(&hz[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
                                         T(2) * cf.pmBeta * dcq,
                                     &rz[c]);
-        __builtin_nontemporal_store(conv, &hz[c]);
+        C3D_ST_ONCE(conv, &hz[c]);
       }
       // roll
       u_m = u_c; v_m = v_c; w_m = w_c; p_m = p_c;
@@ -389,7 +389,7 @@ __global__ void k_update_uy_ym(int n, T uybc, T* __restrict__ uy_ym, T* __restri
 // (m_TSchemeFIMP_inc.f90:1371-1386).  With ghost_from_interior the Neumann
 // ghost phi(0)=phi(1) of SetBC_and_UpdateHalo_pr is applied on the fly.
 // ---------------------------------------------------------------------------
-template <typename T, bool VEL, bool PRS, bool GHOST>
+template <typename T, bool VEL, bool PRS, bool GHOST, bool NTS = false>
 __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2, T rdzEta2,
                                                  T alpha, const T* __restrict__ phi,
                                                  T* __restrict__ ux, T* __restrict__ uy,
@@ -411,14 +411,14 @@ __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2
       const T phy = (GHOST && jc == 1) ? locphi : phi[c - g.nx];
       const T phz = phi[(long long)g.nx * jc + g.plane * km + i];
       // u, v, w and p are read and written once: non-temporal
-      __builtin_nontemporal_store(__builtin_nontemporal_load(&ux[c]) - (locphi - phx) * rdxEta2,
+      C3D_ST_ONCE(C3D_LD_ONCE(&ux[c]) - (locphi - phx) * rdxEta2,
                                   &ux[c]);
-      __builtin_nontemporal_store(__builtin_nontemporal_load(&uy[c]) - (locphi - phy) * sucacEta2,
+      C3D_ST_ONCE(C3D_LD_ONCE(&uy[c]) - (locphi - phy) * sucacEta2,
                                   &uy[c]);
-      __builtin_nontemporal_store(__builtin_nontemporal_load(&uz[c]) - (locphi - phz) * rdzEta2,
+      C3D_ST_ONCE(C3D_LD_ONCE(&uz[c]) - (locphi - phz) * rdzEta2,
                                   &uz[c]);
     }
-    if (PRS) __builtin_nontemporal_store(__builtin_nontemporal_load(&pr[c]) + locphi, &pr[c]);
+    if (PRS) C3D_ST_ONCE(C3D_LD_ONCE(&pr[c]) + locphi, &pr[c]);
   }
 }
 
@@ -582,8 +582,14 @@ static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const voi
   auto U = [](const void* p) { return (const T*)p; };
   auto W = [](void* p) { return (T*)p; };
 #define RHS_LAUNCH(M, R)                                                                       \
-  hipLaunchKernelGGL((k_rhs<T, M, R>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), U(uz), \
-                     U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials, kc)
+  if (g.nt)                                                                                    \
+    hipLaunchKernelGGL((k_rhs<T, M, R, true>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), \
+                       U(uz), U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials,  \
+                       kc);                                                                    \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_rhs<T, M, R, false>), grid, blk, 0, c->stream, g, m, cf, U(ux),       \
+                       U(uy), U(uz), U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz),          \
+                       c->d_partials, kc)
 #define RHS_ROWS_LAUNCH(M) RHS_LAUNCH(M, RHS_ROWS)
   switch (mask) {
     case 1: RHS_ROWS_LAUNCH(1); break;
@@ -740,8 +746,14 @@ static int proj_t(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, voi
   const T a = (T)c->pmAlpha;
   const T rdxE = (T)(c->rdx * c->pmAlpha), rdzE = (T)(c->rdz * c->pmAlpha);
 #define PROJ_LAUNCH(V, P, G)                                                                     \
-  hipLaunchKernelGGL((k_proj<T, V, P, G>), cell_grid(g), dim3(BX, BY), 0, c->stream, g,         \
-                     mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz, (T*)pr)
+  if (g.nt)                                                                                      \
+    hipLaunchKernelGGL((k_proj<T, V, P, G, true>), cell_grid(g), dim3(BX, BY), 0, c->stream, g, \
+                       mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,        \
+                       (T*)pr);                                                                  \
+  else                                                                                           \
+    hipLaunchKernelGGL((k_proj<T, V, P, G, false>), cell_grid(g), dim3(BX, BY), 0, c->stream,   \
+                       g, mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,     \
+                       (T*)pr)
   if (vel && prs && ghost) PROJ_LAUNCH(true, true, true);
   else if (vel && prs) PROJ_LAUNCH(true, true, false);
   else if (vel && ghost) PROJ_LAUNCH(true, false, true);

@@ -16,6 +16,10 @@
 
 namespace c3d {
 
+// these kernels run mostly on grids that fit in the caches: ordinary
+// accesses for the once-touched words
+constexpr bool NTS = false;
+
 constexpr int B4X = 64, B4Y = 4, K4CH = 4;
 
 __device__ __forceinline__ int wr(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -158,8 +162,10 @@ __global__ __launch_bounds__(B4X* B4Y, 2) void k_rhs4(Geom g, Metrics<T> m, Coef
         const T conv = -h11 - h12 - h13 + cf.xnu * dcq13 + cf.gx;
         const T gradp = cf.dx1 * pr(i - 2, jc, k) + cf.dx2 * pr(i - 1, jc, k) +
                         cf.dx3 * pr(i, jc, k) + cf.dx4 * pr(i + 1, jc, k);
-        rx[c] = cf.pmGamma * conv + cf.pmTheta * hx[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22;
-        hx[c] = conv;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hx[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold_ - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22, &rx[c]);
+        C3D_ST_ONCE(conv, &hx[c]);
       }
       // ---- RhsY ----
       {
@@ -183,8 +189,10 @@ __global__ __launch_bounds__(B4X* B4Y, 2) void k_rhs4(Geom g, Metrics<T> m, Coef
         const T dcq13 = d11 + d33;
         const T conv = -h21 - h22 - h23 + cf.xnu * dcq13 + cf.gy;
         const T gradp = (pr(i, jc, k) - pr(i, jm, k)) * sucac;
-        ry[c] = cf.pmGamma * conv + cf.pmTheta * hy[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22;
-        hy[c] = conv;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hy[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold_ - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22, &ry[c]);
+        C3D_ST_ONCE(conv, &hy[c]);
       }
       // ---- RhsZ ----
       {
@@ -207,8 +215,10 @@ __global__ __launch_bounds__(B4X* B4Y, 2) void k_rhs4(Geom g, Metrics<T> m, Coef
         const T conv = -h31 - h32 - h33 + cf.xnu * dcq13 + cf.gz;
         const T gradp = cf.dz1 * pr(i, jc, k - 2) + cf.dz2 * pr(i, jc, k - 1) +
                         cf.dz3 * pr(i, jc, k) + cf.dz4 * pr(i, jc, k + 1);
-        rz[c] = cf.pmGamma * conv + cf.pmTheta * hz[c] - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22;
-        hz[c] = conv;
+        // read and written once: non-temporal
+        const T hold_ = C3D_LD_ONCE(&hz[c]);
+        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold_ - cf.pmAlpha * gradp + T(2) * cf.pmBeta * d22, &rz[c]);
+        C3D_ST_ONCE(conv, &hz[c]);
       }
     }
   }
