@@ -1563,6 +1563,7 @@ extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
   }
   if (les_type == 0) {
     c->les_type = 0;
+    c->les_closure = nullptr;  // a learned SGS model belongs to an LES context
     return C3D_OK;
   }
   if (filter_type != 0 && filter_type != 1) {
