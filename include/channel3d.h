@@ -61,7 +61,7 @@ extern "C" {
 
 typedef struct c3d_ctx c3d_ctx;
 
-/* Inputs the reference keeps as module globals (SURVEY of m_Parameters.f90:
+/* Inputs the reference keeps as module globals (m_Parameters.f90:
  * 21-63, m_TScheme.f90:16-17, m_MeshAndMetries.f90:10-56). */
 typedef struct c3d_params {
   int nxc, nyc, nzc;           /* cells (nxc/2 and nzc must have FFT plans)   */

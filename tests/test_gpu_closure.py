@@ -164,7 +164,7 @@ def test_dsm_matches_oracle():
 
 
 def test_training_beats_dynamic_smagorinsky_a_priori():
-    """train on four DNS snapshots, judge on two later ones: correlation of
+    """train on four DNS snapshots, evaluate on two later ones: correlation of
     the predicted with the filtered-DNS stresses (tau_12 and the deviatoric
     tensor) against the dynamic Smagorinsky model's"""
     from oracle import closure as C
