@@ -85,6 +85,21 @@ def make_params(case: dict, precision: int = L.FP64) -> L.Params:
     return p
 
 
+class Batch:
+    """a batch of device fields in one allocation: base address and the
+    handle of each field"""
+
+    def __init__(self, base, fields):
+        self.base = base
+        self.fields = fields
+
+    def __len__(self):
+        return len(self.fields)
+
+    def __getitem__(self, q):
+        return self.fields[q]
+
+
 class Channel:
     def __init__(self, case: dict, yp=None, precision: int = L.FP64):
         self.lib = L.load()
@@ -146,7 +161,31 @@ class Channel:
         L.check(self.lib.c3d_field_alloc(self._h, layout, ctypes.byref(v)), "c3d_field_alloc")
         return v
 
+    def alloc_batch(self, layout, nbatch):
+        """one device allocation of nbatch fields of a layout, back to back;
+        returns the handles of its fields (field(q) addresses)"""
+        v = ctypes.c_void_p()
+        L.check(self.lib.c3d_field_alloc_batch(self._h, layout, int(nbatch), ctypes.byref(v)),
+                "c3d_field_alloc_batch")
+        fb = int(self.lib.c3d_field_bytes(self._h, layout))
+        return Batch(v, [ctypes.c_void_p(v.value + q * fb) for q in range(nbatch)])
+
+    def ppe_batch(self, prsrc, phi, sweeps=-1):
+        """nbatch pressure solves (Batch handles: INTERIOR sources, HALO phi)"""
+        assert len(prsrc) == len(phi)
+        L.check(self.lib.c3d_ppe_batch(self._h, len(prsrc), prsrc.base, phi.base, int(sweeps)),
+                "c3d_ppe_batch")
+
+    def helmholtz_batch(self, comp, ns, rhs, du, sweeps=-1):
+        """(1 - b Dxx)(1 - b Dzz)(1 - b Dyy) du = rhs of component comp (0..2)
+        at substep ns, for every field of the batch (INTERIOR layouts)"""
+        assert len(rhs) == len(du)
+        L.check(self.lib.c3d_helmholtz_batch(self._h, int(comp), int(ns), len(rhs), rhs.base,
+                                             du.base, int(sweeps)), "c3d_helmholtz_batch")
+
     def free(self, handle):
+        if isinstance(handle, Batch):
+            handle = handle.base
         L.check(self.lib.c3d_field_free(self._h, handle), "c3d_field_free")
 
     def upload(self, f, host, layout=None):

@@ -445,10 +445,17 @@ static void spike_table(const std::vector<double>& a, const std::vector<double>&
   for (int q = 0; q < R * R; ++q) mi[q] = (double)inv[q];
 }
 
+// per substep: x taps (nx), z taps (nz), then ten y vectors of ny+2: the
+// cell operator's a, vecm, arr, the node operator's a, vecm, arr, the cell
+// operator's b and c, the node operator's b and c
+static size_t adi_table_len(const Geom& g) {
+  return (size_t)g.nx + (size_t)g.nz + 10 * ((size_t)g.ny + 2);
+}
+
 static int build_adi_tables(c3d_ctx* c) {
   const int ny = c->g.ny, nx = c->g.nx, nz = c->g.nz;
   const size_t L = (size_t)ny + 2;
-  const size_t per_ns = (size_t)nx + (size_t)nz + 6 * L;
+  const size_t per_ns = adi_table_len(c->g);
   std::vector<double> host(3 * per_ns, 0.0);
   const int M = c->spike_m;
   const size_t spl = M > 0 ? (size_t)spike_table_len(M) : 0;
@@ -483,7 +490,10 @@ static int build_adi_tables(c3d_ctx* c) {
     std::memcpy(yb + 0 * L, a.data(), sizeof(double) * L);
     std::memcpy(yb + 1 * L, vecm.data(), sizeof(double) * L);
     std::memcpy(yb + 2 * L, arr.data(), sizeof(double) * L);
+    std::memcpy(yb + 6 * L, b.data(), sizeof(double) * L);
+    std::memcpy(yb + 7 * L, cc.data(), sizeof(double) * L);
     for (int q = 0; q < 3; ++q) tb.d_yc[q] = db + nx + nz + q * L;
+    for (int q = 3; q < 5; ++q) tb.d_yc[q] = db + nx + nz + (q + 3) * L;
     // y, uy: wall row then node metrics (:700-712)
     cc[1] = 0.0; b[1] = 1.0; a[1] = 0.0;
     for (int j = 2; j <= ny; ++j) {
@@ -499,7 +509,10 @@ static int build_adi_tables(c3d_ctx* c) {
     std::memcpy(yb + 3 * L, a.data(), sizeof(double) * L);
     std::memcpy(yb + 4 * L, vecm.data(), sizeof(double) * L);
     std::memcpy(yb + 5 * L, arr.data(), sizeof(double) * L);
+    std::memcpy(yb + 8 * L, b.data(), sizeof(double) * L);
+    std::memcpy(yb + 9 * L, cc.data(), sizeof(double) * L);
     for (int q = 0; q < 3; ++q) tb.d_yp[q] = db + nx + nz + (3 + q) * L;
+    for (int q = 3; q < 5; ++q) tb.d_yp[q] = db + nx + nz + (q + 5) * L;
   }
   C3D_HIP(hipMemcpyAsync(c->d_tabs, host.data(), sizeof(double) * host.size(),
                          hipMemcpyHostToDevice, c->stream));
@@ -636,16 +649,24 @@ static size_t layout_elems(const c3d_ctx* c, int layout) {
 
 // the captured steps bake kernel arguments (dt-derived coefficients, solver
 // forms, LES tables): every setter that changes one drops them
-static void drop_graphs(c3d_ctx* c) {
+void c3d::drop_graphs(c3d_ctx* c) {
   for (auto& x : c->gexec) {
     if (x) hipGraphExecDestroy(x);
     x = nullptr;
   }
 }
 
+// A write into the state velocity outside c3d_step: the CFL number the last
+// step's closing pass computed for the next step no longer describes it.
+static void state_written(c3d_ctx* c, const void* dptr) {
+  if (dptr && (dptr == c->state[C3D_UX] || dptr == c->state[C3D_UY] || dptr == c->state[C3D_UZ]))
+    c->cfl_valid = false;
+}
+
 extern "C" int c3d_destroy(c3d_ctx* c) {
   if (!c) return C3D_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->les_closure) closure_detach(c->les_closure, c);
   for (auto& q : c->pending) {
     hipEventDestroy(q.a);
     hipEventDestroy(q.b);
@@ -658,7 +679,8 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
   for (void* p : c->user_fields) hipFree(p);
   void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
                   c->d_dyp, c->d_spike, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
-                  c->d_lesmet, c->d_spec1d, c->spec2, c->d_spec2d};
+                  c->d_lesmet, c->d_spec1d, c->spec2, c->d_spec2d, c->cln_d, c->cln_f,
+                  c->cln_fb, c->bspec, c->bsarr, c->bmean, c->bres, c->btmp};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (c->staging) hipHostFree(c->staging);
@@ -732,8 +754,18 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   g.P = ((g.nk + 1) + 7) / 8 * 8;
   g.nspec = (long long)g.P * g.ny * g.nz;
   // non-temporal once-touched words above 64 MiB per field (the L2s hold
-  // 32 MiB, the last-level cache 256 MiB: smaller grids stay cached)
+  // 32 MiB, the last-level cache 256 MiB: smaller grids stay cached);
+  // C3D_NT=0 / C3D_NT=1 forces either kernel variant (tests of both forms)
   g.nt = (double)g.nfield * (p->precision == C3D_FP64 ? 8 : 4) >= 64.0 * 1024 * 1024;
+  if (const char* nt = std::getenv("C3D_NT")) {
+    if (std::strcmp(nt, "0") == 0) g.nt = 0;
+    else if (std::strcmp(nt, "1") == 0) g.nt = 1;
+    else {
+      delete c;
+      set_error("c3d_create: C3D_NT must be 0 or 1");
+      return C3D_ERR_ARG;
+    }
+  }
   compute_metrics(c, yp);
   // m_Parameters.f90:135-150
   if (p->ischeme == 1) {
@@ -787,8 +819,7 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   }
   // implicit-operator tables
   {
-    const size_t per_ns = (size_t)g.nx + (size_t)g.nz + 6 * ((size_t)g.ny + 2);
-    if ((rc = dalloc(c, &c->d_tabs, 3 * per_ns * sizeof(double)))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_tabs, 3 * adi_table_len(g) * sizeof(double)))) return fail(rc);
     // y solver: Thomas with segment checkpoints (default), Thomas storing the
     // forward pass (C3D_Y_SOLVER=store), or the partitioned solve
     // (C3D_Y_SOLVER=partitioned, when a row count per lane fits the registers)
@@ -941,6 +972,7 @@ extern "C" int c3d_field_copy(c3d_ctx* c, int layout, void* dst, const void* src
     set_error("c3d_field_copy: bad arguments");
     return C3D_ERR_ARG;
   }
+  state_written(c, dst);
   const size_t bytes = layout_elems(c, layout) * c->esize;
   const long long n16 = (long long)(bytes / 16);
   if (n16 > 0)
@@ -1065,6 +1097,7 @@ extern "C" int c3d_upload(c3d_ctx* c, int layout, void* dptr, const double* host
     set_error("c3d_upload: bad arguments");
     return C3D_ERR_ARG;
   }
+  state_written(c, dptr);
   const size_t n = layout_elems(c, layout);
   int rc = ensure_staging(c, n * c->esize);
   if (rc) return rc;
@@ -1178,18 +1211,21 @@ extern "C" int c3d_clcRhsXYZ(c3d_ctx* c, const void* ux, const void* uy, const v
 
 extern "C" int c3d_clcU1Hat(c3d_ctx* c, void* ux, void* rx) {
   CHECK_CTX(c);
+  state_written(c, ux);
   if (c->les_type) return launch_les_y(c, 0, ux, rx, nullptr, 0);
   return launch_adi(c, 0, c->cur_ns, ux, rx, nullptr, 0);
 }
 
 extern "C" int c3d_clcU2Hat(c3d_ctx* c, void* uy, void* ry, const void* duy_ym) {
   CHECK_CTX(c);
+  state_written(c, uy);
   if (c->les_type) return launch_les_y(c, 1, uy, ry, duy_ym, 0);
   return launch_adi(c, 1, c->cur_ns, uy, ry, duy_ym, 0);
 }
 
 extern "C" int c3d_clcU3Hat(c3d_ctx* c, void* uz, void* rz) {
   CHECK_CTX(c);
+  state_written(c, uz);
   if (c->les_type) return launch_les_y(c, 2, uz, rz, nullptr, 0);
   return launch_adi(c, 2, c->cur_ns, uz, rz, nullptr, 0);
 }
@@ -1197,6 +1233,7 @@ extern "C" int c3d_clcU3Hat(c3d_ctx* c, void* uz, void* rz) {
 extern "C" int c3d_SetBC_and_UpdateHaloForPrSrc(c3d_ctx* c, void* ux, void* uy, void* uz,
                                                 const void* uy_ym) {
   CHECK_CTX(c);
+  for (const void* f : {ux, uy, uz}) state_written(c, f);
   return c->order == 4 ? launch_bc4_vel(c, ux, uy, uz, uy_ym, 0)
                        : launch_bc_vel(c, ux, uy, uz, uy_ym, 0);
 }
@@ -1209,6 +1246,7 @@ extern "C" int c3d_SetBC_and_UpdateHalo_pr(c3d_ctx* c, void* pr) {
 extern "C" int c3d_SetBC_and_UpdateHalo(c3d_ctx* c, void* ux, void* uy, void* uz,
                                         const void* uy_ym) {
   CHECK_CTX(c);
+  for (const void* f : {ux, uy, uz}) state_written(c, f);
   return c->order == 4 ? launch_bc4_vel(c, ux, uy, uz, uy_ym, 1)
                        : launch_bc_vel(c, ux, uy, uz, uy_ym, 1);
 }
@@ -1243,8 +1281,88 @@ extern "C" int c3d_ppe(c3d_ctx* c, void* prsrc, void* phi) {
   return run_ppe(c, prsrc, phi);
 }
 
+// ===========================================================================
+// batched solves
+// ===========================================================================
+int c3d::ensure_batch(c3d_ctx* c, int nb, bool mixed) {
+  const Geom& g = c->g;
+  if (nb > c->bcap) {
+    C3D_HIP(hipStreamSynchronize(c->stream));
+    for (void** p : {&c->bspec, &c->bsarr, (void**)&c->bmean, &c->bres, &c->btmp})
+      if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+      }
+    c->bres_ok = false;
+    c->bcap = 0;
+    int rc;
+    if ((rc = dalloc(c, &c->bspec, (size_t)nb * g.nspec * 2 * c->esize))) return rc;
+    if ((rc = dalloc(c, &c->bsarr, (size_t)nb * g.nspec * c->esize))) return rc;
+    if ((rc = dalloc(c, (void**)&c->bmean, (size_t)nb * batch_mean_len(g) * sizeof(double))))
+      return rc;
+    C3D_HIP(hipMemset(c->bmean, 0, (size_t)nb * batch_mean_len(g) * sizeof(double)));
+    if ((rc = dalloc(c, &c->btmp, (size_t)nb * g.nfield * c->esize))) return rc;
+    c->bcap = nb;
+  }
+  if (mixed && !c->bres_ok) {
+    int rc;
+    if ((rc = dalloc(c, &c->bres, (size_t)c->bcap * g.nfield * sizeof(float)))) return rc;
+    // rows outside 1..ny are never written by the residual kernels
+    C3D_HIP(hipMemset(c->bres, 0, (size_t)c->bcap * g.nfield * sizeof(float)));
+    c->bres_ok = true;
+  }
+  return C3D_OK;
+}
+
+extern "C" size_t c3d_field_bytes(c3d_ctx* c, int layout) {
+  if (!c || layout < C3D_HALO || layout > C3D_WALL) return 0;
+  return layout_elems(c, layout) * c->esize;
+}
+
+extern "C" int c3d_field_alloc_batch(c3d_ctx* c, int layout, int nbatch, void** dptr) {
+  if (!c || !dptr || layout < C3D_HALO || layout > C3D_WALL || nbatch < 1) {
+    set_error("c3d_field_alloc_batch: bad arguments");
+    return C3D_ERR_ARG;
+  }
+  const size_t bytes = (size_t)nbatch * layout_elems(c, layout) * c->esize;
+  int rc = dalloc(c, dptr, bytes);
+  if (rc) return rc;
+  C3D_HIP(hipMemset(*dptr, 0, bytes));
+  c->user_fields.push_back(*dptr);
+  return C3D_OK;
+}
+
+static int batch_args(c3d_ctx* c, const char* what, int nbatch, const void* in, void* out,
+                      int sweeps) {
+  if (!c || !in || !out || nbatch < 1 || sweeps < -1 || sweeps > 8) {
+    set_error(std::string(what) + ": bad arguments (nbatch >= 1, sweeps -1..8)");
+    return C3D_ERR_ARG;
+  }
+  if (c->prec != C3D_FP64 || c->order != 2) {
+    set_error(std::string(what) + ": needs an FP64 2nd-order context");
+    return C3D_ERR_ARG;
+  }
+  return C3D_OK;
+}
+
+extern "C" int c3d_ppe_batch(c3d_ctx* c, int nbatch, const void* prsrc, void* phi, int sweeps) {
+  if (int rc = batch_args(c, "c3d_ppe_batch", nbatch, prsrc, phi, sweeps)) return rc;
+  return run_ppe_batch(c, nbatch, prsrc, phi, sweeps);
+}
+
+extern "C" int c3d_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nbatch, const void* rhs,
+                                   void* du, int sweeps) {
+  if (int rc = batch_args(c, "c3d_helmholtz_batch", nbatch, rhs, du, sweeps)) return rc;
+  if (comp < 0 || comp > 2 || ns < 1 || ns > c->iadvance) {
+    set_error("c3d_helmholtz_batch: comp must be 0..2 and ns 1..iadvance");
+    return C3D_ERR_ARG;
+  }
+  return run_helmholtz_batch(c, comp, ns, nbatch, rhs, du, sweeps);
+}
+
 extern "C" int c3d_FluidVelUpdate(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz) {
   CHECK_CTX(c);
+  for (const void* f : {ux, uy, uz}) state_written(c, f);
   if (c->order == 4) return launch_proj4(c, phi, ux, uy, uz, nullptr, 1, 0, 0);
   return launch_proj(c, phi, ux, uy, uz, nullptr, 1, 0, 0);
 }
@@ -1368,6 +1486,7 @@ static int substep_les(c3d_ctx* c, int ns) {
 
 extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   CHECK_CTX(c);
+  c->cfl_valid = false;  // step_body validates it again after its last substep
   if (c->order == 4) return substep4(c, ns);
   if (c->les_type) return substep_les(c, ns);
   int rc;
@@ -1547,6 +1666,8 @@ static void fill_lesmet(c3d_ctx* c, std::vector<T>& h, LesMet<T>& lm, T* d) {
 
 extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
   CHECK_CTX(c);
+  // a replayed step may still run: finish it before its graph goes
+  C3D_HIP(hipStreamSynchronize(c->stream));
   drop_graphs(c);
   if (c->order != 2) {
     set_error("c3d_set_les: the LES layer belongs to the 2nd-order solver");
@@ -1563,7 +1684,9 @@ extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
   }
   if (les_type == 0) {
     c->les_type = 0;
-    c->les_closure = nullptr;  // a learned SGS model belongs to an LES context
+    // a learned SGS model belongs to an LES context
+    if (c->les_closure) closure_detach(c->les_closure, c);
+    c->les_closure = nullptr;
     return C3D_OK;
   }
   if (filter_type != 0 && filter_type != 1) {

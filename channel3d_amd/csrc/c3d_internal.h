@@ -160,6 +160,16 @@ struct LesMet {
 
 enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1, YS_PARTITIONED = 2 };
 
+// A batch of n fields solved by one launch per pass: field q lies `field`
+// reals after field 0, its spectrum `spec` complex elements after (FP64
+// solves) or `spec32` complex FP32 elements after (FP32 solves), its
+// checkpoints `ck` reals after, its (0,0)-mode work `mean` doubles after.
+// The default is one field.
+struct Batch {
+  int n = 1;
+  long long field = 0, spec = 0, spec32 = 0, ck = 0, mean = 0;
+};
+
 enum Stage {
   ST_RHS = 0, ST_RHS1, ST_BULK, ST_ADI_X, ST_ADI_Z, ST_ADI_XZ, ST_ADI_Y, ST_BC, ST_PRSRC,
   ST_FFT_X_FWD, ST_FFT_Z_FWD, ST_TRIDIAG, ST_FFT_Z_INV, ST_FFT_X_INV, ST_PROJ,
@@ -202,8 +212,9 @@ struct c3d_ctx {
     int xlo, xn, zlo, zn;     // tap offset range and count
     double* d_xtaps;          // device, xn entries (fp64 copy)
     double* d_ztaps;
-    double* d_yc[3];          // CN ux/uz operator: a, vecm, arr (ny+2 each)
-    double* d_yp[3];          // uy operator
+    double* d_yc[5];          // CN ux/uz operator: a, vecm, arr, then the matrix's
+                              // diagonal b and upper c (ny+2 each)
+    double* d_yp[5];          // uy operator
     double* d_spk[2];         // partitioned-solve tables (cell / node operator)
   } adi[3];
   void* d_tabs = nullptr;
@@ -247,12 +258,31 @@ struct c3d_ctx {
   double* d_mean64 = nullptr; // the FP64 (0,0)-mode solve: 3 ny work, gauge value at
                               // [3 ny], gauge partial sums from [3 (ny+1)]
 
+  // batched solves (c3d_ppe_batch, c3d_helmholtz_batch), sized for bcap
+  // fields on first use: spectra (FP64 complex, nspec each), checkpoints
+  // (nspec reals each), (0,0)-mode work (batch_mean_len doubles each), FP32
+  // residual fields and field-shaped scratch (nfield each)
+  int bcap = 0;
+  bool bres_ok = false;
+  void* bspec = nullptr;
+  void* bsarr = nullptr;
+  double* bmean = nullptr;
+  void* bres = nullptr;
+  void* btmp = nullptr;
+
   // LES layer (c3d_set_les): LES_type (0 off, 2 WALE, 3 MTS), FilterType and
   // its weights, the metric block; the eddy viscosity is state[C3D_NUT]
   int les_type = 0;
   int filter_type = 0;
   // learned closure replacing the WALE / MTS viscosity (c3d_set_les_closure)
+  // and the scratch of its viscosity evaluation for this context's grid:
+  // LES-row tables and filter / feature work (doubles), network inputs and
+  // outputs (floats), the fallback counter; allocated when a closure is first
+  // attached, freed by c3d_destroy
   struct c3d_closure* les_closure = nullptr;
+  double* cln_d = nullptr;
+  float* cln_f = nullptr;
+  int* cln_fb = nullptr;
   double les_fcoe[9] = {};
   void* d_lesmet = nullptr;
   c3d::LesMet<double> lmd{};
@@ -359,6 +389,10 @@ int launch_sgs(c3d_ctx* c, const void* ux, const void* uy, const void* uz, void*
 // nut from the learned closure, dynamic Smagorinsky where it backscatters
 int closure_nut(struct c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy,
                 const void* uz, void* nut);
+// forget an LES context that uses the closure (detach, LES off, destroy)
+void closure_detach(struct c3d_closure* cl, c3d_ctx* c);
+// drop the captured step graphs of a context (c3d_api.hip)
+void drop_graphs(c3d_ctx* c);
 int launch_rhs_les(c3d_ctx* c, int mask, const void* ux, const void* uy, const void* uz,
                    const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz,
                    int want_bulk);
@@ -374,10 +408,11 @@ void fmt_es24_15(double v, char out[32]);
 
 // ---- launchers (k_fft.hip / k_ppe.hip) ----
 bool fft_size_supported(int n);
-int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec);
-int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale);
-int launch_fft_z(c3d_ctx* c, void* spec, int inverse);
-int launch_tridiag(c3d_ctx* c, void* spec);
+int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec, const Batch& b = Batch());
+int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale,
+                     const Batch& b = Batch());
+int launch_fft_z(c3d_ctx* c, void* spec, int inverse, const Batch& b = Batch());
+int launch_tridiag(c3d_ctx* c, void* spec, void* ck, const Batch& b = Batch());
 int run_ppe(c3d_ctx* c, void* prsrc, void* phi);
 // the pressure source (with the -nu/2 div pressure correction when pr is
 // given) fused into the x forward pass of the FP64-solve pressure equation;
@@ -388,8 +423,20 @@ int run_ppe_from_spectrum(c3d_ctx* c, void* phi);
 // FP32 transforms of the mixed-precision solve
 enum { FFT32_X_FWD_F64, FFT32_X_FWD_F32, FFT32_Z_FWD, FFT32_Z_INV, FFT32_X_INV_SET,
        FFT32_X_INV_ADD };
-int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale);
+int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale,
+                 const Batch& b = Batch());
 // FP32 twiddle and metric tables of an FP64 context (c3d_api.hip)
 int ensure_mixed_tables(c3d_ctx* c);
+// (0,0)-mode work of one field of a mixed-precision solve: 3 (ny+1) doubles,
+// then the gauge partial sums
+inline long long batch_mean_len(const Geom& g) {
+  return 3LL * (g.ny + 1) + (long long)((g.nx + 63) / 64) * g.nz;
+}
+// batch scratch for nb fields (mixed: also the FP32 residual fields)
+int ensure_batch(c3d_ctx* c, int nb, bool mixed);
+int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps);
+// (1 - b Dxx)(1 - b Dzz)(1 - b Dyy) du = rhs for nb fields (k_adi.hip)
+int run_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nb, const void* rhs, void* du,
+                        int sweeps);
 
 }  // namespace c3d

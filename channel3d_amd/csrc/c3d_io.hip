@@ -228,6 +228,10 @@ extern "C" int c3d_read_restart(c3d_ctx* c, const char* path, void* ux, void* uy
   FILE* fp = std::fopen(path, "rb");
   if (!fp) return io_fail("c3d_read_restart: cannot open file", path);
   C3D_HIP(hipStreamSynchronize(c->stream));
+  // a new state velocity: the CFL of the next step must be computed from it
+  for (int q = 0; q < 3; ++q)
+    if (f[q] == c->state[C3D_UX] || f[q] == c->state[C3D_UY] || f[q] == c->state[C3D_UZ])
+      c->cfl_valid = false;
   const double ucrf = c->order == 4 ? c->p.u_crf : 0.0;
   const int nf = want_hist ? 7 : 4;
   for (int q = 0; q < nf && !rc; ++q) rc = read_field(c, r, pb, fp, f[q], q == 0 ? ucrf : 0.0, path);

@@ -81,16 +81,20 @@ __global__ __launch_bounds__(ABX* ABY) void k_conv_z(Geom g, const double* __res
 // lead-in), every output element written once.
 constexpr int MAXMX = 32;
 
-template <typename T, int MZ, bool ADD, bool NTS = false>
+// TIN: type of the right-hand side read (an FP32 solve of an FP64 right-hand
+// side); batches: blockIdx.y selects the field, bf reals apart in f and out.
+template <typename T, int MZ, bool ADD, bool NTS = false, typename TIN = T>
 __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restrict__ xt, int mx,
                                                  const double* __restrict__ zt,
-                                                 const T* __restrict__ f, T* __restrict__ out,
+                                                 const TIN* __restrict__ f, T* __restrict__ out,
                                                  const DevScalars* __restrict__ sc, int kchunk,
-                                                 int gxb, int gyb) {
+                                                 int gxb, int gyb, long long bf) {
   // the convolutions replace the reference's periodic Thomas solves, so their
   // rounding differs from it regardless: fused multiply-adds are allowed here
 #pragma clang fp contract(fast)
   constexpr int NZT = 2 * MZ + 1;
+  f += bf * blockIdx.y;
+  out += bf * blockIdx.y;
   __shared__ T sm[4][64 + 2 * MAXMX];
   const int nbl = gridDim.x;
   const int lb = xcd_remap(blockIdx.x, nbl);
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(256) void k_conv_xz(Geom g, const double* __restric
   auto load = [&](int kk, T (&v)[2]) {
     const long long row = (long long)nx * jl + g.plane * kk;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) v[q] = f[row + ix[q]];
+    for (int q = 0; q < 2; ++q) v[q] = (T)f[row + ix[q]];
   };
   // One plane: stage the row loaded one step earlier in LDS (plus the
   // constant-flow-rate shift), reuse its registers for the next plane's load
@@ -279,19 +283,29 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 // the right-hand side is read twice, u once, u written once (4 words per
 // point instead of 5); the results equal k_adi_y's bit for bit.
 // ---------------------------------------------------------------------------
-template <typename T, int C, bool WALLROW, bool SHIFT, bool NTS = false>
+// WALL: 0 no wall row, 1 row 1 is the wall condition x = duy, 2 the wall
+// condition x = 0 (a solve for the increment alone).  SET: u = x instead of
+// u += x, with u of type TU (an FP32 solve writing an FP64 increment).
+// Batches: blockIdx.z selects the field, bf reals apart in r and u.
+template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
+          typename TU = T>
 __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
                                                   const double* __restrict__ tvecm,
                                                   const double* __restrict__ tarr,
-                                                  const T* __restrict__ r, T* __restrict__ u,
+                                                  const T* __restrict__ r, TU* __restrict__ u,
                                                   const T* __restrict__ duy,
-                                                  const DevScalars* __restrict__ sc) {
+                                                  const DevScalars* __restrict__ sc,
+                                                  long long bf) {
   extern __shared__ unsigned char ck_raw[];
   T* ck = (T*)ck_raw;  // [segment][256 threads]
   const int tid = threadIdx.x + 64 * threadIdx.y;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.nx || k >= g.nz) return;
+  r += bf * blockIdx.z;
+  u += bf * blockIdx.z;
+  constexpr bool WALLROW = WALL != 0;
+  auto wall = [&]() -> T { return WALL == 1 ? duy[i + (long long)g.nx * k] : T(0); };
   const int ny = g.ny;
   const int nseg = (ny + C - 1) / C;
   const long long base = (long long)i + g.plane * k;
@@ -308,7 +322,7 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
       const int j = lo + q;
       y[q] = j <= ny ? ld(j) : T(0);
     }
-    if (s == 0 && WALLROW) y[0] = duy[i + (long long)g.nx * k];
+    if (s == 0 && WALLROW) y[0] = wall();
     if (s == 0) {
       f = y[0] / (T)tvecm[1];
       y[0] = f;
@@ -332,7 +346,7 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
 #pragma unroll
       for (int q = 0; q < C; ++q) y[q] = ld(lo + q);  // a full segment: lo + C - 1 < ny
       if (s == 0) {
-        if (WALLROW) y[0] = duy[i + (long long)g.nx * k];
+        if (WALLROW) y[0] = wall();
         fp = y[0] / (T)tvecm[1];
         y[0] = fp;
       }
@@ -344,17 +358,17 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
         }
       }
     }
-    T uv[C];
+    TU uv[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      uv[q] = j <= ny ? C3D_LD_ONCE(&u[base + sj * j]) : T(0);  // once
+      uv[q] = (!SET && j <= ny) ? C3D_LD_ONCE(&u[base + sj * j]) : TU(0);  // once
     }
 #pragma unroll
     for (int q = C - 1; q >= 0; --q) {
       const int j = lo + q;
       if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
-      if (j <= ny) C3D_ST_ONCE(uv[q] + x, &u[base + sj * j]);
+      if (j <= ny) C3D_ST_ONCE(SET ? (TU)x : uv[q] + (TU)x, &u[base + sj * j]);
     }
   }
 }
@@ -478,14 +492,14 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
     auto launch = [&](auto ntc) {
       constexpr bool NT_ = decltype(ntc)::value;
       if (comp == 1)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, true, false, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 1, false, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc, 0LL);
       else if (shift)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, true, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, true, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc, 0LL);
       else
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, false, false, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, false, NT_>), gy, by, ck_bytes, c->stream, g,
+                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc, 0LL);
     };
     if (g.nt) launch(std::true_type{});
     else launch(std::false_type{});
@@ -538,11 +552,11 @@ static bool launch_xz_fused(c3d_ctx* c, const c3d_ctx::AdiTab& tb, const void* r
     if (g.nt)                                                                                  \
       hipLaunchKernelGGL((k_conv_xz<T, M, ADD, true>), grid, blk, 0, c->stream, g, tb.d_xtaps, \
                          mx, tb.d_ztaps, (const T*)rhs, (T*)c->tmpA,                           \
-                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb);                        \
+                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb, 0LL);                   \
     else                                                                                       \
       hipLaunchKernelGGL((k_conv_xz<T, M, ADD, false>), grid, blk, 0, c->stream, g,            \
                          tb.d_xtaps, mx, tb.d_ztaps, (const T*)rhs, (T*)c->tmpA,               \
-                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb);                        \
+                         (const DevScalars*)c->d_sc, kchunk, gxb, gyb, 0LL);                   \
     break;
     XZ(0) XZ(1) XZ(2) XZ(3) XZ(4) XZ(5) XZ(6) XZ(7) XZ(8) XZ(9) XZ(10) XZ(11) XZ(12)
 #undef XZ
@@ -611,6 +625,188 @@ int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy
   }
   return c->prec == C3D_FP64 ? adi_t<double>(c, comp, ns, u, rhs, duy_ym, add_dpmdxns)
                              : adi_t<float>(c, comp, ns, u, rhs, duy_ym, add_dpmdxns);
+}
+
+// ===========================================================================
+// Batched Helmholtz solves (1 - b Dxx)(1 - b Dzz)(1 - b Dyy) du = rhs, the
+// operator of clcU1/2/3Hat_FIMP_010 (m_TSchemeFIMP_inc.f90:263-347, 636-729,
+// 1035-1128) on nb right-hand sides at once, du written (not added).  FP64:
+// the kernels of the per-routine solve.  Mixed precision: the x/z
+// convolutions and the y sweeps in FP32 on the FP64 right-hand side, then
+// refinement sweeps with the FP64 residual of the factored operator.
+// ===========================================================================
+
+// r = rhs - (1 - b Dxx)(1 - b Dzz) M du, in FP64, stored as TR.  M is the y
+// matrix the sweeps invert: rows 1..ny, diagonal bm(j), lower am(j) (j > 1),
+// upper cm(j) (j < ny).  WALL: row 1 is the wall condition du = 0, its
+// residual is 0.  One thread per cell; the 27 neighbours of a cell come from
+// the caches.  blockIdx.z = q nz + k for field q (bf reals apart).
+template <typename TR, bool WALL>
+__global__ __launch_bounds__(256) void k_helm_resid(Geom g, const double* __restrict__ am,
+                                                    const double* __restrict__ bm,
+                                                    const double* __restrict__ cm, double wxc,
+                                                    double wxo, double wzc, double wzo,
+                                                    const double* __restrict__ rhs,
+                                                    const double* __restrict__ du,
+                                                    TR* __restrict__ r, long long bf) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int jc = blockIdx.y * 4 + threadIdx.y + 1;
+  const int q = blockIdx.z / g.nz;
+  const int k = blockIdx.z - q * g.nz;
+  if (i >= g.nx || jc > g.ny) return;
+  rhs += bf * q;
+  du += bf * q;
+  r += bf * q;
+  const long long c0 = fidx(g, i, jc, k);
+  if (WALL && jc == 1) {
+    r[c0] = TR(0);
+    return;
+  }
+  const int im = i == 0 ? g.nx - 1 : i - 1, ip = i == g.nx - 1 ? 0 : i + 1;
+  const int km = k == 0 ? g.nz - 1 : k - 1, kp = k == g.nz - 1 ? 0 : k + 1;
+  const double a = jc > 1 ? am[jc] : 0.0, b = bm[jc], cc = jc < g.ny ? cm[jc] : 0.0;
+  const int ii[3] = {im, i, ip};
+  const int kk[3] = {km, k, kp};
+  double acc = 0.0;
+#pragma unroll
+  for (int dk = 0; dk < 3; ++dk) {
+    double zs = 0.0;
+#pragma unroll
+    for (int di = 0; di < 3; ++di) {
+      const long long p = fidx(g, ii[di], jc, kk[dk]);
+      double my = b * du[p];
+      if (jc > 1) my += a * du[p - g.nx];
+      if (jc < g.ny) my += cc * du[p + g.nx];
+      zs += (di == 1 ? wxc : wxo) * my;
+    }
+    acc += (dk == 1 ? wzc : wzo) * zs;
+  }
+  r[c0] = (TR)(rhs[c0] - acc);
+}
+
+template <typename T, typename TIN>
+static int xz_batch(c3d_ctx* c, const c3d_ctx::AdiTab& tb, const void* rhs, void* out, int nb,
+                    long long bf) {
+  const Geom& g = c->g;
+  const int mx = -tb.xlo, mz = -tb.zlo;
+  const int gxb = (g.nx + 63) / 64, gyb = (g.ny + 3) / 4;
+  int nch = (4096 + gxb * gyb - 1) / (gxb * gyb);
+  nch = nch < 1 ? 1 : nch;
+  int kchunk = (g.nz + nch - 1) / nch;
+  if (kchunk < 4 * mz + 8) kchunk = 4 * mz + 8;
+  if (kchunk > g.nz) kchunk = g.nz;
+  const int gzb = (g.nz + kchunk - 1) / kchunk;
+  const dim3 grid(gxb * gyb * gzb, nb), blk(64, 4);
+  const DevScalars* sc = (const DevScalars*)c->d_sc;
+  switch (mz) {
+#define XZB(M)                                                                                  \
+  case M:                                                                                       \
+    if (g.nt)                                                                                   \
+      hipLaunchKernelGGL((k_conv_xz<T, M, false, true, TIN>), grid, blk, 0, c->stream, g,       \
+                         tb.d_xtaps, mx, tb.d_ztaps, (const TIN*)rhs, (T*)out, sc, kchunk, gxb, \
+                         gyb, bf);                                                              \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_conv_xz<T, M, false, false, TIN>), grid, blk, 0, c->stream, g,      \
+                         tb.d_xtaps, mx, tb.d_ztaps, (const TIN*)rhs, (T*)out, sc, kchunk, gxb, \
+                         gyb, bf);                                                              \
+    break;
+    XZB(0) XZB(1) XZB(2) XZB(3) XZB(4) XZB(5) XZB(6) XZB(7) XZB(8) XZB(9) XZB(10) XZB(11) XZB(12)
+#undef XZB
+    default: set_error("helmholtz_batch: z kernel wider than the fused window"); return C3D_ERR_UNSUPPORTED;
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+// y sweeps of a batch: rhs (T, the convolved right-hand side) -> du (double),
+// SET: du = x, else du += x
+template <typename T, bool SET>
+static int y_batch(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, const void* rhs, void* du,
+                   int nb, long long bf) {
+  const Geom& g = c->g;
+  constexpr int CSEG = 32;
+  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * 256 * sizeof(T);
+  if (ck_bytes > 64 * 1024) {
+    set_error("helmholtz_batch: too many rows for the checkpointed sweep");
+    return C3D_ERR_UNSUPPORTED;
+  }
+  const dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4, nb), by(64, 4);
+  double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
+  const DevScalars* sc = (const DevScalars*)c->d_sc;
+  auto launch = [&](auto ntc, auto wallc) {
+    constexpr bool NT_ = decltype(ntc)::value;
+    constexpr int W_ = decltype(wallc)::value;
+    hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, W_, false, NT_, SET, double>), gy, by, ck_bytes,
+                       c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (double*)du,
+                       (const T*)nullptr, sc, bf);
+  };
+  using W0 = std::integral_constant<int, 0>;
+  using W2 = std::integral_constant<int, 2>;
+  if (g.nt) {
+    if (comp == 1) launch(std::true_type{}, W2{});
+    else launch(std::true_type{}, W0{});
+  } else {
+    if (comp == 1) launch(std::false_type{}, W2{});
+    else launch(std::false_type{}, W0{});
+  }
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int run_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nb, const void* rhs, void* du,
+                        int sweeps) {
+  int rc;
+  if ((rc = ensure_batch(c, nb, sweeps >= 0))) return rc;
+  const Geom& g = c->g;
+  const auto& tb = c->adi[ns - 1];
+  if (!xz_fusable(g, tb)) {
+    set_error("c3d_helmholtz_batch: the x/z Green's functions are wider than the fused "
+              "convolution's window (use the per-routine clcU*Hat)");
+    return C3D_ERR_UNSUPPORTED;
+  }
+  const long long bf = g.nfield;
+  const double N = (double)g.nx * g.ny * g.nz * nb;
+  hipEvent_t ev;
+  if (sweeps < 0) {
+    prof_begin(c, ST_ADI_XZ, &ev);
+    if ((rc = xz_batch<double, double>(c, tb, rhs, c->btmp, nb, bf))) return rc;
+    prof_end(c, ST_ADI_XZ, ev, 2.0 * N * 8);
+    prof_begin(c, ST_ADI_Y, &ev);
+    if ((rc = y_batch<double, true>(c, tb, comp, c->btmp, du, nb, bf))) return rc;
+    prof_end(c, ST_ADI_Y, ev, 2.0 * N * 8);
+    return C3D_OK;
+  }
+  prof_begin(c, ST_ADI_XZ, &ev);
+  if ((rc = xz_batch<float, double>(c, tb, rhs, c->btmp, nb, bf))) return rc;
+  prof_end(c, ST_ADI_XZ, ev, N * (8 + 4));
+  prof_begin(c, ST_ADI_Y, &ev);
+  if ((rc = y_batch<float, true>(c, tb, comp, c->btmp, du, nb, bf))) return rc;
+  prof_end(c, ST_ADI_Y, ev, N * (4 + 8));
+  const double beta = 0.5 * c->aconst[ns - 1] * c->p.xnu * c->dt;
+  const double wxo = -beta * c->rdx2, wxc = beta * c->rdx2 * 2.0 + 1.0;
+  const double wzo = -beta * c->rdz2, wzc = beta * c->rdz2 * 2.0 + 1.0;
+  double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
+  for (int s = 0; s < sweeps; ++s) {
+    prof_begin(c, ST_RESID, &ev);
+    const dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nb), blk(64, 4);
+    if (comp == 1)
+      hipLaunchKernelGGL((k_helm_resid<float, true>), grid, blk, 0, c->stream, g, yt[0], yt[3],
+                         yt[4], wxc, wxo, wzc, wzo, (const double*)rhs, (const double*)du,
+                         (float*)c->bres, bf);
+    else
+      hipLaunchKernelGGL((k_helm_resid<float, false>), grid, blk, 0, c->stream, g, yt[0], yt[3],
+                         yt[4], wxc, wxo, wzc, wzo, (const double*)rhs, (const double*)du,
+                         (float*)c->bres, bf);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_RESID, ev, N * (8 + 8 + 4));
+    prof_begin(c, ST_ADI_XZ, &ev);
+    if ((rc = xz_batch<float, float>(c, tb, c->bres, c->btmp, nb, bf))) return rc;
+    prof_end(c, ST_ADI_XZ, ev, 2.0 * N * 4);
+    prof_begin(c, ST_ADI_Y, &ev);
+    if ((rc = y_batch<float, false>(c, tb, comp, c->btmp, du, nb, bf))) return rc;
+    prof_end(c, ST_ADI_Y, ev, N * (4 + 16));
+  }
+  return C3D_OK;
 }
 
 }  // namespace c3d

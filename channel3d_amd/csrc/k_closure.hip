@@ -683,12 +683,10 @@ using namespace c3d;
 
 struct c3d_closure {
   hipStream_t stream = nullptr;
-  // scratch of the LES viscosity (closure_nut), sized on first use
-  double* nut_d = nullptr;
-  float* nut_f = nullptr;
-  int* nut_fb = nullptr;
-  long long nut_cap = 0;
-  long long nfallback = 0;
+  // LES contexts that take their viscosity from this network, and the one
+  // that evaluated it last (its fallback count is c3d_closure_fallbacks's)
+  std::vector<c3d_ctx*> users;
+  c3d_ctx* last = nullptr;
   long long cap = 0, n = 0;        // dataset capacity and size (samples)
   float* X = nullptr;              // cap x 16
   float* T = nullptr;              // cap x 6
@@ -790,12 +788,27 @@ extern "C" int c3d_closure_create(long long max_samples, c3d_closure** out) {
   return C3D_OK;
 }
 
+// The weights are read by the viscosity evaluations queued on the streams of
+// the attached contexts: an update waits for them first.  Every update entry
+// point synchronises its own stream before returning, so an evaluation
+// queued after it returns sees the new weights.
+static int sync_users(c3d_closure* cl) {
+  for (c3d_ctx* c : cl->users) C3D_HIP(hipStreamSynchronize(c->stream));
+  return C3D_OK;
+}
+
 extern "C" int c3d_closure_destroy(c3d_closure* cl) {
   if (!cl) return C3D_OK;
+  // contexts still attached fall back to their WALE / MTS model
+  sync_users(cl);
+  for (c3d_ctx* c : cl->users) {
+    drop_graphs(c);
+    c->les_closure = nullptr;
+  }
+  cl->users.clear();
   if (cl->stream) hipStreamSynchronize(cl->stream);
   void* bufs[] = {cl->X, cl->T, cl->S, cl->w, cl->m, cl->v, cl->g, cl->loss, cl->fwd, cl->bwd,
-                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y, cl->nut_d, cl->nut_f,
-                  cl->nut_fb};
+                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (cl->stream) hipStreamDestroy(cl->stream);
@@ -805,6 +818,7 @@ extern "C" int c3d_closure_destroy(c3d_closure* cl) {
 
 extern "C" int c3d_closure_set_params(c3d_closure* cl, const float* host) {
   if (!cl || !host) return C3D_ERR_ARG;
+  if (int rc = sync_users(cl)) return rc;
   C3D_HIP(hipMemcpyAsync(cl->w, host, sizeof(float) * NPARAM, hipMemcpyHostToDevice, cl->stream));
   // a new network starts a new optimisation
   for (float* p : {cl->m, cl->v}) {
@@ -964,6 +978,7 @@ extern "C" int c3d_closure_train(c3d_closure* cl, long long first, long long cou
   }
   int rc = cl_range(cl, first, count, "c3d_closure_train");
   if (rc) return rc;
+  if ((rc = sync_users(cl))) return rc;
   if (batch > count) batch = (int)count;
   if (cl->loss_cap < steps) {
     if (cl->loss) hipFree(cl->loss);
@@ -1010,6 +1025,7 @@ extern "C" int c3d_closure_train(c3d_closure* cl, long long first, long long cou
 // one Adam step with a given gradient (host), for checking the optimiser
 extern "C" int c3d_closure_adam_step(c3d_closure* cl, const float* grad, float lr) {
   if (!cl || !grad || !(lr > 0)) return C3D_ERR_ARG;
+  if (int rc = sync_users(cl)) return rc;
   C3D_HIP(hipMemcpyAsync(cl->g, grad, sizeof(float) * NPARAM, hipMemcpyHostToDevice, cl->stream));
   ++cl->adam_t;
   const float c1 = 1.0f - std::pow(0.9f, (float)cl->adam_t);
@@ -1111,38 +1127,39 @@ extern "C" int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* c, int fx, int fy, int 
 // The learned closure as the SGS viscosity of an LES context: features of
 // every LES cell from the context's own velocity (the grid is the filter:
 // blocks of one cell), the network's stress, its eddy viscosity, dynamic
-// Smagorinsky where it backscatters.  Runs on the context's stream.
-// scratch of closure_nut for the context's grid, allocated when the closure
-// is attached so that the steps themselves never allocate (a step may be
-// captured into a graph)
-static int nut_prepare(c3d_closure* cl, c3d_ctx* c) {
+// Smagorinsky where it backscatters.  Runs on the context's stream, with the
+// context's own scratch: contexts sharing one network never share work
+// memory, and each keeps its own grid's y tables.
+// Scratch layout (c->cln_d, doubles): LES-row centres yc (ny), heights dY
+// (ny), the nodes yp (ny+3), then the filter / feature work: fb (9 nl),
+// w (22 nl), pr (6 nl), lm, mm (nl each), Cs^2 (ny); c->cln_f (floats):
+// X (16 nl), T (6 nl), S (nl), Y (6 nl).
+static long long cln_doubles(const Geom& g) {
+  const long long nl = (long long)g.nx * g.ny * g.nz;
+  return 2LL * g.ny + g.ny + 3 + 39 * nl + g.ny;
+}
+
+// allocated when the closure is attached, so that the steps themselves never
+// allocate (a step may be captured into a graph)
+static int nut_prepare(c3d_ctx* c) {
+  if (c->cln_d) return C3D_OK;
   const Geom& g = c->g;
   const int nJ = g.ny;
   const long long nl = (long long)g.nx * g.ny * g.nz;
   int rc;
-  if (cl->nut_cap < nl) {
-    for (void* p : {(void*)cl->nut_d, (void*)cl->nut_f, (void*)cl->nut_fb})
-      if (p) hipFree(p);
-    cl->nut_d = nullptr;
-    cl->nut_f = nullptr;
-    cl->nut_fb = nullptr;
-    const long long nd = 2 * nJ + g.ny + 3 + 9 * nl + 22 * nl + 6 * nl + 2 * nl + nJ;
-    if ((rc = cl_alloc((void**)&cl->nut_d, sizeof(double) * nd))) return rc;
-    if ((rc = cl_alloc((void**)&cl->nut_f, sizeof(float) * (NPAD + NOUT + 1 + NOUT) * nl)))
-      return rc;
-    if ((rc = cl_alloc((void**)&cl->nut_fb, sizeof(int)))) return rc;
-    // the LES rows are the context's cells: heights and centres never change
-    const auto& yp = c->met[C3D_M_YP];
-    std::vector<double> host(2 * nJ + g.ny + 3);
-    for (int J = 0; J < nJ; ++J) {
-      host[nJ + J] = yp[J + 2] - yp[J + 1];
-      host[J] = 0.5 * (yp[J + 2] + yp[J + 1]);
-    }
-    for (int j = 0; j < g.ny + 3; ++j) host[2 * nJ + j] = yp[j];
-    C3D_HIP(hipMemcpy(cl->nut_d, host.data(), sizeof(double) * host.size(),
-                      hipMemcpyHostToDevice));
-    cl->nut_cap = nl;
+  if ((rc = cl_alloc((void**)&c->cln_d, sizeof(double) * cln_doubles(g)))) return rc;
+  if ((rc = cl_alloc((void**)&c->cln_f, sizeof(float) * (NPAD + NOUT + 1 + NOUT) * nl))) return rc;
+  if ((rc = cl_alloc((void**)&c->cln_fb, sizeof(int)))) return rc;
+  C3D_HIP(hipMemset(c->cln_fb, 0, sizeof(int)));
+  // the LES rows are the context's cells: heights and centres never change
+  const auto& yp = c->met[C3D_M_YP];
+  std::vector<double> host(2 * nJ + g.ny + 3);
+  for (int J = 0; J < nJ; ++J) {
+    host[nJ + J] = yp[J + 2] - yp[J + 1];
+    host[J] = 0.5 * (yp[J + 2] + yp[J + 1]);
   }
+  for (int j = 0; j < g.ny + 3; ++j) host[2 * nJ + j] = yp[j];
+  C3D_HIP(hipMemcpy(c->cln_d, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice));
   return C3D_OK;
 }
 
@@ -1151,11 +1168,11 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
   const Geom& g = c->g;
   const int nI = g.nx, nJ = g.ny, nK = g.nz;
   const long long nl = (long long)nI * nJ * nK;
-  if (cl->nut_cap < nl) {
+  if (!c->cln_d) {
     set_error("closure_nut: closure not prepared for this context (c3d_set_les_closure)");
     return C3D_ERR_ARG;
   }
-  double* dyc = cl->nut_d;
+  double* dyc = c->cln_d;
   double* ddY = dyc + nJ;
   double* dyp = ddY + nJ;
   double* fb = dyp + g.ny + 3;
@@ -1164,7 +1181,7 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
   double* lm = pr + 6 * nl;
   double* mm = lm + nl;
   double* dcs = mm + nl;
-  float* X = cl->nut_f;
+  float* X = c->cln_f;
   float* T = X + NPAD * nl;
   float* S = T + NOUT * nl;
   float* Y = S + nl;
@@ -1187,12 +1204,23 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
   const long long ntile = (nl + 15) / 16;
   hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(4096, ntile)), dim3(64), 0,
                      st, (const float*)X, nl, cl->fwd, cl->w, Y);
-  hipLaunchKernelGGL(k_cl_zero_int, dim3(1), dim3(1), 0, st, cl->nut_fb);
+  hipLaunchKernelGGL(k_cl_zero_int, dim3(1), dim3(1), 0, st, c->cln_fb);
   hipLaunchKernelGGL(k_cl_nut, dim3(grid), dim3(blk), 0, st, g, (int)nl, (const float*)Y,
                      (const float*)S, (const double*)w, (const double*)dcs, (const double*)ddY,
-                     dX, dZ, c->p.xnu, c->p.bc[2], c->p.bc[3], (double*)nut, cl->nut_fb);
+                     dX, dZ, c->p.xnu, c->p.bc[2], c->p.bc[3], (double*)nut, c->cln_fb);
   C3D_HIP(hipGetLastError());
+  cl->last = c;
   return C3D_OK;
+}
+
+void c3d::closure_detach(c3d_closure* cl, c3d_ctx* c) {
+  if (!cl) return;
+  for (size_t q = 0; q < cl->users.size(); ++q)
+    if (cl->users[q] == c) {
+      cl->users.erase(cl->users.begin() + q);
+      break;
+    }
+  if (cl->last == c) cl->last = nullptr;
 }
 
 extern "C" int c3d_set_les_closure(c3d_ctx* c, c3d_closure* cl) {
@@ -1203,25 +1231,27 @@ extern "C" int c3d_set_les_closure(c3d_ctx* c, c3d_closure* cl) {
   }
   C3D_HIP(hipStreamSynchronize(c->stream));
   if (cl) {
-    int rc = nut_prepare(cl, c);
+    int rc = nut_prepare(c);
     if (rc) return rc;
   }
   // graphs of c3d_step baked the previous SGS model's kernels
-  for (auto& x : c->gexec) {
-    if (x) hipGraphExecDestroy(x);
-    x = nullptr;
+  drop_graphs(c);
+  if (c->les_closure != cl) {
+    if (c->les_closure) closure_detach(c->les_closure, c);
+    if (cl) cl->users.push_back(c);
   }
   c->les_closure = cl;
   return C3D_OK;
 }
 
-// cells that fell back to dynamic Smagorinsky in the last closure_nut
+// cells that fell back to dynamic Smagorinsky in the last viscosity
+// evaluation of an attached context (0 when none ran)
 extern "C" int c3d_closure_fallbacks(c3d_closure* cl, long long* n) {
   if (!cl || !n) return C3D_ERR_ARG;
   int h = 0;
-  if (cl->nut_fb) {
-    C3D_HIP(hipDeviceSynchronize());
-    C3D_HIP(hipMemcpy(&h, cl->nut_fb, sizeof(int), hipMemcpyDeviceToHost));
+  if (cl->last) {
+    C3D_HIP(hipStreamSynchronize(cl->last->stream));
+    C3D_HIP(hipMemcpy(&h, cl->last->cln_fb, sizeof(int), hipMemcpyDeviceToHost));
   }
   *n = h;
   return C3D_OK;

@@ -269,14 +269,19 @@ __device__ __forceinline__ void x_fwd_post(const Geom& g, int row0,
 // ---------------------------------------------------------------------------
 // TIN: precision of the field read (the mixed-precision solve transforms an
 // FP64 right-hand side in FP32)
+// Batches: field blockIdx.y of a batch starts bin reals (the input) and bsp
+// complex elements (the spectrum) after the first; 0 for a single field.
 template <typename T, int N, typename TIN = T, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_fwd(Geom g, const TIN* __restrict__ in,
                                                    typename CT<T>::t* __restrict__ spec,
                                                    const typename CT<T>::t* __restrict__ tw,
-                                                   const typename CT<T>::t* __restrict__ twr) {
+                                                   const typename CT<T>::t* __restrict__ twr,
+                                                   long long bin, long long bsp) {
   using T2 = typename CT<T>::t;
   using TI2 = typename CT<TIN>::t;
   constexpr int LINES = xlines(N);
+  in += bin * blockIdx.y;
+  spec += bsp * blockIdx.y;
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
   __shared__ long long rb[LINES];
@@ -387,10 +392,12 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
                                                    TOUT* __restrict__ out,
                                                    const typename CT<T>::t* __restrict__ tw,
                                                    const typename CT<T>::t* __restrict__ twr,
-                                                   TOUT scale) {
+                                                   TOUT scale, long long bsp, long long bout) {
   using T2 = typename CT<T>::t;
   using TO2 = typename CT<TOUT>::t;
   constexpr int LINES = xlines(N);
+  spec += bsp * blockIdx.y;
+  out += bout * blockIdx.y;
   constexpr int LDN = N + 1;
   __shared__ T2 sm[LINES * LDN];
   __shared__ long long rb[LINES];
@@ -437,9 +444,11 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
 // ---------------------------------------------------------------------------
 template <typename T, int N, int S, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __restrict__ spec,
-                                               const typename CT<T>::t* __restrict__ tw) {
+                                               const typename CT<T>::t* __restrict__ tw,
+                                               long long bsp) {
   using T2 = typename CT<T>::t;
   constexpr int TK = ZTK;
+  spec += bsp * blockIdx.z;
   constexpr int LDN = N + 1;
   __shared__ T2 sm[TK * LDN];
   const int kx0 = blockIdx.x * TK;
@@ -475,16 +484,17 @@ bool fft_size_supported(int n) {
 }
 
 template <typename T, bool NTS>
-static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec) {
+static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec, const Batch& b) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
-    hipLaunchKernelGGL((k_fft_x_fwd<T, n_, T, NTS>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
-                       dim3(FNT), 0, c->stream, g, (const T*)in, (T2*)spec, (const T2*)c->tw_x, \
-                       (const T2*)c->tw_xr);                                                     \
+    hipLaunchKernelGGL((k_fft_x_fwd<T, n_, T, NTS>),                                             \
+                       dim3((nrows + xlines(n_) - 1) / xlines(n_), b.n), dim3(FNT), 0, c->stream, \
+                       g, (const T*)in, (T2*)spec, (const T2*)c->tw_x, (const T2*)c->tw_xr,      \
+                       b.field, b.spec);                                                         \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -533,16 +543,17 @@ static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* u
 }
 
 template <typename T, bool NTS>
-static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
+static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale, const Batch& b) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
-    hipLaunchKernelGGL((k_fft_x_inv<T, n_, T, false, NTS>), dim3((nrows + xlines(n_) - 1) / xlines(n_)),       \
-                       dim3(FNT), 0, c->stream, g, (const T2*)spec, (T*)out, (const T2*)c->tw_x, \
-                       (const T2*)c->tw_xr, (T)scale);                                           \
+    hipLaunchKernelGGL((k_fft_x_inv<T, n_, T, false, NTS>),                                      \
+                       dim3((nrows + xlines(n_) - 1) / xlines(n_), b.n), dim3(FNT), 0, c->stream, \
+                       g, (const T2*)spec, (T*)out, (const T2*)c->tw_x, (const T2*)c->tw_xr,     \
+                       (T)scale, b.spec, b.field);                                               \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -553,19 +564,19 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale) {
 }
 
 template <typename T, bool NTS>
-static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
+static int fft_z_t(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
-  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny);
+  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny, b.n);
   switch (g.nz) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
     if (inverse)                                                                               \
       hipLaunchKernelGGL((k_fft_z<T, n_, 1, NTS>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,    \
-                         (const T2*)c->tw_z);                                                  \
+                         (const T2*)c->tw_z, b.spec);                                          \
     else                                                                                       \
       hipLaunchKernelGGL((k_fft_z<T, n_, -1, NTS>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,   \
-                         (const T2*)c->tw_z);                                                  \
+                         (const T2*)c->tw_z, b.spec);                                          \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -577,15 +588,16 @@ static int fft_z_t(c3d_ctx* c, void* spec, int inverse) {
 
 // ---- FP32 transforms of the mixed-precision solve (FP32 twiddle tables) --
 template <typename TIN>
-static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec) {
+static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec, const Batch& b) {
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
-    hipLaunchKernelGGL((k_fft_x_fwd<float, n_, TIN>), dim3((nrows + xlines(n_) - 1) / xlines(n_)), \
-                       dim3(FNT), 0, c->stream, g, in, (float2*)spec, (const float2*)c->tw32_x,   \
-                       (const float2*)c->tw32_xr);                                               \
+    hipLaunchKernelGGL((k_fft_x_fwd<float, n_, TIN>),                                            \
+                       dim3((nrows + xlines(n_) - 1) / xlines(n_), b.n), dim3(FNT), 0, c->stream, \
+                       g, in, (float2*)spec, (const float2*)c->tw32_x,                           \
+                       (const float2*)c->tw32_xr, b.field, b.spec32);                            \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -596,16 +608,16 @@ static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec) {
 }
 
 template <bool ACC>
-static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale) {
+static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale, const Batch& b) {
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
   switch (g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
     hipLaunchKernelGGL((k_fft_x_inv<float, n_, double, ACC>),                                   \
-                       dim3((nrows + xlines(n_) - 1) / xlines(n_)), dim3(FNT), 0, c->stream, g,  \
-                       (const float2*)spec, out, (const float2*)c->tw32_x,                       \
-                       (const float2*)c->tw32_xr, scale);                                        \
+                       dim3((nrows + xlines(n_) - 1) / xlines(n_), b.n), dim3(FNT), 0, c->stream, \
+                       g, (const float2*)spec, out, (const float2*)c->tw32_x,                    \
+                       (const float2*)c->tw32_xr, scale, b.spec32, b.field);                     \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -615,18 +627,18 @@ static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale) 
   return C3D_OK;
 }
 
-static int fft_z32(c3d_ctx* c, void* spec, int inverse) {
+static int fft_z32(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   const Geom& g = c->g;
-  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny);
+  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny, b.n);
   switch (g.nz) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
     if (inverse)                                                                               \
       hipLaunchKernelGGL((k_fft_z<float, n_, 1>), grid, dim3(FNT), 0, c->stream, g,           \
-                         (float2*)spec, (const float2*)c->tw32_z);                             \
+                         (float2*)spec, (const float2*)c->tw32_z, b.spec32);                   \
     else                                                                                       \
       hipLaunchKernelGGL((k_fft_z<float, n_, -1>), grid, dim3(FNT), 0, c->stream, g,          \
-                         (float2*)spec, (const float2*)c->tw32_z);                             \
+                         (float2*)spec, (const float2*)c->tw32_z, b.spec32);                   \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -636,9 +648,10 @@ static int fft_z32(c3d_ctx* c, void* spec, int inverse) {
   return C3D_OK;
 }
 
-int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale) {
+int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, double scale,
+                 const Batch& b) {
   hipEvent_t ev;
-  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz * b.n;
   int rc = C3D_OK, st = ST_MISC;
   double bytes = 0;
   switch (what) {
@@ -651,12 +664,12 @@ int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, 
   }
   prof_begin(c, st, &ev);
   switch (what) {
-    case FFT32_X_FWD_F64: rc = fft_x_fwd32<double>(c, (const double*)in, spec); break;
-    case FFT32_X_FWD_F32: rc = fft_x_fwd32<float>(c, (const float*)in, spec); break;
-    case FFT32_Z_FWD: rc = fft_z32(c, spec, 0); break;
-    case FFT32_Z_INV: rc = fft_z32(c, spec, 1); break;
-    case FFT32_X_INV_SET: rc = fft_x_inv32<false>(c, spec, out, scale); break;
-    case FFT32_X_INV_ADD: rc = fft_x_inv32<true>(c, spec, out, scale); break;
+    case FFT32_X_FWD_F64: rc = fft_x_fwd32<double>(c, (const double*)in, spec, b); break;
+    case FFT32_X_FWD_F32: rc = fft_x_fwd32<float>(c, (const float*)in, spec, b); break;
+    case FFT32_Z_FWD: rc = fft_z32(c, spec, 0, b); break;
+    case FFT32_Z_INV: rc = fft_z32(c, spec, 1, b); break;
+    case FFT32_X_INV_SET: rc = fft_x_inv32<false>(c, spec, out, scale, b); break;
+    case FFT32_X_INV_ADD: rc = fft_x_inv32<true>(c, spec, out, scale, b); break;
     default: set_error("fft32: bad selector"); return C3D_ERR_ARG;
   }
   if (rc) return rc;
@@ -664,12 +677,12 @@ int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, 
   return C3D_OK;
 }
 
-int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec) {
+int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec, const Batch& b) {
   hipEvent_t ev;
   prof_begin(c, ST_FFT_X_FWD, &ev);
-  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_fwd_t<double, true>(c, in, spec) : fft_x_fwd_t<double, false>(c, in, spec)) : (c->g.nt ? fft_x_fwd_t<float, true>(c, in, spec) : fft_x_fwd_t<float, false>(c, in, spec));
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_fwd_t<double, true>(c, in, spec, b) : fft_x_fwd_t<double, false>(c, in, spec, b)) : (c->g.nt ? fft_x_fwd_t<float, true>(c, in, spec, b) : fft_x_fwd_t<float, false>(c, in, spec, b));
   if (rc) return rc;
-  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz * b.n;
   prof_end(c, ST_FFT_X_FWD, ev, 2.0 * N * c->esize);
   return C3D_OK;
 }
@@ -686,24 +699,24 @@ int launch_prsrc_fft_x(c3d_ctx* c, const void* ux, const void* uy, const void* u
   return launch_max_finalize(c, nb, 0);
 }
 
-int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale) {
+int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale, const Batch& b) {
   hipEvent_t ev;
   prof_begin(c, ST_FFT_X_INV, &ev);
-  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_inv_t<double, true>(c, spec, out, scale) : fft_x_inv_t<double, false>(c, spec, out, scale))
-                               : (c->g.nt ? fft_x_inv_t<float, true>(c, spec, out, scale) : fft_x_inv_t<float, false>(c, spec, out, scale));
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_x_inv_t<double, true>(c, spec, out, scale, b) : fft_x_inv_t<double, false>(c, spec, out, scale, b))
+                               : (c->g.nt ? fft_x_inv_t<float, true>(c, spec, out, scale, b) : fft_x_inv_t<float, false>(c, spec, out, scale, b));
   if (rc) return rc;
-  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz * b.n;
   prof_end(c, ST_FFT_X_INV, ev, 2.0 * N * c->esize);
   return C3D_OK;
 }
 
-int launch_fft_z(c3d_ctx* c, void* spec, int inverse) {
+int launch_fft_z(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   hipEvent_t ev;
   const int st = inverse ? ST_FFT_Z_INV : ST_FFT_Z_FWD;
   prof_begin(c, st, &ev);
-  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_z_t<double, true>(c, spec, inverse) : fft_z_t<double, false>(c, spec, inverse)) : (c->g.nt ? fft_z_t<float, true>(c, spec, inverse) : fft_z_t<float, false>(c, spec, inverse));
+  int rc = c->prec == C3D_FP64 ? (c->g.nt ? fft_z_t<double, true>(c, spec, inverse, b) : fft_z_t<double, false>(c, spec, inverse, b)) : (c->g.nt ? fft_z_t<float, true>(c, spec, inverse, b) : fft_z_t<float, false>(c, spec, inverse, b));
   if (rc) return rc;
-  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz * b.n;
   prof_end(c, st, ev, 2.0 * N * c->esize);
   return C3D_OK;
 }

@@ -21,9 +21,9 @@ template <>
 struct CT2<float> { using t = float2; };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_tridiag(Geom g, Metrics<T> m,
-                                                 typename CT2<T>::t* __restrict__ spec,
-                                                 T* __restrict__ sarr) {
+__device__ __forceinline__ void k_tridiag_body(const Geom& g, const Metrics<T>& m,
+                                               typename CT2<T>::t* __restrict__ spec,
+                                               T* __restrict__ sarr) {
   using T2 = typename CT2<T>::t;
   const int kx = blockIdx.x * 64 + threadIdx.x;
   const int kz = blockIdx.y * 4 + threadIdx.y;
@@ -237,17 +237,28 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
 // At least 4 waves per SIMD: the FP64 body fits 128 VGPRs without spilling
 // (136 unconstrained, 3 waves), and at 768x384x576 the 1008 workgroups then
 // run in one round instead of 1.3 -- 0.99 ms per solve instead of 1.21 ms.
+// Batches: blockIdx.z selects the field; its spectrum starts bsp complex
+// elements, its checkpoints bck reals after the first field's.
 template <typename T, int C, bool NTS = false>
 __global__ __launch_bounds__(256, 4) void k_tridiag_ck(Geom g, Metrics<T> m,
                                                     typename CT2<T>::t* __restrict__ spec,
-                                                    T* __restrict__ ckp) {
-  k_tridiag_ck_body<T, C, NTS>(g, m, spec, ckp);
+                                                    T* __restrict__ ckp, long long bsp,
+                                                    long long bck) {
+  k_tridiag_ck_body<T, C, NTS>(g, m, spec + bsp * blockIdx.z, ckp + bck * blockIdx.z);
 }
 
 template <typename T>
-static int tridiag_t(c3d_ctx* c, void* spec) {
+__global__ __launch_bounds__(256) void k_tridiag_b(Geom g, Metrics<T> m,
+                                                   typename CT2<T>::t* __restrict__ spec,
+                                                   T* __restrict__ sarr, long long bsp,
+                                                   long long bck) {
+  k_tridiag_body<T>(g, m, spec + bsp * blockIdx.z, sarr + bck * blockIdx.z);
+}
+
+template <typename T>
+static int tridiag_t(c3d_ctx* c, void* spec, void* ck, const Batch& b) {
   const Geom& g = c->g;
-  dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4), blk(64, 4);
+  dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4, b.n), blk(64, 4);
   const Metrics<T>& m = *(const Metrics<T>*)(c->prec == C3D_FP64 ? (const void*)&c->md
                                                                   : (const void*)&c->mf);
   constexpr int CSEG = 16;
@@ -256,24 +267,25 @@ static int tridiag_t(c3d_ctx* c, void* spec) {
   if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
     if (g.nt)
       hipLaunchKernelGGL((k_tridiag_ck<T, CSEG, true>), grid, blk, 0, c->stream, g, m,
-                         (typename CT2<T>::t*)spec, (T*)c->sarr);
+                         (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
     else
       hipLaunchKernelGGL((k_tridiag_ck<T, CSEG, false>), grid, blk, 0, c->stream, g, m,
-                         (typename CT2<T>::t*)spec, (T*)c->sarr);
+                         (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
   } else {
-    hipLaunchKernelGGL(k_tridiag<T>, grid, blk, 0, c->stream, g, m,
-                       (typename CT2<T>::t*)spec, (T*)c->sarr);
+    hipLaunchKernelGGL(k_tridiag_b<T>, grid, blk, 0, c->stream, g, m,
+                       (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
 }
 
-int launch_tridiag(c3d_ctx* c, void* spec) {
+// ck: the checkpoint (or factor) scratch, nspec reals per field
+int launch_tridiag(c3d_ctx* c, void* spec, void* ck, const Batch& b) {
   hipEvent_t ev;
   prof_begin(c, ST_TRIDIAG, &ev);
-  int rc = c->prec == C3D_FP64 ? tridiag_t<double>(c, spec) : tridiag_t<float>(c, spec);
+  int rc = c->prec == C3D_FP64 ? tridiag_t<double>(c, spec, ck, b) : tridiag_t<float>(c, spec, ck, b);
   if (rc) return rc;
-  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz * b.n;
   prof_end(c, ST_TRIDIAG, ev, 2.0 * N * c->esize);
   return C3D_OK;
 }
@@ -328,7 +340,12 @@ __device__ void mean_mode_fp64(const Geom& g, const Metrics<double>& m, T2* spec
 template <typename T, int C>
 __global__ __launch_bounds__(256) void k_tridiag_mixed(Geom g, Metrics<T> m, Metrics<double> md,
                                                        typename CT2<T>::t* __restrict__ spec,
-                                                       T* __restrict__ ckp, double* w) {
+                                                       T* __restrict__ ckp, double* w,
+                                                       long long bsp, long long bck,
+                                                       long long bw) {
+  spec += bsp * blockIdx.z;
+  ckp += bck * blockIdx.z;
+  w += bw * blockIdx.z;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     mean_mode_fp64(g, md, spec, w);
     return;
@@ -341,20 +358,28 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(Geom g, Metrics<T> m, Met
 // 3-point x and z second differences and the Neumann y operator am2Pr/ac2Pr/
 // ap2Pr (whose wall entries are zero, so wall ghosts are never read)
 // The workgroups holding the plane j = 1 also write the sum of phi over
-// their 64 x-points to part[blockIdx.z * gridDim.x + blockIdx.x] (the gauge).
+// their 64 x-points to part[k * gridDim.x + blockIdx.x] (the gauge).
+// Batches: blockIdx.z = q nz + k for field q (bf reals apart; partial sums
+// bpart doubles apart).
 template <typename TR>
 __global__ __launch_bounds__(256) void k_ppe_resid(Geom g, Metrics<double> m, double rdx2,
                                                    double rdz2, const double* __restrict__ f,
                                                    const double* __restrict__ phi,
-                                                   TR* __restrict__ r, double* __restrict__ part) {
+                                                   TR* __restrict__ r, double* __restrict__ part,
+                                                   long long bf, long long bpart) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int jc = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z;
+  const int q = blockIdx.z / g.nz;
+  const int k = blockIdx.z - q * g.nz;
+  f += bf * q;
+  phi += bf * q;
+  r += bf * q;
+  part += bpart * q;
   if (blockIdx.y == 0 && threadIdx.y == 0) {
     // wavefront 0 of the block: row j = 1
     double v = i < g.nx ? phi[(long long)g.nx + g.plane * k + i] : 0.0;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (threadIdx.x == 0) part[(long long)blockIdx.z * gridDim.x + blockIdx.x] = v;
+    if (threadIdx.x == 0) part[(long long)k * gridDim.x + blockIdx.x] = v;
   }
   if (i >= g.nx || jc > g.ny) return;
   const long long row = (long long)g.nx * jc + g.plane * k;
@@ -371,10 +396,14 @@ __global__ __launch_bounds__(256) void k_ppe_resid(Geom g, Metrics<double> m, do
 }
 
 // w[3 ny] = -(sum of the partials, in index order): the gauge value of the
-// next correction solve; 0 resets it to the reference's pin
-__global__ void k_gauge(const double* __restrict__ part, int n, double* w, int ny, int reset) {
+// next correction solve; 0 resets it to the reference's pin.  One workgroup
+// per field of a batch (w and part bw doubles apart).
+__global__ void k_gauge(const double* __restrict__ part, int n, double* w, int ny, int reset,
+                        long long bw) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
+  part += bw * blockIdx.x;
+  w += bw * blockIdx.x;
   double a = 0.0;
   for (int q = t; q < n; q += 256) a += part[q];
   sh[t] = a;
@@ -386,48 +415,60 @@ __global__ void k_gauge(const double* __restrict__ part, int n, double* w, int n
   if (t == 0) w[3 * ny] = reset ? 0.0 : -sh[0];
 }
 
-static int run_ppe_mixed(c3d_ctx* c, void* prsrc, void* phi, int sweeps) {
+// The mixed-precision solve of a batch of sources (one field: the default
+// batch on the context's own scratch).  spec: FP32 spectra (b.spec32 apart),
+// ck: FP32 checkpoints, w: (0,0)-mode work and gauge partials (b.mean apart),
+// resid: FP32 residual fields (b.field apart).
+static int ppe_mixed(c3d_ctx* c, const void* prsrc, void* phi, int sweeps, void* spec, void* ck,
+                     double* w, float* resid, const Batch& b) {
   int rc = ensure_mixed_tables(c);
   if (rc) return rc;
   const Geom& g = c->g;
-  void* spec = c->spec;   // the FP64 spectral array, used as FP32 storage
   auto solve32 = [&]() -> int {
     int r;
-    if ((r = launch_fft32(c, FFT32_Z_FWD, nullptr, spec, nullptr, 0))) return r;
+    if ((r = launch_fft32(c, FFT32_Z_FWD, nullptr, spec, nullptr, 0, b))) return r;
     hipEvent_t ev;
     prof_begin(c, ST_TRIDIAG, &ev);
     constexpr int CSEG = 16;
-    dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4), blk(64, 4);
+    dim3 grid((g.nk + 1 + 63) / 64, (g.nz + 3) / 4, b.n), blk(64, 4);
     hipLaunchKernelGGL((k_tridiag_mixed<float, CSEG>), grid, blk, 0, c->stream, g, c->mf, c->md,
-                       (float2*)spec, (float*)c->sarr, c->d_mean64);
+                       (float2*)spec, (float*)ck, w, b.spec32, b.ck, b.mean);
     C3D_HIP(hipGetLastError());
-    prof_end(c, ST_TRIDIAG, ev, 2.0 * (double)g.nx * g.ny * g.nz * 4);
-    return launch_fft32(c, FFT32_Z_INV, nullptr, spec, nullptr, 0);
+    prof_end(c, ST_TRIDIAG, ev, 2.0 * (double)g.nx * g.ny * g.nz * b.n * 4);
+    return launch_fft32(c, FFT32_Z_INV, nullptr, spec, nullptr, 0, b);
   };
-  double* gpart = c->d_mean64 + 3 * ((size_t)g.ny + 1);
+  double* gpart = w + 3 * ((size_t)g.ny + 1);
   const int ngp = ((g.nx + 63) / 64) * g.nz;
-  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, c->stream, gpart, 0, c->d_mean64, g.ny, 1);
+  hipLaunchKernelGGL(k_gauge, dim3(b.n), dim3(256), 0, c->stream, gpart, 0, w, g.ny, 1, b.mean);
   C3D_HIP(hipGetLastError());
-  if ((rc = launch_fft32(c, FFT32_X_FWD_F64, prsrc, spec, nullptr, 0))) return rc;
+  if ((rc = launch_fft32(c, FFT32_X_FWD_F64, prsrc, spec, nullptr, 0, b))) return rc;
   if ((rc = solve32())) return rc;
-  if ((rc = launch_fft32(c, FFT32_X_INV_SET, nullptr, spec, (double*)phi, c->normfft))) return rc;
+  if ((rc = launch_fft32(c, FFT32_X_INV_SET, nullptr, spec, (double*)phi, c->normfft, b)))
+    return rc;
   for (int s = 0; s < sweeps; ++s) {
     hipEvent_t ev;
     prof_begin(c, ST_RESID, &ev);
-    dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), blk(64, 4);
+    dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * b.n), blk(64, 4);
     hipLaunchKernelGGL(k_ppe_resid<float>, grid, blk, 0, c->stream, g, c->md, c->rdx2, c->rdz2,
-                       (const double*)prsrc, (const double*)phi, c->resid32, gpart);
+                       (const double*)prsrc, (const double*)phi, resid, gpart, b.field, b.mean);
     C3D_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, c->stream, gpart, ngp, c->d_mean64, g.ny,
-                       0);
+    hipLaunchKernelGGL(k_gauge, dim3(b.n), dim3(256), 0, c->stream, gpart, ngp, w, g.ny, 0,
+                       b.mean);
     C3D_HIP(hipGetLastError());
-    prof_end(c, ST_RESID, ev, (double)g.nx * g.ny * g.nz * (8 + 8 + 4));
-    if ((rc = launch_fft32(c, FFT32_X_FWD_F32, c->resid32, spec, nullptr, 0))) return rc;
+    prof_end(c, ST_RESID, ev, (double)g.nx * g.ny * g.nz * b.n * (8 + 8 + 4));
+    if ((rc = launch_fft32(c, FFT32_X_FWD_F32, resid, spec, nullptr, 0, b))) return rc;
     if ((rc = solve32())) return rc;
-    if ((rc = launch_fft32(c, FFT32_X_INV_ADD, nullptr, spec, (double*)phi, c->normfft)))
+    if ((rc = launch_fft32(c, FFT32_X_INV_ADD, nullptr, spec, (double*)phi, c->normfft, b)))
       return rc;
   }
   return C3D_OK;
+}
+
+static int run_ppe_mixed(c3d_ctx* c, void* prsrc, void* phi, int sweeps) {
+  int rc = ensure_mixed_tables(c);
+  if (rc) return rc;
+  // the FP64 spectral array holds the FP32 spectrum
+  return ppe_mixed(c, prsrc, phi, sweeps, c->spec, c->sarr, c->d_mean64, c->resid32, Batch());
 }
 
 // clcPPE_x1: forward x, forward z, y solves, inverse z, inverse x * normfft.
@@ -443,9 +484,33 @@ int run_ppe(c3d_ctx* c, void* prsrc, void* phi) {
 int run_ppe_from_spectrum(c3d_ctx* c, void* phi) {
   int rc;
   if ((rc = launch_fft_z(c, c->spec, 0))) return rc;
-  if ((rc = launch_tridiag(c, c->spec))) return rc;
+  if ((rc = launch_tridiag(c, c->spec, c->sarr))) return rc;
   if ((rc = launch_fft_z(c, c->spec, 1))) return rc;
   return launch_fft_x_inv(c, c->spec, phi, c->normfft);
+}
+
+// A batch of pressure solves: every pass one launch over the whole batch.
+// sweeps < 0: the FP64 solve (the kernels of the single solve, so each
+// field's result equals c3d_ppe's bit for bit); sweeps >= 0: the
+// mixed-precision solve with that many refinement sweeps.
+int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps) {
+  int rc;
+  if ((rc = ensure_batch(c, nb, sweeps >= 0))) return rc;
+  const Geom& g = c->g;
+  Batch b;
+  b.n = nb;
+  b.field = g.nfield;
+  b.spec = g.nspec;
+  b.spec32 = g.nspec;      // FP32 spectra packed in the FP64 batch array
+  b.ck = g.nspec;
+  b.mean = batch_mean_len(g);
+  if (sweeps >= 0)
+    return ppe_mixed(c, prsrc, phi, sweeps, c->bspec, c->bsarr, c->bmean, (float*)c->bres, b);
+  if ((rc = launch_fft_x_fwd(c, prsrc, c->bspec, b))) return rc;
+  if ((rc = launch_fft_z(c, c->bspec, 0, b))) return rc;
+  if ((rc = launch_tridiag(c, c->bspec, c->bsarr, b))) return rc;
+  if ((rc = launch_fft_z(c, c->bspec, 1, b))) return rc;
+  return launch_fft_x_inv(c, c->bspec, phi, c->normfft, b);
 }
 
 }  // namespace c3d

@@ -52,6 +52,7 @@ EXPORTS = (
     "c3d_closure_get_params", "c3d_closure_size", "c3d_closure_add_samples",
     "c3d_closure_get_samples", "c3d_closure_predict", "c3d_closure_grad", "c3d_closure_train",
     "c3d_closure_adam_step", "c3d_closure_dsm", "c3d_set_les_closure", "c3d_closure_fallbacks",
+    "c3d_field_bytes", "c3d_field_alloc_batch", "c3d_ppe_batch", "c3d_helmholtz_batch",
 )
 
 
@@ -172,6 +173,10 @@ def load(path: str | None = None):
         "c3d_stat_get_grad": ([_VP, _DP], I),
         "c3d_write_vgrad": ([_VP, ctypes.c_char_p], I),
         "c3d_set_les": ([_VP, I, I], I),
+        "c3d_field_bytes": ([_VP, I], ctypes.c_size_t),
+        "c3d_field_alloc_batch": ([_VP, I, I, ctypes.POINTER(_VP)], I),
+        "c3d_ppe_batch": ([_VP, I, _VP, _VP, I], I),
+        "c3d_helmholtz_batch": ([_VP, I, I, I, _VP, _VP, I], I),
         "c3d_Clc_SGS_Vis": ([_VP] * 5, I),
         "c3d_clcSpectra1D": ([_VP] * 5 + [I, I], I),
         "c3d_spectra_get": ([_VP, _DP, _DP, _IP], I),

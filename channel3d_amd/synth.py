@@ -111,3 +111,41 @@ def smooth_modes(nxc, nyc, nzc, yp, xlx, yly, zlz, seed=0, nmodes=48, kmax=6, am
                       * np.sin(2 * np.pi * kz * z / zlz + pz)[None, None, :])
         comp[1:nxc + 1, 1:nyc + 1, 1:nzc + 1] += f
     return ux, uy, uz
+
+
+def spectrum_field(nxc, nyc, nzc, seed=0, k0=8.0, kcut=40, dyp=None):
+    """a random-phase field whose energy spectrum over the index wavenumber
+    |k| = |(kx, ky, kz)| is E(k) ~ k^4 exp(-2 (k/k0)^2): modal amplitudes
+    k exp(-(k/k0)^2) (E(k) over the shell area 4 pi k^2, square-rooted),
+    uniform phases from numpy's default_rng(seed), modes with |k| > kcut left
+    out (their amplitude is below 1e-9 of the peak for k0 = 8), unit RMS.
+    With dyp (nyc cell heights) the dyp-weighted mean is removed, so the field
+    is a compatible source of the Neumann pressure equation (dyp is the left
+    null vector of its wall-normal operator)."""
+    import scipy.fft as sfft
+    rng = np.random.default_rng(seed)
+    K = int(kcut)
+    kx = np.arange(-K, K + 1)
+    ky = np.arange(-K, K + 1)
+    kz = np.arange(0, K + 1)
+    kk = np.sqrt(kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = kk * np.exp(-(kk / k0) ** 2)
+    amp[kk > K] = 0.0
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=amp.shape)
+    spec = np.zeros((nxc, nyc, nzc // 2 + 1), dtype=np.complex128)
+    ix = kx % nxc
+    iy = ky % nyc
+    iz = kz[kz <= nzc // 2]
+    band = amp * np.exp(1j * ph)
+    # modes beyond the grid's own Nyquist limits fold onto lower indices only
+    # on grids smaller than 2 kcut; they are added there
+    for a, i in enumerate(ix):
+        for b, jy in enumerate(iy):
+            spec[i, jy, iz] += band[a, b, :iz.size]
+    f = sfft.irfftn(spec, s=(nxc, nyc, nzc), workers=-1)
+    f = np.asfortranarray(f / f.std())
+    if dyp is not None:
+        w = np.asarray(dyp, dtype=np.float64)
+        m = (f.mean(axis=(0, 2)) * w).sum() / w.sum()
+        f = np.asfortranarray(f - m)
+    return f

@@ -223,6 +223,37 @@ int c3d_project(c3d_ctx* ctx);
  * device fields of the context; exposed for the solves/s benchmark */
 int c3d_ppe(c3d_ctx* ctx, void* prsrc, void* prphiHalo);
 
+/* ------------------------------------------------------ batched solves -- */
+/* The operators of clcPPE (m_Poisson.f90:567-688) and of the implicit
+ * velocity solves clcU1Hat/clcU2Hat/clcU3Hat_FIMP_010
+ * (m_TSchemeFIMP_inc.f90:263-347, 636-729, 1035-1128) applied to nbatch
+ * right-hand sides at once, every pass of the solve one launch over the whole
+ * batch.  No counterpart in the reference, whose routines take one
+ * right-hand side per call; a batch gives the passes nbatch times the
+ * parallelism.  FP64 2nd-order contexts; asynchronous on the context stream.
+ * A batch is nbatch device fields of one layout back to back: field q starts
+ * q * c3d_field_bytes(ctx, layout) bytes after the first
+ * (c3d_field_alloc_batch; c3d_upload / c3d_download take a field's address).
+ * sweeps = -1: the FP64 solve (the per-call solve's kernels: each field's
+ * result equals the single solve's); sweeps >= 0: FP32 transforms,
+ * convolutions and wall-normal sweeps, followed by that many refinement
+ * sweeps of FP64 residual, FP32 correction solve, FP64 update (as
+ * c3d_set_ppe_refine for the pressure solve).  The first call of a batch
+ * size allocates the batch's work space, which the context keeps. */
+size_t c3d_field_bytes(c3d_ctx* ctx, int layout);
+int c3d_field_alloc_batch(c3d_ctx* ctx, int layout, int nbatch, void** dptr);
+/* prsrc: C3D_INTERIOR batch, read only; phi: C3D_HALO batch (interior
+ * written), normfft and the gauge pin as c3d_clcPPE */
+int c3d_ppe_batch(c3d_ctx* ctx, int nbatch, const void* prsrc, void* phi, int sweeps);
+/* (1 - pmBeta Dxx)(1 - pmBeta Dzz)(1 - pmBeta Dyy) du = rhs for velocity
+ * component comp (0 ux, 1 uy, 2 uz) with pmBeta of RK substep ns: rhs
+ * C3D_INTERIOR batch, read only; du C3D_INTERIOR batch, written (u is not
+ * added).  For uy, row 1 is the wall condition with a zero wall increment
+ * (duy_ym = 0, the plain channel's).  Needs x/z Green's functions within the
+ * fused convolution's window (the per-routine solve has no such limit). */
+int c3d_helmholtz_batch(c3d_ctx* ctx, int comp, int ns, int nbatch, const void* rhs, void* du,
+                        int sweeps);
+
 /* ---------------------------------------------------------- profiling -- */
 /* per-stage HIP-event timing on the context stream */
 int c3d_profile_enable(c3d_ctx* ctx, int on);

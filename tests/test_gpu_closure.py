@@ -211,6 +211,50 @@ def _yp():
     return channel_yp(DNS["nyc"], DNS["yly"], DNS["gamma"])
 
 
+def _nut_oracle(g, case, p):
+    """the oracle chain of the closure's LES viscosity on the context's own
+    grid: (nut in the interior cells, fallback mask)"""
+    from oracle import closure as C
+    from channel3d_amd import lib as L
+    ux, uy, uz = (g.download(f) for f in (L.UX, L.UY, L.UZ))
+    dx, dz = case["xlx"] / case["nxc"], case["zlz"] / case["nzc"]
+    yp = g.yp
+    X, _, S = C.samples(ux, uy, uz, yp, dx, dz, case["yly"], fx=1, fy=1, fz=1)
+    Y, _ = C.forward(p, X.astype(np.float32), emulate=True)
+    nI, nJ, nK = case["nxc"], case["nyc"], case["nzc"]
+    t = [(Y[:, q].astype(np.float64) * S.astype(np.float32).astype(np.float64))
+         .reshape((nI, nJ, nK), order="F") for q in range(6)]
+    t = C.deviatoric(t)
+    ub, _ = C.filtered(ux, uy, uz, yp, 1, 1, 1)
+    dY, yc = C.les_grid(yp, 1)
+    g3 = C.gradients(ub, dx, dz, yc)
+    Sij = [0.5 * (g3[a][b] + g3[b][a]) for a, b in C.PAIRS]
+    sn = C.strain_norm(g3)
+    _, cs2 = C.dynamic_smagorinsky(ub, dx, dY, dz, yc)
+    wt = [1, 1, 1, 2, 2, 2]
+    ts = sum(w * a * b for w, a, b in zip(wt, t, Sij))
+    ss = sum(w * b * b for w, b in zip(wt, Sij))
+    nu = -ts / (2 * ss)
+    fb = ~(nu >= 0) | ~np.isfinite(nu)
+    d2 = ((dx * dY * dz) ** (2 / 3))[None, :, None]
+    nu = np.where(fb, np.maximum(cs2, 0)[None, :, None] * d2 * sn, nu)
+    return case["xnu"] + nu, fb
+
+
+def _check_nut(g, case, p, nut, nfb):
+    """bf16 operands: bar on the 99th percentile of the nut error, and the
+    fallback counts agree to within the cells a bf16 flip can move across
+    nu = 0"""
+    ref, fb = _nut_oracle(g, case, p)
+    got = nut[1:-2, 1:-2, 1:-2]
+    err = np.abs(got - ref) / np.abs(ref).max()
+    print(f"nut {case['nxc']}x{case['nyc']}x{case['nzc']}: fallback {nfb} "
+          f"(oracle {int(fb.sum())}), err max {err.max():.2e}, p99 {np.quantile(err, 0.99):.2e}")
+    assert np.quantile(err, 0.99) < 1e-4
+    assert abs(nfb - int(fb.sum())) <= max(2, fb.size // 1000)
+    assert (got >= case["xnu"] * (1 - 1e-12)).all()
+
+
 LESC = dict(nxc=32, nyc=32, nzc=24, xlx=2 * np.pi, yly=2.0, zlz=np.pi, xnu=1.0 / 2000.0,
             dt=0.004, gamma=1.5, ubulk=1.0, les_type=2)
 
@@ -234,38 +278,7 @@ def test_les_viscosity_from_closure():
     g.sync()
     nut = g.download(L.NUT)
     nfb = cl.fallbacks()
-    ux, uy, uz = (g.download(f) for f in (L.UX, L.UY, L.UZ))
-    dx, dz = LESC["xlx"] / LESC["nxc"], LESC["zlz"] / LESC["nzc"]
-    yp = g.yp
-    X, _, S = C.samples(ux, uy, uz, yp, dx, dz, LESC["yly"], fx=1, fy=1, fz=1)
-    Y, _ = C.forward(p, X.astype(np.float32), emulate=True)
-    nI, nJ, nK = LESC["nxc"], LESC["nyc"], LESC["nzc"]
-    t = [(Y[:, q].astype(np.float64) * S.astype(np.float32).astype(np.float64))
-         .reshape((nI, nJ, nK), order="F") for q in range(6)]
-    t = C.deviatoric(t)
-    ub, _ = C.filtered(ux, uy, uz, yp, 1, 1, 1)
-    dY, yc = C.les_grid(yp, 1)
-    g3 = C.gradients(ub, dx, dz, yc)
-    Sij = [0.5 * (g3[a][b] + g3[b][a]) for a, b in C.PAIRS]
-    sn = C.strain_norm(g3)
-    _, cs2 = C.dynamic_smagorinsky(ub, dx, dY, dz, yc)
-    wt = [1, 1, 1, 2, 2, 2]
-    ts = sum(w * a * b for w, a, b in zip(wt, t, Sij))
-    ss = sum(w * b * b for w, b in zip(wt, Sij))
-    nu = -ts / (2 * ss)
-    fb = ~(nu >= 0) | ~np.isfinite(nu)
-    d2 = ((dx * dY * dz) ** (2 / 3))[None, :, None]
-    nu = np.where(fb, np.maximum(cs2, 0)[None, :, None] * d2 * sn, nu)
-    ref = LESC["xnu"] + nu
-    got = nut[1:-2, 1:-2, 1:-2]
-    err = np.abs(got - ref) / np.abs(ref).max()
-    # bf16 operands: bar on the 99th percentile, and the fallback counts agree
-    # to within the cells a bf16 flip can move across nu = 0
-    print(f"nut: fallback {nfb} (oracle {int(fb.sum())}), err max {err.max():.2e}, "
-          f"p99 {np.quantile(err, 0.99):.2e}")
-    assert np.quantile(err, 0.99) < 1e-4
-    assert abs(nfb - int(fb.sum())) <= max(2, fb.size // 1000)
-    assert (got >= LESC["xnu"] * (1 - 1e-12)).all()
+    _check_nut(g, LESC, p, nut, nfb)
     # LES steps with the learned model
     for _ in range(5):
         g.step()
@@ -302,3 +315,76 @@ def test_les_closure_steps_replay_from_graphs():
         g.set_les_closure(None)
         g.close()
     cl.close()
+
+
+def test_one_closure_two_contexts():
+    """one network attached to two LES contexts of different grids (ny and
+    stretching differ): each context's viscosity follows its own y tables
+    and scratch, before and after the other evaluates, against the oracle"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from channel3d_amd import lib as L
+    cl = Closure(1 << 12)
+    p = C.init_params(6)
+    cl.set_params(p)
+    case_b = dict(LESC, nyc=40, gamma=2.0)
+    a = _smooth_dns(LESC, seed=21)
+    b = _smooth_dns(case_b, seed=22)
+    a.set_les_closure(cl)
+    b.set_les_closure(cl)
+    a.Clc_SGS_Vis()
+    b.Clc_SGS_Vis()
+    a.sync()
+    b.sync()
+    nfb_b = cl.fallbacks()            # b evaluated last
+    nut_a, nut_b = a.download(L.NUT), b.download(L.NUT)
+    _check_nut(b, case_b, p, nut_b, nfb_b)
+    a.Clc_SGS_Vis()                   # a again, after b's evaluation
+    a.sync()
+    assert np.array_equal(a.download(L.NUT), nut_a)
+    _check_nut(a, LESC, p, nut_a, cl.fallbacks())
+    for g in (a, b):
+        g.set_les_closure(None)
+        g.close()
+    cl.close()
+
+
+def test_two_contexts_share_closure_concurrently():
+    """two LES contexts with different states share one network and step
+    interleaved on their own streams, without synchronising between them;
+    each ends bit for bit where it ends when it runs alone"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from channel3d_amd import lib as L
+    cl = Closure(1 << 12)
+    cl.set_params(C.init_params(2))
+    ref = {}
+    for seed in (31, 32):
+        g = _smooth_dns(LESC, seed=seed)
+        g.set_les_closure(cl)
+        for _ in range(3):
+            g.step()
+        g.sync()
+        ref[seed] = [g.download(f) for f in (L.UX, L.UY, L.UZ, L.PR, L.NUT)]
+        g.set_les_closure(None)
+        g.close()
+    a = _smooth_dns(LESC, seed=31)
+    b = _smooth_dns(LESC, seed=32)
+    for g in (a, b):
+        g.set_les_closure(cl)
+    for _ in range(3):
+        a.step()
+        b.step()
+    a.sync()
+    b.sync()
+    for seed, g in ((31, a), (32, b)):
+        got = [g.download(f) for f in (L.UX, L.UY, L.UZ, L.PR, L.NUT)]
+        for x, y in zip(got, ref[seed]):
+            assert np.array_equal(x, y), seed
+    # destroying the closure detaches it: the contexts step on with WALE
+    cl.close()
+    a.step()
+    a.sync()
+    assert np.isfinite(a.diagnostics()["vmax_x"])
+    for g in (a, b):
+        g.close()

@@ -362,3 +362,56 @@ def test_multistep_parity(O, name, nsteps):
     assert abs(d["PrGradAver"] - o.scalar(O.S_PRGRADAVER)) < 1e-10
     assert abs(d["SimTime"] - o.scalar(O.S_SIMTIME)) < 1e-13
     g.close()
+
+
+def test_cfl_follows_uploaded_state(O):
+    """a state uploaded between steps replaces the one the last step's closing
+    pass computed the next CFL number from: the next step reports the CFL of
+    the state it actually starts from (m_ChannelSystem.f90:80-81)"""
+    from channel3d_amd import lib as L
+    case = cases.SMALL
+    o, g = make_pair(O, case, seed=1)
+    g.step()
+    g.step()
+    o2, g2 = make_pair(O, case, seed=7)
+    for gid in (L.UX, L.UY, L.UZ, L.PR):
+        g.upload(gid, g2.download(gid))
+    g2.close()
+    o2.iterate()
+    g.step()
+    d = g.diagnostics()
+    assert relerr(np.array([d["cfl"]]), np.array([o2.scalar(O.S_CFL)])) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["small", "odd"])
+def test_nontemporal_variants_bitwise(O, name, monkeypatch):
+    """the kernel variants with non-temporal loads and stores (chosen for
+    fields of 64 MiB and more) forced on a small grid: bit for bit the cached
+    variants, and both at the oracle within the bar"""
+    from channel3d_amd import Channel, lib as L
+    case = CASES[name]
+    o, g0 = make_pair(O, case, seed=3)
+    monkeypatch.setenv("C3D_NT", "1")
+    g1 = Channel(case, yp=o.yp)
+    monkeypatch.setenv("C3D_NT", "0")
+    for gid in (L.UX, L.UY, L.UZ, L.PR):
+        g1.upload(gid, g0.download(gid))
+    src = np.random.default_rng(5).standard_normal((o.nxc, o.nyc, o.nzc))
+    src = np.asfortranarray(src - src.mean())
+    phis = []
+    for g in (g0, g1):
+        g.upload(L.ARR1, src)
+        g.clcPPE(L.ARR1, L.HALOARR)
+        phis.append(g.download(L.HALOARR))
+    assert np.array_equal(phis[0], phis[1])
+    for _ in range(2):
+        o.iterate()
+        g0.step()
+        g1.step()
+    for gid in (L.UX, L.UY, L.UZ, L.PR, L.HISTX, L.HISTY, L.HISTZ):
+        assert np.array_equal(g0.download(gid), g1.download(gid)), gid
+    compare_state(O, o, g1, TOL)
+    assert g0.diagnostics() == g1.diagnostics()
+    g0.close()
+    g1.close()
