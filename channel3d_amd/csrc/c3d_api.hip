@@ -710,7 +710,7 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
     return C3D_ERR_ARG;
   }
   if (!fft_size_supported(p->nxc / 2) || !fft_size_supported(p->nzc)) {
-    set_error("c3d_create: nxc/2 and nzc must be supported FFT lengths (2^a 3^b 5^c list)");
+    set_error("c3d_create: nxc/2 and nzc must be FFT lengths 2^a 3^b 5^c up to 4095");
     return C3D_ERR_UNSUPPORTED;
   }
   if (p->precision != C3D_FP64 && p->precision != C3D_FP32) {
@@ -757,6 +757,16 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   // 32 MiB, the last-level cache 256 MiB: smaller grids stay cached);
   // C3D_NT=0 / C3D_NT=1 forces either kernel variant (tests of both forms)
   g.nt = (double)g.nfield * (p->precision == C3D_FP64 ? 8 : 4) >= 64.0 * 1024 * 1024;
+  // C3D_FFT_PLAN=runtime: every transform takes the run-time radix plan (the
+  // kernels of the lengths outside the compiled list; tests compare the two)
+  if (const char* fp = std::getenv("C3D_FFT_PLAN")) {
+    if (std::strcmp(fp, "runtime") == 0) g.fftrt = 1;
+    else if (std::strcmp(fp, "compiled") != 0) {
+      delete c;
+      set_error("c3d_create: C3D_FFT_PLAN must be compiled or runtime");
+      return C3D_ERR_ARG;
+    }
+  }
   if (const char* nt = std::getenv("C3D_NT")) {
     if (std::strcmp(nt, "0") == 0) g.nt = 0;
     else if (std::strcmp(nt, "1") == 0) g.nt = 1;
@@ -1468,7 +1478,7 @@ static int substep_les(c3d_ctx* c, int ns) {
   if ((rc = launch_les_y(c, 1, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
   if ((rc = launch_les_y(c, 2, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
-  if (c->ppe_sweeps < 0) {
+  if (c->ppe_sweeps < 0 && fft_size_compiled(c->g.nk) && !c->g.fftrt) {
     if ((rc = launch_prsrc_fft_x(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr))) return rc;
     if ((rc = run_ppe_from_spectrum(c, s[C3D_HALOARR]))) return rc;
   } else {
@@ -1504,8 +1514,9 @@ extern "C" int c3d_substep(c3d_ctx* c, int ns) {
   if ((rc = launch_adi(c, 2, ns, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
   // :110-114
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
-  if (c->ppe_sweeps < 0) {
-    // clcPrSrc fused into the first pass of clcPPE (RealArr1 is not written)
+  if (c->ppe_sweeps < 0 && fft_size_compiled(c->g.nk) && !c->g.fftrt) {
+    // clcPrSrc fused into the first pass of clcPPE (RealArr1 is not written);
+    // x lengths with a run-time plan take the separate source kernel below
     if ((rc = launch_prsrc_fft_x(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR]))) return rc;
     if ((rc = run_ppe_from_spectrum(c, s[C3D_HALOARR]))) return rc;
   } else {

@@ -35,6 +35,7 @@ struct Geom {
   long long nspec;    // P * ny * nz
   int nt = 0;         // fields far larger than the caches: the kernels that read
                       // or write words once take their non-temporal variant
+  int fftrt = 0;      // every transform length takes the run-time radix plan
 };
 
 // A word a kernel reads or writes exactly once: non-temporal in the kernel
@@ -407,7 +408,10 @@ int launch_stat(c3d_ctx* c, const void* ux, const void* uy, const void* uz, cons
 void fmt_es24_15(double v, char out[32]);
 
 // ---- launchers (k_fft.hip / k_ppe.hip) ----
+// 2^a 3^b 5^c up to 4095 (compile-time kernels for the listed lengths,
+// run-time radix plans for the others)
 bool fft_size_supported(int n);
+bool fft_size_compiled(int n);
 int launch_fft_x_fwd(c3d_ctx* c, const void* in, void* spec, const Batch& b = Batch());
 int launch_fft_x_inv(c3d_ctx* c, const void* spec, void* out, double scale,
                      const Batch& b = Batch());

@@ -14,6 +14,8 @@
 // rounded from long-double values on the host.
 #include "c3d_internal.h"
 
+#include <algorithm>
+
 namespace c3d {
 
 template <typename T>
@@ -467,13 +469,285 @@ __global__ __launch_bounds__(FNT) void k_fft_z(Geom g, typename CT<T>::t* __rest
 }
 
 // ===========================================================================
+// Any length 2^a 3^b 5^c: the radix plan is made from the length on the host
+// (the compile-time kernels' greedy choice: 8, then 4, 2, 3, 5) and the
+// kernels loop over it.  A workgroup stages its lines in LDS (dynamic, at
+// most 64 KiB), runs every pass in place there -- each pass reads all its
+// butterflies' inputs into registers, synchronises, then writes -- and
+// writes the result out.  Two LDS round trips more than the compiled
+// lengths' kernels, which fuse the first and last passes with the global
+// accesses; these kernels serve the lengths outside that list.
+// ===========================================================================
+constexpr int GEN_MAXP = 24;   // passes: 2^24 exceeds every supported length
+constexpr int GEN_MAXE = 16;   // complex values a thread holds in one pass
+constexpr int GEN_NMAX = 4095; // one line of (N+1) FP64 complex fits 64 KiB
+
+struct FftPlan {
+  int n;                       // transform length
+  int np;                      // passes
+  int lines;                   // lines per workgroup
+  int r[GEN_MAXP];             // radix of each pass
+};
+
+// lines per workgroup: the LDS tile (lines x (n+1) complex of esz bytes) at
+// most 64 KiB, lines x n at most 8 x FNT values so that every pass's
+// butterflies fit GEN_MAXE registers per thread, and at most `cap`
+static FftPlan make_plan(int n, size_t esz, int cap) {
+  FftPlan pl{};
+  pl.n = n;
+  int m = n;
+  while (m > 1 && pl.np < GEN_MAXP) {
+    const int R = pick_radix(m);
+    pl.r[pl.np++] = R;
+    m /= R;
+  }
+  int lines = (int)(65536 / ((size_t)(n + 1) * esz));
+  lines = std::min(lines, 8 * FNT / n);
+  pl.lines = std::max(1, std::min(lines, cap));
+  return pl;
+}
+
+template <typename T2, int S, int R>
+__device__ __forceinline__ void gen_pass(T2* sm, int ldn, const T2* __restrict__ tw, int N,
+                                         int lines, int NS) {
+  constexpr int PMAX = GEN_MAXE / R;
+  const int NR = N / R, NB = lines * NR, TWS = N / (NS * R);
+  const int tid = threadIdx.x;
+  T2 v[PMAX][R];
+  int ln[PMAX], od[PMAX];
+#pragma unroll
+  for (int p = 0; p < PMAX; ++p) {
+    const int gb = tid + p * FNT;
+    ln[p] = -1;
+    if (gb < NB) {
+      const int line = gb / NR, j = gb - line * NR;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[p][r] = sm[line * ldn + j + r * NR];
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          T2 w = tw[k * r * TWS];
+          if (S > 0) w.y = -w.y;
+          v[p][r] = cmul(v[p][r], w);
+        }
+      }
+      Dft<R, S, T2>::run(v[p]);
+      ln[p] = line;
+      od[p] = (j / NS) * NS * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PMAX; ++p) {
+    if (ln[p] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) sm[ln[p] * ldn + od[p] + r * NS] = v[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+// the whole transform of the lines staged in sm (line stride ldn), in place
+template <typename T2, int S>
+__device__ void gen_fft(T2* sm, int ldn, const T2* __restrict__ tw, const FftPlan& pl) {
+  int NS = 1;
+  for (int q = 0; q < pl.np; ++q) {
+    switch (pl.r[q]) {
+      case 8: gen_pass<T2, S, 8>(sm, ldn, tw, pl.n, pl.lines, NS); break;
+      case 4: gen_pass<T2, S, 4>(sm, ldn, tw, pl.n, pl.lines, NS); break;
+      case 2: gen_pass<T2, S, 2>(sm, ldn, tw, pl.n, pl.lines, NS); break;
+      case 3: gen_pass<T2, S, 3>(sm, ldn, tw, pl.n, pl.lines, NS); break;
+      default: gen_pass<T2, S, 5>(sm, ldn, tw, pl.n, pl.lines, NS); break;
+    }
+    NS *= pl.r[q];
+  }
+}
+
+template <typename T, typename TIN, bool NTS>
+__global__ __launch_bounds__(FNT) void k_fftg_x_fwd(Geom g, FftPlan pl, const TIN* __restrict__ in,
+                                                    typename CT<T>::t* __restrict__ spec,
+                                                    const typename CT<T>::t* __restrict__ tw,
+                                                    const typename CT<T>::t* __restrict__ twr,
+                                                    long long bin, long long bsp) {
+  using T2 = typename CT<T>::t;
+  using TI2 = typename CT<TIN>::t;
+  extern __shared__ __align__(16) unsigned char gen_raw[];
+  T2* sm = (T2*)gen_raw;
+  __shared__ long long rb[16];
+  in += bin * blockIdx.y;
+  spec += bsp * blockIdx.y;
+  const int N = pl.n, lines = pl.lines, ldn = N + 1;
+  const int row0 = blockIdx.x * lines;
+  const int nrows = g.ny * g.nz;
+  row_offsets(g, row0, lines, rb);
+  __syncthreads();
+  for (int e = threadIdx.x; e < lines * N; e += FNT) {
+    const int l = e / N, m = e - l * N;
+    const long long b = rb[l];
+    T2 z = mk<T2>(0, 0);
+    if (b >= 0) {
+      const TI2 w = ((const TI2*)(in + b))[m];
+      z = mk<T2>((T)w.x, (T)w.y);
+    }
+    sm[l * ldn + m] = z;
+  }
+  __syncthreads();
+  gen_fft<T2, -1>(sm, ldn, tw, pl);
+  // X_k = (Z_k + conj Z_{N-k})/2 + w_k (Z_k - conj Z_{N-k})/(2i)
+  for (int e = threadIdx.x; e < lines * (N + 1); e += FNT) {
+    const int l = e / (N + 1), k = e - l * (N + 1), row = row0 + l;
+    if (row >= nrows) continue;
+    const T2 zk = sm[l * ldn + (k == N ? 0 : k)];
+    const T2 zn = sm[l * ldn + (k == 0 ? 0 : N - k)];
+    const T er = (T)0.5 * (zk.x + zn.x), ei = (T)0.5 * (zk.y - zn.y);
+    const T dr = zk.x - zn.x, di = zk.y + zn.y;
+    const T2 o = mk<T2>((T)0.5 * di, (T)-0.5 * dr);
+    const T2 wo = cmul(twr[k], o);
+    T2* d = &spec[(long long)g.P * row + k];
+    C3D_ST_ONCE(er + wo.x, &d->x);
+    C3D_ST_ONCE(ei + wo.y, &d->y);
+  }
+}
+
+template <typename T, typename TOUT, bool ACC, bool NTS>
+__global__ __launch_bounds__(FNT) void k_fftg_x_inv(Geom g, FftPlan pl,
+                                                    const typename CT<T>::t* __restrict__ spec,
+                                                    TOUT* __restrict__ out,
+                                                    const typename CT<T>::t* __restrict__ tw,
+                                                    const typename CT<T>::t* __restrict__ twr,
+                                                    TOUT scale, long long bsp, long long bout) {
+  using T2 = typename CT<T>::t;
+  using TO2 = typename CT<TOUT>::t;
+  extern __shared__ __align__(16) unsigned char gen_raw[];
+  T2* sm = (T2*)gen_raw;
+  __shared__ long long rb[16];
+  spec += bsp * blockIdx.y;
+  out += bout * blockIdx.y;
+  const int N = pl.n, lines = pl.lines, ldn = N + 1;
+  const int row0 = blockIdx.x * lines;
+  const int nrows = g.ny * g.nz;
+  row_offsets(g, row0, lines, rb);
+  // Z_k = (X_k + conj X_{N-k}) + i conj(w_k) (X_k - conj X_{N-k})
+  for (int e = threadIdx.x; e < lines * N; e += FNT) {
+    const int l = e / N, k = e - l * N, row = row0 + l;
+    T2 z = mk<T2>(0, 0);
+    if (row < nrows) {
+      const T2* sp = spec + (long long)g.P * row;
+      T2 xk = sp[k], xn = sp[N - k];
+      if (k == 0) {
+        xk.y = 0;
+        xn.y = 0;
+      }
+      const T2 a = mk<T2>(xk.x + xn.x, xk.y - xn.y);
+      const T2 b = mk<T2>(xk.x - xn.x, xk.y + xn.y);
+      T2 w = twr[k];
+      w.y = -w.y;
+      const T2 wb = cmul(w, b);
+      z = mk<T2>(a.x - wb.y, a.y + wb.x);
+    }
+    sm[l * ldn + k] = z;
+  }
+  __syncthreads();
+  gen_fft<T2, 1>(sm, ldn, tw, pl);
+  for (int e = threadIdx.x; e < lines * N; e += FNT) {
+    const int l = e / N, m = e - l * N;
+    const long long b = rb[l];
+    if (b < 0) continue;
+    const T2 z = sm[l * ldn + m];
+    TO2* rp = (TO2*)(out + b);
+    if (ACC) {
+      const TO2 o = rp[m];
+      rp[m] = mk<TO2>(o.x + (TOUT)z.x * scale, o.y + (TOUT)z.y * scale);
+    } else {
+      C3D_ST_ONCE((TOUT)z.x * scale, &rp[m].x);
+      C3D_ST_ONCE((TOUT)z.y * scale, &rp[m].y);
+    }
+  }
+}
+
+// z, in place: pl.lines consecutive kx of one y plane, all kz
+template <typename T, int S, bool NTS>
+__global__ __launch_bounds__(FNT) void k_fftg_z(Geom g, FftPlan pl,
+                                                typename CT<T>::t* __restrict__ spec,
+                                                const typename CT<T>::t* __restrict__ tw,
+                                                long long bsp) {
+  using T2 = typename CT<T>::t;
+  extern __shared__ __align__(16) unsigned char gen_raw[];
+  T2* sm = (T2*)gen_raw;
+  spec += bsp * blockIdx.z;
+  const int N = pl.n, lines = pl.lines, ldn = N + 1;
+  const int kx0 = blockIdx.x * lines;
+  const int nl = min(lines, g.nk + 1 - kx0);
+  const long long sk = (long long)g.P * g.ny;
+  T2* base = spec + (long long)g.P * blockIdx.y + kx0;
+  for (int e = threadIdx.x; e < lines * N; e += FNT) {
+    const int l = e % lines, m = e / lines;
+    sm[l * ldn + m] = l < nl ? base[sk * m + l] : mk<T2>(0, 0);
+  }
+  __syncthreads();
+  gen_fft<T2, S>(sm, ldn, tw, pl);
+  for (int e = threadIdx.x; e < lines * N; e += FNT) {
+    const int l = e % lines, m = e / lines;
+    if (l < nl) {
+      const T2 v = sm[l * ldn + m];
+      T2* d = &base[sk * m + l];
+      C3D_ST_ONCE(v.x, &d->x);
+      C3D_ST_ONCE(v.y, &d->y);
+    }
+  }
+}
+
+template <typename T, typename TIN, bool NTS>
+static void gen_x_fwd(c3d_ctx* c, const TIN* in, void* spec, const void* tw, const void* twr,
+                      long long bin, long long bsp, int nb) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const FftPlan pl = make_plan(g.nk, sizeof(T2), 16);
+  const int nrows = g.ny * g.nz;
+  const size_t lds = (size_t)pl.lines * (pl.n + 1) * sizeof(T2);
+  hipLaunchKernelGGL((k_fftg_x_fwd<T, TIN, NTS>), dim3((nrows + pl.lines - 1) / pl.lines, nb),
+                     dim3(FNT), lds, c->stream, g, pl, in, (T2*)spec, (const T2*)tw,
+                     (const T2*)twr, bin, bsp);
+}
+
+template <typename T, typename TOUT, bool ACC, bool NTS>
+static void gen_x_inv(c3d_ctx* c, const void* spec, TOUT* out, const void* tw, const void* twr,
+                      TOUT scale, long long bsp, long long bout, int nb) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const FftPlan pl = make_plan(g.nk, sizeof(T2), 16);
+  const int nrows = g.ny * g.nz;
+  const size_t lds = (size_t)pl.lines * (pl.n + 1) * sizeof(T2);
+  hipLaunchKernelGGL((k_fftg_x_inv<T, TOUT, ACC, NTS>),
+                     dim3((nrows + pl.lines - 1) / pl.lines, nb), dim3(FNT), lds, c->stream, g, pl,
+                     (const T2*)spec, out, (const T2*)tw, (const T2*)twr, scale, bsp, bout);
+}
+
+template <typename T, bool NTS>
+static void gen_z(c3d_ctx* c, void* spec, const void* tw, int inverse, long long bsp, int nb) {
+  using T2 = typename CT<T>::t;
+  const Geom& g = c->g;
+  const FftPlan pl = make_plan(g.nz, sizeof(T2), ZTK);
+  const size_t lds = (size_t)pl.lines * (pl.n + 1) * sizeof(T2);
+  const dim3 grid((g.nk + 1 + pl.lines - 1) / pl.lines, g.ny, nb);
+  if (inverse)
+    hipLaunchKernelGGL((k_fftg_z<T, 1, NTS>), grid, dim3(FNT), lds, c->stream, g, pl, (T2*)spec,
+                       (const T2*)tw, bsp);
+  else
+    hipLaunchKernelGGL((k_fftg_z<T, -1, NTS>), grid, dim3(FNT), lds, c->stream, g, pl, (T2*)spec,
+                       (const T2*)tw, bsp);
+}
+
+// ===========================================================================
 // size dispatch
 // ===========================================================================
 #define C3D_FFT_SIZES(X)                                                                       \
   X(8) X(12) X(16) X(18) X(20) X(24) X(32) X(36) X(40) X(48) X(64) X(72) X(80) X(96) X(128)    \
   X(144) X(160) X(192) X(256) X(288) X(320) X(384) X(512) X(576) X(640) X(768) X(1024)
 
-bool fft_size_supported(int n) {
+// lengths with compile-time kernels
+bool fft_size_compiled(int n) {
   switch (n) {
 #define X(n_) case n_:
     C3D_FFT_SIZES(X)
@@ -483,12 +757,21 @@ bool fft_size_supported(int n) {
   }
 }
 
+// every 2^a 3^b 5^c length up to GEN_NMAX: the compiled kernels or the
+// run-time plan
+bool fft_size_supported(int n) {
+  if (n < 2 || n > GEN_NMAX) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
 template <typename T, bool NTS>
 static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec, const Batch& b) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
-  switch (g.nk) {
+  switch (g.fftrt ? 0 : g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
     hipLaunchKernelGGL((k_fft_x_fwd<T, n_, T, NTS>),                                             \
@@ -498,7 +781,8 @@ static int fft_x_fwd_t(c3d_ctx* c, const void* in, void* spec, const Batch& b) {
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default:
+      gen_x_fwd<T, T, NTS>(c, (const T*)in, spec, c->tw_x, c->tw_xr, b.field, b.spec, b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -517,7 +801,7 @@ static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* u
   d.sudtal = (T)(1.0 / alpha);
   d.xnuhm = (T)(-0.5 * c->p.xnu);
   d.partials = c->d_partials;
-  switch (g.nk) {
+  switch (g.fftrt ? 0 : g.nk) {
 #define X(n_)                                                                                    \
   case n_: {                                                                                     \
     const int nb = (nrows + xlines(n_) - 1) / xlines(n_);                                       \
@@ -536,7 +820,7 @@ static int prsrc_fft_t(c3d_ctx* c, const void* ux, const void* uy, const void* u
   }
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default: set_error("prsrc_fft: the fused source pass needs a compiled x length"); return C3D_ERR_UNSUPPORTED;
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -547,7 +831,7 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale, co
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
-  switch (g.nk) {
+  switch (g.fftrt ? 0 : g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
     hipLaunchKernelGGL((k_fft_x_inv<T, n_, T, false, NTS>),                                      \
@@ -557,7 +841,9 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale, co
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default:
+      gen_x_inv<T, T, false, NTS>(c, spec, (T*)out, c->tw_x, c->tw_xr, (T)scale, b.spec, b.field,
+                                  b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -568,7 +854,7 @@ static int fft_z_t(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
   dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny, b.n);
-  switch (g.nz) {
+  switch (g.fftrt ? 0 : g.nz) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
     if (inverse)                                                                               \
@@ -580,7 +866,7 @@ static int fft_z_t(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_z: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default: gen_z<T, NTS>(c, spec, c->tw_z, inverse, b.spec, b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -591,7 +877,7 @@ template <typename TIN>
 static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec, const Batch& b) {
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
-  switch (g.nk) {
+  switch (g.fftrt ? 0 : g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
     hipLaunchKernelGGL((k_fft_x_fwd<float, n_, TIN>),                                            \
@@ -601,7 +887,8 @@ static int fft_x_fwd32(c3d_ctx* c, const TIN* in, void* spec, const Batch& b) {
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default:
+      gen_x_fwd<float, TIN, false>(c, in, spec, c->tw32_x, c->tw32_xr, b.field, b.spec32, b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -611,7 +898,7 @@ template <bool ACC>
 static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale, const Batch& b) {
   const Geom& g = c->g;
   const int nrows = g.ny * g.nz;
-  switch (g.nk) {
+  switch (g.fftrt ? 0 : g.nk) {
 #define X(n_)                                                                                    \
   case n_:                                                                                       \
     hipLaunchKernelGGL((k_fft_x_inv<float, n_, double, ACC>),                                   \
@@ -621,7 +908,9 @@ static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale, 
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_x: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default:
+      gen_x_inv<float, double, ACC, false>(c, spec, out, c->tw32_x, c->tw32_xr, scale, b.spec32,
+                                           b.field, b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
@@ -630,7 +919,7 @@ static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale, 
 static int fft_z32(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   const Geom& g = c->g;
   dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny, b.n);
-  switch (g.nz) {
+  switch (g.fftrt ? 0 : g.nz) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
     if (inverse)                                                                               \
@@ -642,7 +931,7 @@ static int fft_z32(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
     break;
     C3D_FFT_SIZES(X)
 #undef X
-    default: set_error("fft_z: unsupported length"); return C3D_ERR_UNSUPPORTED;
+    default: gen_z<float, false>(c, spec, c->tw32_z, inverse, b.spec32, b.n);
   }
   C3D_HIP(hipGetLastError());
   return C3D_OK;
