@@ -96,6 +96,31 @@ def run_cpu_baseline(case, ux, uy, uz, threads):
     return t_ppe, t_step
 
 
+def run_cpu_batch_baseline(threads):
+    """the oracle's clcPPE and clcU1Hat (substep 1) on one source of the
+    512x256x512 precision-sweep batch, on the host cores"""
+    from oracle import oracle as O
+    from channel3d_amd import synth
+    O.set_threads(threads)
+    c = BATCH_CASE
+    o = O.Oracle(c)
+    dyp = o.metric(O.M_DYP, 1, o.nyc)
+    f = synth.spectrum_field(o.nxc, o.nyc, o.nzc, seed=1000, dyp=dyp)
+    phi = np.zeros((o.nxc + 3, o.nyc + 3, o.nzc + 3), order="F")
+    t0 = time.perf_counter()
+    o.call("clcPPE", np.asfortranarray(f.copy()), phi)
+    t_ppe = time.perf_counter() - t0
+    o.call("PMcoeUpdate", 1)
+    u = o.field(O.UX)
+    u[...] = 0.0
+    t0 = time.perf_counter()
+    o.call("clcU1Hat", u, np.asfortranarray(f.copy()))
+    t_helm = time.perf_counter() - t0
+    log(f"cpu: 512x256x512 PPE {t_ppe:.2f}s, Helmholtz {t_helm:.2f}s")
+    o.close()
+    return t_ppe, t_helm
+
+
 def time_steps(g, steps, warmup, graph):
     """seconds for `steps` steps after `warmup` untimed ones, no profiling;
     graph: replay HIP graphs of the step (the warm-up captures them)"""
@@ -325,6 +350,95 @@ def first_substep_source(g, L):
     del saved
 
 
+# PPE + Helmholtz precision sweep: 512x256x512 stretched grid, batch of 8
+# spectrum sources, the production step's nu and dt
+BATCH_CASE = dict(nxc=512, nyc=256, nzc=512, xlx=2 * math.pi, yly=2.0, zlz=math.pi,
+                  xnu=9.9369e-5, dt=0.002, gamma=1.5, ubulk=1.0)
+BATCH_N = 8
+
+
+def batch_sources(g, L, nb):
+    from channel3d_amd import synth
+    dyp = g.metric(L.M_DYP, 1, g.nyc)
+    return [synth.spectrum_field(g.nxc, g.nyc, g.nzc, seed=1000 + q, dyp=dyp) for q in range(nb)]
+
+
+def run_batch_sweep(reps):
+    """batched PPE and Helmholtz (ux operator, substep 1) solves of 8
+    sources at 512x256x512 in FP64, FP32 and FP32 + 1 / 2 refinement sweeps:
+    ms per batch and per solve, error against the FP64 solve (max and L2,
+    relative, over the batch)"""
+    from channel3d_amd import Channel, lib as L
+    c = BATCH_CASE
+    g = Channel(c)
+    nb = BATCH_N
+    src = g.alloc_batch(L.INTERIOR, nb)
+    out = g.alloc_batch(L.HALO, nb)
+    du = g.alloc_batch(L.INTERIOR, nb)
+    t0 = time.perf_counter()
+    for q, f in enumerate(batch_sources(g, L, nb)):
+        g.upload(src[q], f, L.INTERIOR)
+    log(f"batch sweep: {nb} sources of {c['nxc']}x{c['nyc']}x{c['nzc']} made and uploaded "
+        f"({time.perf_counter() - t0:.1f}s)")
+    g.set_ppe_refine(2)     # FP32 tables
+    g.set_ppe_refine(-1)
+
+    def timed(fn):
+        fn()
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        g.sync()
+        return (time.perf_counter() - t0) / reps
+
+    def errs(ref, get):
+        mx = l2n = l2d = 0.0
+        for q in range(nb):
+            a = get(q)
+            d = a - ref[q]
+            mx = max(mx, float(np.abs(d).max() / np.abs(ref[q]).max()))
+            l2n += float((d * d).sum())
+            l2d += float((ref[q] * ref[q]).sum())
+        return mx, math.sqrt(l2n / l2d)
+
+    res = {}
+    for kind in ("ppe", "helmholtz"):
+        if kind == "ppe":
+            get = lambda q: g.download(out[q], L.HALO)[1:-2, 1:-2, 1:-2]
+            run = lambda s: g.ppe_batch(src, out, s)
+        else:
+            get = lambda q: g.download(du[q], L.INTERIOR)
+            run = lambda s: g.helmholtz_batch(0, 1, src, du, s)
+        rows = []
+        ref = None
+        for s in (-1, 0, 1, 2):
+            t = timed(lambda: run(s))
+            if s == -1:
+                ref = [get(q).copy() for q in range(nb)]
+                mx, l2 = 0.0, 0.0
+            else:
+                mx, l2 = errs(ref, get)
+            form = "fp64" if s < 0 else f"fp32+{s}"
+            rows.append(dict(form=form, ms_per_batch=round(1e3 * t, 4),
+                             ms_per_solve=round(1e3 * t / nb, 4),
+                             solves_per_s=round(nb / t, 2), max_rel_err_vs_fp64=mx,
+                             l2_rel_err_vs_fp64=l2))
+            log(f"{kind} batch {form}: {1e3 * t:.2f} ms per batch of {nb}, max err {mx:.2e}, "
+                f"L2 err {l2:.2e}")
+        del ref
+        res[kind] = rows
+    g.close()
+    return dict(workload=f"PPE + Helmholtz precision sweep {c['nxc']}x{c['nyc']}x{c['nzc']}, "
+                         f"batch of {nb} sources with E(k) ~ k^4 exp(-2 (k/8)^2), random phases, "
+                         "compatible; Helmholtz: the ux operator of RK substep 1 with the "
+                         "production nu and dt",
+                nbatch=nb, **res,
+                note="errors against the batch's own FP64 solve, whose parity with the "
+                     "oracle is tested at 64x48x64 (1e-12) and whose linearity at this size is "
+                     "tested to 1e-12")
+
+
 def run_precision_sweep(g, L, nsolves, fp64_per_s, nsteps):
     """FP64 pressure solve against FP32 + s refinement sweeps on the same
     source (ARR1, a first substep's from the current state): solves/s and relative
@@ -419,6 +533,8 @@ def main(argv=None):
                     help="skip the learned-closure training measurement")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32 + refinement pressure-solve sweep")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched PPE + Helmholtz precision sweep (512x256x512 x 8)")
     ap.add_argument("--probe", choices=["copy"], default=None,
                     help="only run plain field copies (HBM reference for traffic counters)")
     a = ap.parse_args(argv)
@@ -539,10 +655,15 @@ def main(argv=None):
         threads = cpu_threads(a.cpu_threads)
         log(f"cpu baseline: oracle with {threads} threads")
         t_ppe_cpu, t_step_cpu = run_cpu_baseline(case, ux, uy, uz, threads)
+        t_bppe, t_bhelm = (None, None) if a.no_batch else run_cpu_batch_baseline(threads)
         avail, total = host_cores()
         cpu = dict(value=round(1.0 / t_step_cpu, 5), unit="RK3 steps/s", cores=threads,
                    host_cores_available=avail, host_cpu_count=total,
                    kind="port", poisson_solves_per_s=round(1.0 / t_ppe_cpu, 4),
+                   batch_sweep_512x256x512=None if t_bppe is None else dict(
+                       ppe_solves_per_s=round(1.0 / t_bppe, 4),
+                       helmholtz_solves_per_s=round(1.0 / t_bhelm, 4),
+                       sample="one source of the batch, FP64"),
                    sample=(f"oracle (C, FP64, OpenMP, {threads} threads = every core the "
                            f"job may use) on the host: one PPE solve and one whole RK3 step "
                            f"(CFL, 3 substeps, divergence and vmax) at {nx}x{ny}x{nz}"))
@@ -562,6 +683,10 @@ def main(argv=None):
             log(f"LES step {key}")
             les[key] = run_les(case_l, max(a.steps, 20), max(a.warmup, 2), lbl)
             log(f"{key}: {les[key]['ms_per_step']} ms/step")
+    batch = None
+    if not a.no_batch and not a.small:
+        log("batched PPE + Helmholtz precision sweep 512x256x512 x 8")
+        batch = run_batch_sweep(5)
     closure = None
     if not a.no_closure and not a.small:
         log("learned closure 384x192x288 -> 96x96x72")
@@ -597,6 +722,7 @@ def main(argv=None):
         "les": les,
         "closure": closure,
         "precision_sweep": precision,
+        "batch_precision_sweep": batch,
     }
     print(json.dumps(out), flush=True)
     g.close()

@@ -353,98 +353,6 @@ static void thomas_factors(const std::vector<double>& a, const std::vector<doubl
   }
 }
 
-// Partitioned form of a tridiagonal y system shared by every (i, k) line:
-// SPK_P partitions of M rows (rows past n are decoupled identity rows).
-// out: a, vecm, arr of each partition's own LU (InverseTridiagonal's
-// recurrence restarted at the partition's first row), the spikes
-// v = A_p^-1 (a_first e_first) and w = A_p^-1 (c_last e_last), and the
-// inverse of the 2P x 2P system for the partitions' first and last
-// unknowns.  Spikes and inverse are formed in long double.
-static void spike_table(const std::vector<double>& a, const std::vector<double>& b,
-                        const std::vector<double>& cc, int n, int M, double* out) {
-  const int P = SPK_P, N = P * M, R = 2 * P;
-  std::vector<double> A(N), B(N), C(N);
-  for (int r = 0; r < N; ++r) {
-    const int j = r + 1;
-    A[r] = j <= n ? a[j] : 0.0;
-    B[r] = j <= n ? b[j] : 1.0;
-    C[r] = j < n ? cc[j] : 0.0;
-  }
-  double* ta = out;
-  double* tv = out + N;
-  double* tr = out + 2 * N;
-  double* sv = out + 3 * N;
-  double* sw = out + 4 * N;
-  double* mi = out + 5 * N;
-  std::vector<long double> vl(N, 0.0L), wl(N, 0.0L);
-  for (int p = 0; p < P; ++p) {
-    const int s0 = p * M;
-    tv[s0] = B[s0];
-    tr[s0] = 0.0;
-    for (int r = s0 + 1; r < s0 + M; ++r) {
-      tr[r] = C[r - 1] / tv[r - 1];
-      tv[r] = B[r] - A[r] * tr[r];
-    }
-    for (int r = s0; r < s0 + M; ++r) ta[r] = r == s0 ? 0.0 : A[r];
-    // local solve in long double for the two spikes
-    auto solve = [&](std::vector<long double>& x, int row, long double val) {
-      std::vector<long double> vm(M), ar(M), f(M, 0.0L);
-      f[row] = val;
-      vm[0] = B[s0];
-      f[0] = f[0] / vm[0];
-      for (int q = 1; q < M; ++q) {
-        ar[q] = (long double)C[s0 + q - 1] / vm[q - 1];
-        vm[q] = (long double)B[s0 + q] - (long double)A[s0 + q] * ar[q];
-        f[q] = (f[q] - (long double)A[s0 + q] * f[q - 1]) / vm[q];
-      }
-      for (int q = M - 2; q >= 0; --q) f[q] = f[q] - ar[q + 1] * f[q + 1];
-      for (int q = 0; q < M; ++q) x[s0 + q] = f[q];
-    };
-    if (p > 0) solve(vl, 0, (long double)A[s0]);
-    if (p < P - 1) solve(wl, M - 1, (long double)C[s0 + M - 1]);
-  }
-  for (int r = 0; r < N; ++r) {
-    sv[r] = (double)vl[r];
-    sw[r] = (double)wl[r];
-  }
-  // reduced system: t[2p] = first unknown of partition p, t[2p+1] = last;
-  // t[2p(+1)] + v_{first/last} t[2p-1] + w_{first/last} t[2p+2] = y_{first/last}
-  std::vector<long double> m(R * R, 0.0L), inv(R * R, 0.0L);
-  for (int p = 0; p < P; ++p) {
-    const int s0 = p * M, e0 = s0 + M - 1;
-    for (int h = 0; h < 2; ++h) {
-      const int row = 2 * p + h, r = h ? e0 : s0;
-      m[row * R + row] = 1.0L;
-      if (p > 0) m[row * R + 2 * p - 1] += vl[r];
-      if (p < P - 1) m[row * R + 2 * p + 2] += wl[r];
-    }
-  }
-  for (int q = 0; q < R; ++q) inv[q * R + q] = 1.0L;
-  for (int col = 0; col < R; ++col) {  // Gauss-Jordan, partial pivoting
-    int piv = col;
-    for (int r = col + 1; r < R; ++r)
-      if (fabsl(m[r * R + col]) > fabsl(m[piv * R + col])) piv = r;
-    for (int q = 0; q < R; ++q) {
-      std::swap(m[col * R + q], m[piv * R + q]);
-      std::swap(inv[col * R + q], inv[piv * R + q]);
-    }
-    const long double d = m[col * R + col];
-    for (int q = 0; q < R; ++q) {
-      m[col * R + q] /= d;
-      inv[col * R + q] /= d;
-    }
-    for (int r = 0; r < R; ++r)
-      if (r != col) {
-        const long double f = m[r * R + col];
-        for (int q = 0; q < R; ++q) {
-          m[r * R + q] -= f * m[col * R + q];
-          inv[r * R + q] -= f * inv[col * R + q];
-        }
-      }
-  }
-  for (int q = 0; q < R * R; ++q) mi[q] = (double)inv[q];
-}
-
 // per substep: x taps (nx), z taps (nz), then ten y vectors of ny+2: the
 // cell operator's a, vecm, arr, the node operator's a, vecm, arr, the cell
 // operator's b and c, the node operator's b and c
@@ -457,9 +365,6 @@ static int build_adi_tables(c3d_ctx* c) {
   const size_t L = (size_t)ny + 2;
   const size_t per_ns = adi_table_len(c->g);
   std::vector<double> host(3 * per_ns, 0.0);
-  const int M = c->spike_m;
-  const size_t spl = M > 0 ? (size_t)spike_table_len(M) : 0;
-  std::vector<double> shost(6 * spl, 0.0);
   for (int s = 0; s < 3; ++s) {
     auto& tb = c->adi[s];
     double* hb = host.data() + s * per_ns;
@@ -482,10 +387,6 @@ static int build_adi_tables(c3d_ctx* c) {
       b[j] = c->order == 4 ? -beta * c->met[C3D_M_AC2CCN][j] + 1.0 : -cc[j] - a[j] + 1.0;
     }
     thomas_factors(a, b, cc, ny, vecm, arr);
-    if (M > 0) {
-      spike_table(a, b, cc, ny, M, shost.data() + (2 * s) * spl);
-      tb.d_spk[0] = (double*)c->d_spike + (2 * s) * spl;
-    }
     double* yb = hb + nx + nz;
     std::memcpy(yb + 0 * L, a.data(), sizeof(double) * L);
     std::memcpy(yb + 1 * L, vecm.data(), sizeof(double) * L);
@@ -502,10 +403,6 @@ static int build_adi_tables(c3d_ctx* c) {
       a[j] = -beta * c->met[C3D_M_AM2P][j];
     }
     thomas_factors(a, b, cc, ny, vecm, arr);
-    if (M > 0) {
-      spike_table(a, b, cc, ny, M, shost.data() + (2 * s + 1) * spl);
-      tb.d_spk[1] = (double*)c->d_spike + (2 * s + 1) * spl;
-    }
     std::memcpy(yb + 3 * L, a.data(), sizeof(double) * L);
     std::memcpy(yb + 4 * L, vecm.data(), sizeof(double) * L);
     std::memcpy(yb + 5 * L, arr.data(), sizeof(double) * L);
@@ -516,9 +413,6 @@ static int build_adi_tables(c3d_ctx* c) {
   }
   C3D_HIP(hipMemcpyAsync(c->d_tabs, host.data(), sizeof(double) * host.size(),
                          hipMemcpyHostToDevice, c->stream));
-  if (M > 0)
-    C3D_HIP(hipMemcpyAsync(c->d_spike, shost.data(), sizeof(double) * shost.size(),
-                           hipMemcpyHostToDevice, c->stream));
   C3D_HIP(hipStreamSynchronize(c->stream));
   return C3D_OK;
 }
@@ -678,7 +572,7 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
     if (c->state[i]) hipFree(c->state[i]);
   for (void* p : c->user_fields) hipFree(p);
   void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
-                  c->d_dyp, c->d_spike, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
+                  c->d_dyp, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
                   c->d_lesmet, c->d_spec1d, c->spec2, c->d_spec2d, c->cln_d, c->cln_f,
                   c->cln_fb, c->bspec, c->bsarr, c->bmean, c->bres, c->btmp};
   for (void* p : bufs)
@@ -830,31 +724,17 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   // implicit-operator tables
   {
     if ((rc = dalloc(c, &c->d_tabs, 3 * adi_table_len(g) * sizeof(double)))) return fail(rc);
-    // y solver: Thomas with segment checkpoints (default), Thomas storing the
-    // forward pass (C3D_Y_SOLVER=store), or the partitioned solve
-    // (C3D_Y_SOLVER=partitioned, when a row count per lane fits the registers)
-    c->spike_m = 0;
+    // y solver: Thomas with segment checkpoints (default) or Thomas storing the
+    // forward pass (C3D_Y_SOLVER=store; the same operations, equal results)
     c->ymode = YS_CHECKPOINT;
     if (const char* ys = std::getenv("C3D_Y_SOLVER")) {
       const std::string v(ys);
       if (v == "store") c->ymode = YS_STORE;
-      else if (v == "partitioned") c->ymode = YS_PARTITIONED;
       else if (v != "checkpoint") {
-        set_error("c3d_create: C3D_Y_SOLVER must be checkpoint, store or partitioned");
+        set_error("c3d_create: C3D_Y_SOLVER must be checkpoint or store");
         return fail(C3D_ERR_ARG);
       }
     }
-    if (c->ymode == YS_PARTITIONED) {
-      for (int q = 0; q < SPK_NM; ++q)
-        if (SPK_P * SPK_MS[q] >= g.ny) {
-          c->spike_m = SPK_MS[q];
-          break;
-        }
-      if (c->spike_m == 0) c->ymode = YS_CHECKPOINT;
-    }
-    if (c->spike_m > 0 &&
-        (rc = dalloc(c, &c->d_spike, 6 * (size_t)spike_table_len(c->spike_m) * sizeof(double))))
-      return fail(rc);
     if ((rc = build_adi_tables(c))) return fail(rc);
   }
   rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);

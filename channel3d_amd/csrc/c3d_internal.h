@@ -159,7 +159,7 @@ struct LesMet {
   const T* ap2ccn;
 };
 
-enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1, YS_PARTITIONED = 2 };
+enum YSolver { YS_CHECKPOINT = 0, YS_STORE = 1 };
 
 // A batch of n fields solved by one launch per pass: field q lies `field`
 // reals after field 0, its spectrum `spec` complex elements after (FP64
@@ -216,13 +216,10 @@ struct c3d_ctx {
     double* d_yc[5];          // CN ux/uz operator: a, vecm, arr, then the matrix's
                               // diagonal b and upper c (ny+2 each)
     double* d_yp[5];          // uy operator
-    double* d_spk[2];         // partitioned-solve tables (cell / node operator)
   } adi[3];
   void* d_tabs = nullptr;
-  // y solver (YS_*); partitioned: SPK_P lanes per line, spike_m rows per lane
+  // y solver (YS_*): checkpointed sweeps, or the sweeps storing the forward pass
   int ymode = 0;
-  int spike_m = 0;
-  void* d_spike = nullptr;
 
   // state and scratch
   void* state[C3D_NSTATE] = {};
@@ -357,11 +354,6 @@ int launch_diag(c3d_ctx* c, const void* ux, const void* uy, const void* uz, int 
 int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot);
 
 // ---- launchers (k_adi.hip) ----
-// partitioned y solve: threads per line, rows per thread chosen from SPK_MS
-constexpr int SPK_P = 16;
-constexpr int SPK_NM = 6;
-constexpr int SPK_MS[SPK_NM] = {2, 4, 8, 16, 24, 32};
-__host__ __device__ constexpr int spike_table_len(int m) { return 5 * SPK_P * m + 4 * SPK_P * SPK_P; }
 // comp: 0 ux (CN metrics), 1 uy (node metrics + wall row), 2 uz (CN metrics).
 // add_dpmdxns: 2nd order adds dpmdxns to RhsX during the x sweep; 4th order
 // (y solve only) subtracts dp1ns from RhsX as the sweep loads it
@@ -439,6 +431,19 @@ inline long long batch_mean_len(const Geom& g) {
 // batch scratch for nb fields (mixed: also the FP32 residual fields)
 int ensure_batch(c3d_ctx* c, int nb, bool mixed);
 int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps);
+// work space of a batch of pressure solves: nb spectra (nspec FP64 complex
+// each), checkpoints (nspec reals each), (0,0)-mode work (batch_mean_len
+// doubles each), FP32 residual fields (nfield floats each; mixed solves only)
+struct BatchWork {
+  void* spec = nullptr;
+  void* ck = nullptr;
+  double* mean = nullptr;
+  float* resid = nullptr;
+};
+// the batched pressure solve on a given work space; allocates nothing (the
+// mixed solve's FP32 tables must exist)
+int ppe_batch_ws(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps,
+                 const BatchWork& ws);
 // (1 - b Dxx)(1 - b Dzz)(1 - b Dyy) du = rhs for nb fields (k_adi.hip)
 int run_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nb, const void* rhs, void* du,
                         int sweeps);

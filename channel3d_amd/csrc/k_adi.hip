@@ -373,116 +373,11 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
   }
 }
 
-// ---------------------------------------------------------------------------
-// The same y solve, every line read once and written once.  Every (i, k)
-// line shares the matrix, so the line is split over SPK_P threads of M rows:
-// each thread solves its partition with the partition's own LU factors, the
-// threads publish their first and last values in LDS, the 2P boundary
-// unknowns follow from the precomputed inverse of the reduced system, and
-// each thread adds its spike corrections (x = y - x_left v - x_right w)
-// before the update u += x.  Rows stay in registers; the partial solution
-// never goes back to memory.  Tables (host-built, shared by all lines): a,
-// vecm, arr of the partition LUs, the spikes v and w, and the reduced
-// inverse.  A workgroup holds 16 lines (consecutive i, one 128-byte segment
-// per row) x 16 partitions.
-// ---------------------------------------------------------------------------
-template <typename T, int M, bool WALLROW, bool SHIFT>
-__global__ __launch_bounds__(256) void k_adi_y_spk(Geom g, const double* __restrict__ tab,
-                                                   const T* __restrict__ r, T* __restrict__ u,
-                                                   const T* __restrict__ duy,
-                                                   const DevScalars* __restrict__ sc) {
-  constexpr int P = SPK_P, PM = P * M, R = 2 * P;
-  __shared__ double st[spike_table_len(M)];
-  __shared__ T xb[R][16];
-  for (int q = threadIdx.x; q < spike_table_len(M); q += blockDim.x) st[q] = tab[q];
-  const double* ta = st;
-  const double* tv = st + PM;
-  const double* tr = st + 2 * PM;
-  const double* sv = st + 3 * PM;
-  const double* sw = st + 4 * PM;
-  const double* mi = st + 5 * PM;
-  const int l = threadIdx.x & 15, p = threadIdx.x >> 4;
-  const int lpk = (g.nx + 15) >> 4;                      // 16-line groups per k plane
-  const int k = blockIdx.x / lpk;
-  const int i = (blockIdx.x - k * lpk) * 16 + l;
-  const bool act = i < g.nx && k < g.nz;
-  const int ny = g.ny;
-  const long long base = (long long)(act ? i : 0) + g.plane * (long long)(act ? k : 0);
-  const long long sj = g.nx;
-  const int j0 = p * M + 1;                              // reference row of q = 0
-  const T shift = SHIFT ? (T)sc->dp1ns : T(0);
-  T y[M];
-#pragma unroll
-  for (int q = 0; q < M; ++q) {
-    const int j = j0 + q;
-    y[q] = (act && j <= ny) ? (SHIFT ? r[base + sj * j] - shift : r[base + sj * j]) : T(0);
-  }
-  if (WALLROW && p == 0) y[0] = act ? duy[i + (long long)g.nx * k] : T(0);
-  __syncthreads();                                       // tables in LDS
-  // partition LU: forward and backward substitution (InverseTridiagonal's
-  // operation order within the partition)
-  y[0] = y[0] / (T)tv[p * M];
-#pragma unroll
-  for (int q = 1; q < M; ++q) y[q] = (y[q] - (T)ta[p * M + q] * y[q - 1]) / (T)tv[p * M + q];
-#pragma unroll
-  for (int q = M - 2; q >= 0; --q) y[q] = y[q] - (T)tr[p * M + q + 1] * y[q + 1];
-  xb[2 * p][l] = y[0];
-  xb[2 * p + 1][l] = y[M - 1];
-  __syncthreads();
-  T xl = T(0), xr = T(0);
-  if (p > 0) {
-#pragma unroll
-    for (int m = 0; m < R; ++m) xl += (T)mi[(2 * p - 1) * R + m] * xb[m][l];
-  }
-  if (p < P - 1) {
-#pragma unroll
-    for (int m = 0; m < R; ++m) xr += (T)mi[(2 * p + 2) * R + m] * xb[m][l];
-  }
-  if (!act) return;
-#pragma unroll
-  for (int q = 0; q < M; ++q) {
-    const int j = j0 + q;
-    if (j <= ny) {
-      const T x = y[q] - xl * (T)sv[p * M + q] - xr * (T)sw[p * M + q];
-      u[base + sj * j] = u[base + sj * j] + x;
-    }
-  }
-}
-
-template <typename T, bool WALLROW, bool SHIFT>
-static bool launch_y_spk(c3d_ctx* c, const double* tab, const void* rhs, void* u,
-                         const void* duy) {
-  const Geom& g = c->g;
-  const long long groups = (long long)((g.nx + 15) / 16) * g.nz;
-  const dim3 grid((unsigned)groups), blk(256);
-  const DevScalars* sc = (const DevScalars*)c->d_sc;
-  switch (c->spike_m) {
-#define SPK(MM)                                                                                 \
-  case MM:                                                                                      \
-    hipLaunchKernelGGL((k_adi_y_spk<T, MM, WALLROW, SHIFT>), grid, blk, 0, c->stream, g, tab,   \
-                       (const T*)rhs, (T*)u, (const T*)duy, sc);                                \
-    return true;
-    SPK(2) SPK(4) SPK(8) SPK(16) SPK(24) SPK(32)
-#undef SPK
-    default: return false;
-  }
-}
-
-// y solve + update of one component; the partitioned kernel when tables exist
+// y solve + update of one component
 template <typename T>
 static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, void* u,
                    const void* duy_ym, bool shift) {
   const Geom& g = c->g;
-  if (c->spike_m > 0) {
-    const double* tab = tb.d_spk[comp == 1 ? 1 : 0];
-    bool ok = comp == 1 ? launch_y_spk<T, true, false>(c, tab, rhs, u, duy_ym)
-              : shift   ? launch_y_spk<T, false, true>(c, tab, rhs, u, nullptr)
-                        : launch_y_spk<T, false, false>(c, tab, rhs, u, nullptr);
-    if (ok) {
-      C3D_HIP(hipGetLastError());
-      return C3D_OK;
-    }
-  }
   dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   const DevScalars* sc = (const DevScalars*)c->d_sc;

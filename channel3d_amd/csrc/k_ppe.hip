@@ -493,9 +493,9 @@ int run_ppe_from_spectrum(c3d_ctx* c, void* phi) {
 // sweeps < 0: the FP64 solve (the kernels of the single solve, so each
 // field's result equals c3d_ppe's bit for bit); sweeps >= 0: the
 // mixed-precision solve with that many refinement sweeps.
-int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps) {
+int ppe_batch_ws(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps,
+                 const BatchWork& ws) {
   int rc;
-  if ((rc = ensure_batch(c, nb, sweeps >= 0))) return rc;
   const Geom& g = c->g;
   Batch b;
   b.n = nb;
@@ -505,12 +505,23 @@ int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps) 
   b.ck = g.nspec;
   b.mean = batch_mean_len(g);
   if (sweeps >= 0)
-    return ppe_mixed(c, prsrc, phi, sweeps, c->bspec, c->bsarr, c->bmean, (float*)c->bres, b);
-  if ((rc = launch_fft_x_fwd(c, prsrc, c->bspec, b))) return rc;
-  if ((rc = launch_fft_z(c, c->bspec, 0, b))) return rc;
-  if ((rc = launch_tridiag(c, c->bspec, c->bsarr, b))) return rc;
-  if ((rc = launch_fft_z(c, c->bspec, 1, b))) return rc;
-  return launch_fft_x_inv(c, c->bspec, phi, c->normfft, b);
+    return ppe_mixed(c, prsrc, phi, sweeps, ws.spec, ws.ck, ws.mean, ws.resid, b);
+  if ((rc = launch_fft_x_fwd(c, prsrc, ws.spec, b))) return rc;
+  if ((rc = launch_fft_z(c, ws.spec, 0, b))) return rc;
+  if ((rc = launch_tridiag(c, ws.spec, ws.ck, b))) return rc;
+  if ((rc = launch_fft_z(c, ws.spec, 1, b))) return rc;
+  return launch_fft_x_inv(c, ws.spec, phi, c->normfft, b);
+}
+
+int run_ppe_batch(c3d_ctx* c, int nb, const void* prsrc, void* phi, int sweeps) {
+  int rc;
+  if ((rc = ensure_batch(c, nb, sweeps >= 0))) return rc;
+  BatchWork ws;
+  ws.spec = c->bspec;
+  ws.ck = c->bsarr;
+  ws.mean = c->bmean;
+  ws.resid = (float*)c->bres;
+  return ppe_batch_ws(c, nb, prsrc, phi, sweeps, ws);
 }
 
 }  // namespace c3d

@@ -570,6 +570,17 @@ void orc_clcRhsX(orc_ctx* c, const double* ux, const double* uy, const double* u
     for (int kc = 1; kc <= nzc; ++kc)
       for (int q = 0; q < 4; ++q) tot[q] += s1k[4 * (size_t)kc + q];
     double s1tot1 = xsum_value(tot);
+    if (c->sum_seq) {
+      /* :46, one rank: s1tot = s1tot + ux * s1totCoe in loop order */
+      double s1tot = 0.0;
+      for (int kc = 1; kc <= nzc; ++kc)
+        for (int jc = 1; jc <= nyc; ++jc) {
+          double s1totCoe = dyp[jc];
+          if (yc[jc] < c->p.ybulk1 || yc[jc] >= c->p.ybulk2) s1totCoe = 0.0;
+          for (int ic = 1; ic <= nxc; ++ic) s1tot = s1tot + UX(ic, jc, kc) * s1totCoe;
+        }
+      s1tot1 = s1tot;
+    }
     s1tot1 = s1tot1 / ((double)nxc * (double)nzc) / (c->p.ybulk2 - c->p.ybulk1);
     double dpmdxns = c->pmAlphaC * (c->p.ubulk - s1tot1);
 #ifdef _OPENMP
@@ -1129,6 +1140,7 @@ static void ppe_tridiag(orc_ctx* c, double* arr) {
 }
 
 void orc_set_ppe_route(orc_ctx* c, int route) { c->ppe_route = route ? 1 : 0; }
+void orc_set_sum_order(orc_ctx* c, int seq) { c->sum_seq = seq ? 1 : 0; }
 
 void orc_clcPPE(orc_ctx* c, double* prsrc, double* prphi) {
   const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
@@ -1265,10 +1277,14 @@ double orc_CalcUxAver(orc_ctx* c, const double* ux) {
   const int nxc = c->nxc, nyc = c->nyc, nzc = c->nzc;
   const double* dyp = c->met[ORC_DYP];
   long long res1[4] = {0, 0, 0, 0};
+  double seq = 0.0;
   for (int kc = 1; kc <= nzc; ++kc)
     for (int jc = 1; jc <= nyc; ++jc)
-      for (int ic = 1; ic <= nxc; ++ic) xsum_add(res1, UX(ic, jc, kc) * dyp[jc]);
-  return xsum_value(res1) / ((double)nxc * (double)nzc) / c->p.yly;
+      for (int ic = 1; ic <= nxc; ++ic) {
+        xsum_add(res1, UX(ic, jc, kc) * dyp[jc]);
+        seq = seq + UX(ic, jc, kc) * dyp[jc];
+      }
+  return (c->sum_seq ? seq : xsum_value(res1)) / ((double)nxc * (double)nzc) / c->p.yly;
 }
 
 /* ------------------------------------------------------------------------- */

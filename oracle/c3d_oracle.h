@@ -107,6 +107,11 @@ void orc_clcPrSrc(orc_ctx* c, const double* ux, const double* uy, const double* 
 void orc_clcPPE(orc_ctx* c, double* prsrc, double* prphi_halo);
 /* 0: clcPPE_x1 route (default), 1: clcPPE_z1 route (m_Poisson.f90:1078-1197) */
 void orc_set_ppe_route(orc_ctx* c, int route);
+/* seq = 1: clcRhsX_FIMP's s1tot and CalcUxAver are summed as the reference
+ * sums them on one rank, a double accumulator in kc, jc, ic order
+ * (m_TSchemeFIMP_inc.f90:46, m_Tools.f90:262); seq = 0 (default): the exact
+ * fixed-point sum, independent of order (what the library computes) */
+void orc_set_sum_order(orc_ctx* c, int seq);
 void orc_SetBC_and_UpdateHalo_pr(orc_ctx* c, double* pr);
 void orc_FluidVelUpdate(orc_ctx* c, const double* prphi_halo, double* ux, double* uy,
                         double* uz);

@@ -24,6 +24,8 @@ struct orc_ctx {
   double PrGradNow, PrGradAver;
   double divmax1, divmax2, cfl, uxaver, vmax[3];
   int ppe_route;
+  int sum_seq;                  /* 1: the bulk sums in the reference's own sequential
+                                   order instead of the exact fixed-point sum */
   double fcoe[9];               /* LES test-filter weights (InitFilterCoe) */
   double* f[ORC_NFIELDS];
   fftplan px, pz;

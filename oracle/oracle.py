@@ -124,6 +124,7 @@ def lib():
         L.orc_clcPrSrc.argtypes = [ctypes.c_void_p] + [_DP] * 6
         L.orc_clcPPE.argtypes = [ctypes.c_void_p, _DP, _DP]
         L.orc_set_ppe_route.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_sum_order.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_FluidVelUpdate.argtypes = [ctypes.c_void_p] + [_DP] * 4
         L.orc_PressureUpdate.argtypes = [ctypes.c_void_p, _DP, _DP]
         for name in ("orc_CalcMaxCFL", "orc_CheckDivergence"):
@@ -351,6 +352,11 @@ class Oracle:
     def set_ppe_route(self, route):
         """0: clcPPE_x1 (x first), 1: clcPPE_z1 (z first)"""
         lib().orc_set_ppe_route(self._h, int(route))
+
+    def set_sum_order(self, seq):
+        """1: s1tot and CalcUxAver summed in the reference's sequential order;
+        0: the exact fixed-point sum (default, what the library computes)"""
+        lib().orc_set_sum_order(self._h, int(seq))
 
     def call(self, name, *arrays):
         fn = getattr(lib(), "orc_" + name)

@@ -111,13 +111,65 @@ def test_gradient_matches_oracle():
     go = C.backward(p, X, T, emulate=True)
     lo = C.loss(p, X, T, emulate=True)
     assert abs(loss - lo) <= 1e-5 * lo
-    # each block's gradient: relative 2-norm error below 1e-2 (bf16 operands
-    # of the products), and the GPU's W1 rows 10..15 stay exactly zero
-    for k in go:
-        e = np.linalg.norm(g[k] - go[k]) / np.linalg.norm(go[k])
-        assert e < 1e-2, (k, e)
+    # Each weight-gradient element is a sum over the samples of products of
+    # bf16 operands, accumulated in FP32.  Its rounding scale is the sum of
+    # the terms' magnitudes, s = |A|^T |dZ|: FP32 accumulation in another
+    # order moves it by ~2^-24 s per term, and an operand that rounds to the
+    # neighbouring bf16 value (its FP32 input differing in the last bits)
+    # moves one term by 2^-8 of itself.  Bars on |g - go| / s: 99th
+    # percentile 2^-12, maximum 2^-7; the biases, sums of one operand,
+    # likewise with s = sum |dZ|.
+    a_in = _layer_inputs(C, p, X)
+    dzs = _layer_dz(C, p, X, T)
+    worst = {}
+    for l in range(1, 5):
+        sw = np.abs(a_in[l - 1]).T @ np.abs(C.bf16(dzs[l - 1]).astype(np.float64))
+        sb = np.abs(dzs[l - 1]).sum(axis=0)
+        for k, sc in ((f"W{l}", sw), (f"b{l}", sb)):
+            live = sc > 0
+            r = np.abs(g[k] - go[k])[live] / sc[live]
+            worst[k] = (float(np.quantile(r, 0.99)), float(r.max()))
+            assert worst[k][0] < 2.0 ** -12 and worst[k][1] < 2.0 ** -7, (k, worst[k])
+    print("gradient error / rounding scale (p99, max):",
+          {k: f"{v[0]:.1e} {v[1]:.1e}" for k, v in worst.items()})
     assert not g["W1"][10:].any()
+    # a well-conditioned case: output bias +100, so every output error
+    # y - T (targets clipped to +-10) is positive and the b4 and W4 sums have
+    # no cancellation: per element relative to the value, p99 1e-4, max 1e-3
+    # (a systematic error of 0.1 % in these blocks fails)
+    q = {k: v.copy() for k, v in p.items()}
+    q["b4"] = q["b4"] + np.float32(100.0)
+    cl.set_params(q)
+    g2, _ = cl.grad(first, n)
+    go2 = C.backward(q, X, T, emulate=True)
+    for k in ("b4", "W4"):
+        big = np.abs(go2[k]) > 1e-3 * np.abs(go2[k]).max()
+        r = np.abs(g2[k] - go2[k])[big] / np.abs(go2[k])[big]
+        print(f"{k} one-signed: rel err p99 {np.quantile(r, 0.99):.1e} max {r.max():.1e}")
+        assert big.mean() > 0.9 and np.quantile(r, 0.99) < 1e-4 and r.max() < 1e-3, (k, r.max())
     cl.close()
+
+
+def _layer_inputs(C, p, X):
+    """the bf16-rounded input of each layer, as the forward pass uses it"""
+    _, cache = C.forward(p, X, emulate=True)
+    return [a.astype(np.float64) for a, _ in cache]
+
+
+def _layer_dz(C, p, X, T):
+    """dL/dZ of each layer (FP32, before bf16 rounding), the backward pass's"""
+    y, cache = C.forward(p, X, emulate=True)
+    n = X.shape[0]
+    dz = ((2.0 / (n * C.NOUT)) * (y - T.astype(np.float32))).astype(np.float32)
+    out = [None] * 4
+    for l in range(4, 0, -1):
+        _, z = cache[l - 1]
+        if l < 4:
+            dz = (dz * C.silu_grad(z)).astype(np.float32)
+        out[l - 1] = dz
+        if l > 1:
+            dz = C.bf16(dz).astype(np.float32) @ C.bf16(p[f"W{l}"]).astype(np.float32).T
+    return out
 
 
 def test_adam_step_matches_oracle():

@@ -415,3 +415,40 @@ def test_nontemporal_variants_bitwise(O, name, monkeypatch):
     assert g0.diagnostics() == g1.diagnostics()
     g0.close()
     g1.close()
+
+
+@pytest.mark.parametrize("ischeme", [1, 2], ids=["ab2", "rk2"])
+def test_ab2_rk2_steps(O, ischeme):
+    """the AB2 and RK2 tables (ischeme 1, 2; m_Parameters.f90:135-142):
+    free-running steps against the oracle with the same scheme"""
+    case = dict(cases.SMALL, ischeme=ischeme)
+    o, g = make_pair(O, case, seed=6)
+    for _ in range(4):
+        o.iterate()
+        g.step()
+    compare_state(O, o, g, TOL)
+    d = g.diagnostics()
+    assert d["divmax2"] < 1e-12
+    assert abs(d["SimTime"] - o.scalar(O.S_SIMTIME)) < 1e-13
+    g.close()
+
+
+def test_store_y_solver_bitwise(O, monkeypatch):
+    """C3D_Y_SOLVER=store (the two-pass Thomas sweeps storing the forward
+    pass, also the fallback for lines too long for the checkpoints) runs the
+    same operations as the default checkpointed sweeps: equal bit for bit"""
+    from channel3d_amd import Channel, lib as L
+    case = cases.ODD
+    o, g0 = make_pair(O, case, seed=8)
+    monkeypatch.setenv("C3D_Y_SOLVER", "store")
+    g1 = Channel(case, yp=o.yp)
+    monkeypatch.delenv("C3D_Y_SOLVER")
+    for gid in (L.UX, L.UY, L.UZ, L.PR):
+        g1.upload(gid, g0.download(gid))
+    for _ in range(2):
+        g0.step()
+        g1.step()
+    for gid in (L.UX, L.UY, L.UZ, L.PR, L.HISTX, L.HISTY, L.HISTZ):
+        assert np.array_equal(g0.download(gid), g1.download(gid)), gid
+    g0.close()
+    g1.close()

@@ -180,11 +180,15 @@ def assemble_laplacian(o, O):
 
 
 @pytest.mark.parametrize("shape", [(8, 6, 8), (12, 9, 10)])
-def test_ppe_vs_sparse_direct(oracle_mod, shape):
+@pytest.mark.parametrize("route", [0, 1], ids=["x1", "z1"])
+def test_ppe_vs_sparse_direct(oracle_mod, shape, route):
+    """both PPE routes (clcPPE_x1 and clcPPE_z1, m_Poisson.f90:567-688,
+    1078-1197) against a direct solve of the assembled operator"""
     O = oracle_mod
     nx, ny, nz = shape
     case = dict(cases.SMALL, nxc=nx, nyc=ny, nzc=nz)
     o = O.Oracle(case)
+    o.set_ppe_route(route)
     Lop = assemble_laplacian(o, O)
     dyp = o.metric(O.M_DYP, 1, ny)
     f = np.random.default_rng(5).standard_normal((nx, ny, nz))

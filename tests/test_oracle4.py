@@ -202,11 +202,16 @@ def test_wavenumber4_is_symbol_of_div_grad(oracle_mod):
 
 
 @pytest.mark.parametrize("shape", [(8, 6, 8), (12, 9, 10)])
-def test_ppe4_vs_sparse_direct(oracle_mod, shape):
+@pytest.mark.parametrize("route", [0, 1], ids=["x1", "z1"])
+def test_ppe4_vs_sparse_direct(oracle_mod, shape, route):
+    """both PPE routes of the reference (x-first clcPPE_x1, z-first
+    clcPPE_z1, m_Poisson.f90:371-489) against a direct solve of the assembled
+    operator with the same gauge"""
     O = oracle_mod
     nx, ny, nz = shape
     case = dict(C4, nxc=nx, nyc=ny, nzc=nz)
     o = O.Oracle4(case)
+    o.set_ppe_route(route)
     hx, hz = case["xlx"] / nx, case["zlz"] / nz
     Gx, Gz = _stag_derivative(nx, hx), _stag_derivative(nz, hz)
     Lx, Lz = (-Gx.T @ Gx), (-Gz.T @ Gz)

@@ -72,6 +72,47 @@ def test_rk3_tables_third_order(oracle_mod):
     o.close()
 
 
+
+@pytest.mark.parametrize("ischeme", [1, 2])
+def test_ab2_rk2_tables_second_order(oracle_mod, ischeme):
+    """the other two schemes of m_Parameters.f90:135-142 through the same
+    recurrence u_k = u_{k-1} + gamma_k z u_{k-1} + zeta_k z u_{k-2}: RK2 (two
+    substeps) reproduces the midpoint method 1 + z + z^2/2 exactly; AB2 (one
+    substep, the history carried from the previous step) is the two-step
+    Adams-Bashforth method, whose principal root is exp(z) to second order"""
+    O = oracle_mod
+    case = dict(_case(), ischeme=ischeme)
+    o = O.Oracle(case)
+    dt = case["dt"]
+    nsub = 2 if ischeme == 2 else 1
+    g, t = [], []
+    for ns in range(1, nsub + 1):
+        o.call("PMcoeUpdate", ns)
+        g.append(o.scalar(O.S_PMGAMMA) / dt)
+        t.append(o.scalar(O.S_PMTHETA) / dt)
+        assert abs(o.scalar(O.S_PMALPHA) / dt - (g[-1] + t[-1])) < 1e-15
+    assert abs(sum(gk + tk for gk, tk in zip(g, t)) - 1.0) < 1e-15
+    assert abs(o.scalar(O.S_SIMTIME) - dt) < 1e-15
+    if ischeme == 2:
+        zs = np.array([-0.7, -0.3, 0.2, 0.5, 0.9])
+        amp = []
+        for z in zs:
+            um, u = 0.0, 1.0
+            for k in range(2):
+                un = u + g[k] * z * u + t[k] * z * um
+                um, u = u, un
+            amp.append(u)
+        coef = np.polyfit(zs, amp, 2)[::-1]
+        assert np.allclose(coef, [1.0, 1.0, 0.5], rtol=0, atol=1e-13)
+    else:
+        # zeta^2 - (1 + gamma z) zeta - theta z = 0, principal root
+        for z in (1e-2, -1e-2, 2e-2):
+            r = np.roots([1.0, -(1.0 + g[0] * z), -t[0] * z])
+            root = r[np.argmin(np.abs(r - 1.0))].real
+            assert abs(root - np.exp(z)) < 0.5 * abs(z) ** 3, z
+            assert abs(root - (1 + z + z * z / 2)) < 0.5 * abs(z) ** 3
+    o.close()
+
 # ----------------------------------------------------- y-operator pins --
 def test_wall_normal_operators_exact(oracle_mod):
     """finite-volume y operators on the stretched grid:
@@ -518,3 +559,41 @@ def test_rhs_wall_normal_diffusion_exact_on_stretched_grid(oracle_mod):
             j = np.arange(2, ny)
             want = -0.8 * (yc[j + 1] - yc[j - 1]) / dyp[j]
             assert np.abs(d22[:, 1:ny - 1, :] - want[None, :, None]).max() < 1e-8, comp
+
+
+def test_bulk_sum_order_gap(oracle_mod):
+    """The library and the oracle sum the bulk velocity (clcRhsX_FIMP's s1tot,
+    CalcUxAver) exactly in fixed point; the reference sums in double, in
+    kc, jc, ic order (m_TSchemeFIMP_inc.f90:46, m_Tools.f90:262).  The oracle's
+    sequential mode reproduces that order: the gap it opens on config 1
+    (64^3, laminar + noise) after five steps stays well inside the 1e-12
+    parity bar (measured: velocity ~4e-15, pressure ~5e-13 after 1-20 steps,
+    the pressure source amplifying the 1-ulp change of dpmdxns)"""
+    O = oracle_mod
+    from channel3d_amd import synth
+    from tests import cases
+    os_ = []
+    for seq in (0, 1):
+        o = O.Oracle(cases.C1)
+        ux, uy, uz = synth.laminar_noise(o.nxc, o.nyc, o.nzc, o.yp, seed=4)
+        o.field(O.UX)[...] = ux
+        o.field(O.UY)[...] = uy
+        o.field(O.UZ)[...] = uz
+        o.project()
+        o.set_sum_order(seq)
+        os_.append(o)
+    ux_av = [o.call("CalcUxAver", o.field(O.UX)) for o in os_]
+    assert abs(ux_av[1] - ux_av[0]) <= 1e-14 * abs(ux_av[0])
+    for _ in range(5):
+        for o in os_:
+            o.iterate()
+
+    def rel(f):
+        a, b = os_[1].field(f)[1:-2, :, 1:-2], os_[0].field(f)[1:-2, :, 1:-2]
+        return float(np.abs(a - b).max() / np.abs(b).max())
+    gaps = {nm: rel(f) for nm, f in (("ux", O.UX), ("uy", O.UY), ("uz", O.UZ), ("p", O.PR))}
+    print("sequential vs fixed-point bulk sum, 5 steps:", gaps)
+    assert max(gaps["ux"], gaps["uy"], gaps["uz"]) < 1e-13
+    assert 0.0 < gaps["p"] < 1e-12
+    for o in os_:
+        o.close()
