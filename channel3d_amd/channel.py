@@ -176,6 +176,13 @@ class Channel:
         L.check(self.lib.c3d_ppe_batch(self._h, len(prsrc), prsrc.base, phi.base, int(sweeps)),
                 "c3d_ppe_batch")
 
+    def ppe_batch_noalloc(self, prsrc, phi, sweeps=-1):
+        """ppe_batch on the library's fixed work space (no device allocation;
+        calls that overlap on one GPU race)"""
+        assert len(prsrc) == len(phi)
+        L.check(self.lib.c3d_ppe_batch_noalloc(self._h, len(prsrc), prsrc.base, phi.base,
+                                               int(sweeps)), "c3d_ppe_batch_noalloc")
+
     def helmholtz_batch(self, comp, ns, rhs, du, sweeps=-1):
         """(1 - b Dxx)(1 - b Dzz)(1 - b Dyy) du = rhs of component comp (0..2)
         at substep ns, for every field of the batch (INTERIOR layouts)"""

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <utility>
 
@@ -462,8 +463,9 @@ static void twiddles(int n, int count, std::vector<T>& out) {
 static int dalloc(c3d_ctx* c, void** p, size_t bytes);
 
 namespace c3d {
-// FP32 tables for the mixed-precision pressure solve of an FP64 context;
-// built on first use
+// FP32 tables for the mixed-precision pressure solve of an FP64 2nd-order
+// context and its (0,0)-mode work; c3d_create builds them, so the solves on a
+// caller's work space allocate nothing
 int ensure_mixed_tables(c3d_ctx* c) {
   if (c->d_tw32) return C3D_OK;
   const Geom& g = c->g;
@@ -489,10 +491,6 @@ int ensure_mixed_tables(c3d_ctx* c) {
   c->tw32_x = d;
   c->tw32_xr = d + a.size();
   c->tw32_z = d + a.size() + b.size();
-  void* r = nullptr;
-  if ((rc = dalloc(c, &r, (size_t)g.nfield * sizeof(float)))) return rc;
-  C3D_HIP(hipMemset(r, 0, (size_t)g.nfield * sizeof(float)));
-  c->resid32 = (float*)r;
   void* w = nullptr;
   // mean-mode solve (3 ny), gauge value, then the gauge partial sums
   const size_t nw = 3 * ((size_t)g.ny + 1) + (size_t)((g.nx + 63) / 64) * g.nz;
@@ -500,6 +498,16 @@ int ensure_mixed_tables(c3d_ctx* c) {
   C3D_HIP(hipMemset(w, 0, nw * sizeof(double)));
   c->d_mean64 = (double*)w;
   c->d_tw32 = tw;
+  return C3D_OK;
+}
+
+int ensure_resid32(c3d_ctx* c) {
+  if (c->resid32) return C3D_OK;
+  void* r = nullptr;
+  int rc = dalloc(c, &r, (size_t)c->g.nfield * sizeof(float));
+  if (rc) return rc;
+  C3D_HIP(hipMemset(r, 0, (size_t)c->g.nfield * sizeof(float)));
+  c->resid32 = (float*)r;
   return C3D_OK;
 }
 }  // namespace c3d
@@ -581,6 +589,41 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return C3D_OK;
+}
+
+// ===========================================================================
+// work space of c3d_ppe_batch_noalloc: a file-scope device array, one copy on
+// every device the library runs on, held until the process ends.  In host
+// code the name `scratch` is only a host-side stand-in: its address on a
+// device comes from hipGetSymbolAddress, taken once per device at
+// c3d_create with that device current, and only that address is handed to
+// kernels.
+// ===========================================================================
+static __device__ __align__(256) unsigned char scratch[C3D_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static int noalloc_scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("c3d_create: device ordinal out of range");
+    return C3D_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev]) return C3D_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_dev[dev] = p;
+  return C3D_OK;
+}
+
+static void* noalloc_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return dev >= 0 && dev < kMaxDevices ? g_scratch_dev[dev] : nullptr;
 }
 
 extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) {
@@ -696,7 +739,10 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
     c3d_destroy(c);
     return r;
   };
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&c->dev);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipGetDevice"));
+  if ((rc = noalloc_scratch_setup(c->dev))) return fail(rc);
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     c->stream = nullptr;
     return fail(hip_fail(e, "hipStreamCreate"));
@@ -739,6 +785,7 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
   }
   rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);
   if (rc) return fail(rc);
+  if (c->prec == C3D_FP64 && c->order == 2 && (rc = ensure_mixed_tables(c))) return fail(rc);
   // state and scratch
   const size_t fbytes = (size_t)g.nfield * c->esize;
   const size_t wbytes = (size_t)g.nx * g.nz * c->esize;
@@ -793,7 +840,7 @@ extern "C" int c3d_set_ppe_refine(c3d_ctx* c, int sweeps) {
   }
   int rc = c3d_sync(c);
   if (rc) return rc;
-  if (sweeps >= 0 && (rc = ensure_mixed_tables(c))) return rc;
+  if (sweeps >= 0 && ((rc = ensure_mixed_tables(c)) || (rc = ensure_resid32(c)))) return rc;
   if (sweeps != c->ppe_sweeps) drop_graphs(c);
   c->ppe_sweeps = sweeps;
   return C3D_OK;
@@ -1238,6 +1285,51 @@ static int batch_args(c3d_ctx* c, const char* what, int nbatch, const void* in, 
 extern "C" int c3d_ppe_batch(c3d_ctx* c, int nbatch, const void* prsrc, void* phi, int sweeps) {
   if (int rc = batch_args(c, "c3d_ppe_batch", nbatch, prsrc, phi, sweeps)) return rc;
   return run_ppe_batch(c, nbatch, prsrc, phi, sweeps);
+}
+
+static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+extern "C" int c3d_ppe_batch_noalloc(c3d_ctx* c, int nbatch, const void* prsrc, void* phi,
+                                     int sweeps) {
+  if (int rc = batch_args(c, "c3d_ppe_batch_noalloc", nbatch, prsrc, phi, sweeps)) return rc;
+  const Geom& g = c->g;
+  const size_t nb = (size_t)nbatch;
+  // the layout of ensure_batch's work space, each part 256-byte aligned
+  const size_t spec_b = align256(nb * g.nspec * 2 * sizeof(double));
+  const size_t ck_b = align256(nb * g.nspec * sizeof(double));
+  const size_t mean_b = align256(nb * batch_mean_len(g) * sizeof(double));
+  const size_t res_b = sweeps >= 0 ? align256(nb * g.nfield * sizeof(float)) : 0;
+  const size_t need = spec_b + ck_b + mean_b + res_b;
+  if (need > (size_t)C3D_NOALLOC_SCRATCH_BYTES) {
+    set_error("c3d_ppe_batch_noalloc: this grid and batch need " + std::to_string(need) +
+              " bytes of work space, above the fixed " +
+              std::to_string((size_t)C3D_NOALLOC_SCRATCH_BYTES) +
+              " (use c3d_ppe_batch, or a smaller batch)");
+    return C3D_ERR_ARG;
+  }
+  if (sweeps >= 0 && !c->d_tw32) {
+    set_error("c3d_ppe_batch_noalloc: the context has no FP32 tables");
+    return C3D_ERR_ARG;
+  }
+  unsigned char* base = (unsigned char*)noalloc_scratch(c->dev);
+  if (!base) {
+    set_error("c3d_ppe_batch_noalloc: no work space on this device");
+    return C3D_ERR_HIP;
+  }
+  // Nothing here needs zeros on entry: the (0,0)-mode work and the gauge
+  // partials are written before they are read, and the transforms read only
+  // the residual's rows 1..ny, which the residual pass writes.  Profiling is
+  // suspended for the call (its events are created on demand).
+  BatchWork ws;
+  ws.spec = base;
+  ws.ck = base + spec_b;
+  ws.mean = (double*)(base + spec_b + ck_b);
+  ws.resid = sweeps >= 0 ? (float*)(base + spec_b + ck_b + mean_b) : nullptr;
+  const bool prof = c->prof;
+  c->prof = false;
+  const int rc = ppe_batch_ws(c, nbatch, prsrc, phi, sweeps, ws);
+  c->prof = prof;
+  return rc;
 }
 
 extern "C" int c3d_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nbatch, const void* rhs,

@@ -244,8 +244,12 @@ struct c3d_ctx {
   void* staging = nullptr;    // pinned host staging buffer for upload/download
   size_t staging_bytes = 0;
 
-  // mixed-precision pressure solve (FP64 context): FP32 twiddles and metrics,
-  // the FP32 residual field; ppe_sweeps < 0 keeps the plain FP64 solve
+  // device the context was created on (the current device at c3d_create)
+  int dev = 0;
+
+  // mixed-precision pressure solve (FP64 context): FP32 twiddles and metrics
+  // (built at c3d_create for FP64 2nd-order contexts), the FP32 residual
+  // field (on first use); ppe_sweeps < 0 keeps the plain FP64 solve
   int ppe_sweeps = -1;
   void* d_tw32 = nullptr;
   const void* tw32_x = nullptr;
@@ -423,6 +427,8 @@ int launch_fft32(c3d_ctx* c, int what, const void* in, void* spec, double* out, 
                  const Batch& b = Batch());
 // FP32 twiddle and metric tables of an FP64 context (c3d_api.hip)
 int ensure_mixed_tables(c3d_ctx* c);
+// the single mixed-precision solve's FP32 residual field
+int ensure_resid32(c3d_ctx* c);
 // (0,0)-mode work of one field of a mixed-precision solve: 3 (ny+1) doubles,
 // then the gauge partial sums
 inline long long batch_mean_len(const Geom& g) {

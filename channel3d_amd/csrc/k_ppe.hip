@@ -466,6 +466,7 @@ static int ppe_mixed(c3d_ctx* c, const void* prsrc, void* phi, int sweeps, void*
 
 static int run_ppe_mixed(c3d_ctx* c, void* prsrc, void* phi, int sweeps) {
   int rc = ensure_mixed_tables(c);
+  if (!rc) rc = ensure_resid32(c);
   if (rc) return rc;
   // the FP64 spectral array holds the FP32 spectrum
   return ppe_mixed(c, prsrc, phi, sweeps, c->spec, c->sarr, c->d_mean64, c->resid32, Batch());

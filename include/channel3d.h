@@ -254,6 +254,21 @@ int c3d_ppe_batch(c3d_ctx* ctx, int nbatch, const void* prsrc, void* phi, int sw
 int c3d_helmholtz_batch(c3d_ctx* ctx, int comp, int ns, int nbatch, const void* rhs, void* du,
                         int sweeps);
 
+/* c3d_ppe_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  Same
+ * arguments, kernels and results as c3d_ppe_batch; the work space is a fixed
+ * device array of C3D_NOALLOC_SCRATCH_BYTES inside the library (one per GPU,
+ * held until the process ends) instead of the context's batch buffers.  A
+ * grid and batch whose work space (about nbatch * (24 * nspec + 4 * nfield)
+ * bytes, nspec = (nxc/2+1 rounded up to 8) * nyc * nzc, nfield = nxc *
+ * (nyc+3) * nzc, the 4 * nfield for sweeps >= 0 only) exceeds it is rejected
+ * with C3D_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two contexts) race and must be ordered by
+ * the caller. */
+#define C3D_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int c3d_ppe_batch_noalloc(c3d_ctx* ctx, int nbatch, const void* prsrc, void* phi, int sweeps);
+
 /* ---------------------------------------------------------- profiling -- */
 /* per-stage HIP-event timing on the context stream */
 int c3d_profile_enable(c3d_ctx* ctx, int on);
