@@ -106,6 +106,11 @@ class Closure:
         L.check(self.lib.c3d_closure_adam_step(self._h, _fp(v), float(lr)),
                 "c3d_closure_adam_step")
 
+    def set_mode(self, mode):
+        """0: network viscosity with the dynamic Smagorinsky fallback;
+        1: dynamic Smagorinsky in every cell"""
+        L.check(self.lib.c3d_closure_set_mode(self._h, int(mode)), "c3d_closure_set_mode")
+
     def fallbacks(self):
         """cells of the last LES viscosity evaluation that fell back to
         dynamic Smagorinsky"""

@@ -641,6 +641,9 @@ __global__ void k_cl_zero_int(int* p) { p[threadIdx.x] = 0; }
 // or not finite, the dynamic Smagorinsky value max(Cs^2, 0) Delta^2 |S|.
 // nut = xnu + nu, with the wall ghosts of SetBC_and_UpdateHalo_SGS
 // (m_LESModels.f90:375-428).  LES cell c = (i, j-1, k) of the context.
+// DSM: every cell takes the dynamic Smagorinsky value (the network is not
+// evaluated; Y and scale are not read).
+template <bool DSM>
 __global__ void k_cl_nut(Geom g, int nl, const float* __restrict__ Y,
                          const float* __restrict__ scale, const double* __restrict__ w,
                          const double* __restrict__ cs2, const double* __restrict__ dY,
@@ -649,19 +652,22 @@ __global__ void k_cl_nut(Geom g, int nl, const float* __restrict__ Y,
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nl) return;
   const int i = (int)(c % g.nx), J = (int)((c / g.nx) % g.ny), k = (int)(c / ((long long)g.nx * g.ny));
-  double t[6];
-  for (int q = 0; q < 6; ++q) t[q] = (double)Y[c * NOUT + q] * (double)scale[c];
-  const double tr = (t[0] + t[1] + t[2]) / 3.0;
-  for (int q = 0; q < 3; ++q) t[q] -= tr;
-  double ts = 0.0, ss = 0.0;
-  for (int q = 0; q < 6; ++q) {
-    const double wt = q < 3 ? 1.0 : 2.0;
-    const double S = w[q * (long long)nl + c];
-    ts += wt * t[q] * S;
-    ss += wt * S * S;
+  double nu = -1.0;
+  if (!DSM) {
+    double t[6];
+    for (int q = 0; q < 6; ++q) t[q] = (double)Y[c * NOUT + q] * (double)scale[c];
+    const double tr = (t[0] + t[1] + t[2]) / 3.0;
+    for (int q = 0; q < 3; ++q) t[q] -= tr;
+    double ts = 0.0, ss = 0.0;
+    for (int q = 0; q < 6; ++q) {
+      const double wt = q < 3 ? 1.0 : 2.0;
+      const double S = w[q * (long long)nl + c];
+      ts += wt * t[q] * S;
+      ss += wt * S * S;
+    }
+    nu = -ts / (2.0 * ss);
   }
-  double nu = -ts / (2.0 * ss);
-  const bool back = !(nu >= 0.0) || !isfinite(nu);
+  const bool back = DSM || !(nu >= 0.0) || !isfinite(nu);
   if (back) {
     const double sn = w[6 * (long long)nl + c];
     nu = fmax(cs2[J], 0.0) * pow(dX * dY[J] * dZ, 2.0 / 3.0) * sn;
@@ -687,6 +693,7 @@ struct c3d_closure {
   // that evaluated it last (its fallback count is c3d_closure_fallbacks's)
   std::vector<c3d_ctx*> users;
   c3d_ctx* last = nullptr;
+  int mode = 0;                    // 0 network (DSM fallback), 1 DSM everywhere
   long long cap = 0, n = 0;        // dataset capacity and size (samples)
   float* X = nullptr;              // cap x 16
   float* T = nullptr;              // cap x 6
@@ -1199,15 +1206,25 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
                      (const double*)pr, (const double*)dyc, (const double*)ddY, dX, dZ, lm, mm);
   hipLaunchKernelGGL(k_dsm_plane, dim3(nJ), dim3(256), 0, st, nI, nJ, nK, (const double*)lm,
                      (const double*)mm, dcs);
-  hipLaunchKernelGGL(k_cl_samples, dim3(grid), dim3(blk), 0, st, nI, nJ, nK, (const double*)fb,
-                     (const double*)dyc, (const double*)ddY, dX, dZ, c->p.yly, X, T, S);
-  const long long ntile = (nl + 15) / 16;
-  hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(4096, ntile)), dim3(64), 0,
-                     st, (const float*)X, nl, cl->fwd, cl->w, Y);
+  if (cl->mode == 0) {
+    hipLaunchKernelGGL(k_cl_samples, dim3(grid), dim3(blk), 0, st, nI, nJ, nK,
+                       (const double*)fb, (const double*)dyc, (const double*)ddY, dX, dZ,
+                       c->p.yly, X, T, S);
+    const long long ntile = (nl + 15) / 16;
+    hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(4096, ntile)), dim3(64),
+                       0, st, (const float*)X, nl, cl->fwd, cl->w, Y);
+  }
   hipLaunchKernelGGL(k_cl_zero_int, dim3(1), dim3(1), 0, st, c->cln_fb);
-  hipLaunchKernelGGL(k_cl_nut, dim3(grid), dim3(blk), 0, st, g, (int)nl, (const float*)Y,
-                     (const float*)S, (const double*)w, (const double*)dcs, (const double*)ddY,
-                     dX, dZ, c->p.xnu, c->p.bc[2], c->p.bc[3], (double*)nut, c->cln_fb);
+  if (cl->mode == 0)
+    hipLaunchKernelGGL(k_cl_nut<false>, dim3(grid), dim3(blk), 0, st, g, (int)nl,
+                       (const float*)Y, (const float*)S, (const double*)w, (const double*)dcs,
+                       (const double*)ddY, dX, dZ, c->p.xnu, c->p.bc[2], c->p.bc[3],
+                       (double*)nut, c->cln_fb);
+  else
+    hipLaunchKernelGGL(k_cl_nut<true>, dim3(grid), dim3(blk), 0, st, g, (int)nl,
+                       (const float*)nullptr, (const float*)nullptr, (const double*)w,
+                       (const double*)dcs, (const double*)ddY, dX, dZ, c->p.xnu, c->p.bc[2],
+                       c->p.bc[3], (double*)nut, c->cln_fb);
   C3D_HIP(hipGetLastError());
   cl->last = c;
   return C3D_OK;
@@ -1241,6 +1258,22 @@ extern "C" int c3d_set_les_closure(c3d_ctx* c, c3d_closure* cl) {
     if (cl) cl->users.push_back(c);
   }
   c->les_closure = cl;
+  return C3D_OK;
+}
+
+// 0: the network's viscosity with the dynamic Smagorinsky fallback; 1: the
+// dynamic Smagorinsky viscosity in every cell (the fallback model run on its
+// own).  Graphs of attached contexts baked the old kernels and are dropped.
+extern "C" int c3d_closure_set_mode(c3d_closure* cl, int mode) {
+  if (!cl || mode < 0 || mode > 1) {
+    set_error("c3d_closure_set_mode: mode must be 0 (network) or 1 (dynamic Smagorinsky)");
+    return C3D_ERR_ARG;
+  }
+  for (c3d_ctx* c : cl->users) {
+    C3D_HIP(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  cl->mode = mode;
   return C3D_OK;
 }
 

@@ -53,7 +53,7 @@ EXPORTS = (
     "c3d_closure_get_samples", "c3d_closure_predict", "c3d_closure_grad", "c3d_closure_train",
     "c3d_closure_adam_step", "c3d_closure_dsm", "c3d_set_les_closure", "c3d_closure_fallbacks",
     "c3d_field_bytes", "c3d_field_alloc_batch", "c3d_ppe_batch", "c3d_helmholtz_batch",
-    "c3d_ppe_batch_noalloc",
+    "c3d_ppe_batch_noalloc", "c3d_closure_set_mode",
 )
 
 
@@ -178,6 +178,7 @@ def load(path: str | None = None):
         "c3d_field_alloc_batch": ([_VP, I, I, ctypes.POINTER(_VP)], I),
         "c3d_ppe_batch": ([_VP, I, _VP, _VP, I], I),
         "c3d_ppe_batch_noalloc": ([_VP, I, _VP, _VP, I], I),
+        "c3d_closure_set_mode": ([_VP, I], I),
         "c3d_helmholtz_batch": ([_VP, I, I, I, _VP, _VP, I], I),
         "c3d_Clc_SGS_Vis": ([_VP] * 5, I),
         "c3d_clcSpectra1D": ([_VP] * 5 + [I, I], I),

@@ -149,3 +149,52 @@ def spectrum_field(nxc, nyc, nzc, seed=0, k0=8.0, kcut=40, dyp=None):
         m = (f.mean(axis=(0, 2)) * w).sum() / w.sum()
         f = np.asfortranarray(f - m)
     return f
+
+
+def channel_init(nxc, nyc, nzc, yp, xlx, zlz, xnu, ubulk=1.0, yly=2.0, seed=0, order=2,
+                 u_crf=0.0):
+    """The reference's turbulent-channel start: a laminar profile
+    3 ub (ybar - ybar^2/2) plus streamwise streaks and spanwise rolls of
+    amplitude ub ybar exp(-4.5 ybar^2), modulated by cos(wz z+) and
+    sin(wx x+) with wavelengths near 200 and 500 wall units and a random
+    factor in [0.8, 1.2]; the streamwise velocity rescaled to the bulk velocity
+    and the plane mean of the spanwise one removed (2nd order
+    m_FlowType_Channel.f90:37-115 for ybulk = [0, yly]; 4th order
+    m_FlowCase.f90:99-157, which rescales by CalcUxAver and subtracts uCRF).
+    The random factor comes from this module's seeded generator instead of
+    the clock-seeded Fortran random_number.  Returns interior arrays
+    (nxc, nyc, nzc) of ux, uy (zero), uz; 2nd order: v is zero on the walls
+    already."""
+    height = 0.5 * yly
+    rem = ubulk * height / xnu
+    retau = 0.1538 * rem ** 0.887741
+    utau = retau * xnu / height
+    dev = 0.2 * uniform01(seed, 41, nxc * nyc * nzc).reshape((nxc, nyc, nzc), order="F") + 0.9
+    wx, wz = 2.0 * np.pi / 500.0, 2.0 * np.pi / 200.0
+    xlxp, zlzp = xlx * utau / xnu, zlz * utau / xnu
+    m1 = np.floor(xlxp * wx / (2.0 * np.pi)) + 1
+    wx = m1 * 2.0 * np.pi / xlxp
+    m2 = np.floor(zlzp * wz / (2.0 * np.pi)) + 1
+    wz = m2 * 2.0 * np.pi / zlzp
+    yc = yc_from_yp(np.asarray(yp, dtype=np.float64))
+    yct = height - np.abs(height - yc)
+    ybar = yct / height
+    dx, dz = xlx / nxc, zlz / nzc
+    xplus = utau * ((np.arange(nxc) + 0.5) * dx) / xnu
+    zplus = utau * ((np.arange(nzc) + 0.5) * dz) / xnu
+    env = (ubulk * ybar * np.exp(-4.5 * ybar * ybar))[None, :, None]
+    ux = env * np.cos(wz * zplus)[None, None, :] * dev
+    uz = env * np.sin(wx * xplus)[:, None, None] * dev
+    ux = ux + (3.0 * ubulk * (ybar - 0.5 * ybar * ybar))[None, :, None]
+    if order == 4:
+        # CalcUxAver of the 4th-order code: the dyp-weighted volume mean
+        dyp = np.diff(np.asarray(yp[1:nyc + 2], dtype=np.float64))
+        uxav = (ux.mean(axis=(0, 2)) * dyp).sum() / yly
+        ux = ux * (ubulk / uxav) - u_crf
+    else:
+        dyp = np.diff(np.asarray(yp[1:nyc + 2], dtype=np.float64))
+        ratio = (ux.mean(axis=(0, 2)) * dyp).sum() / yly / ubulk
+        ux = ux * ratio
+    uz = uz - uz.mean(axis=(0, 2))[None, :, None]
+    uy = np.zeros((nxc, nyc, nzc))
+    return np.asfortranarray(ux), np.asfortranarray(uy), np.asfortranarray(uz)

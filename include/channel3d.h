@@ -452,6 +452,11 @@ int c3d_closure_dsm(c3d_closure* cl, c3d_ctx* dns, int fx, int fy, int fz, doubl
  * fell back in the last evaluation.  The closure must outlive its use. */
 int c3d_set_les_closure(c3d_ctx* ctx, c3d_closure* cl);
 int c3d_closure_fallbacks(c3d_closure* cl, long long* n);
+/* mode 0 (default): the network's viscosity with the dynamic Smagorinsky
+ * fallback, as above; mode 1: the dynamic Smagorinsky viscosity in every
+ * cell (an LES with the fallback model alone, the a-posteriori comparator;
+ * every cell counts as a fallback).  No counterpart in the reference. */
+int c3d_closure_set_mode(c3d_closure* cl, int mode);
 
 #ifdef __cplusplus
 }

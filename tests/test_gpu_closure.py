@@ -263,9 +263,10 @@ def _yp():
     return channel_yp(DNS["nyc"], DNS["yly"], DNS["gamma"])
 
 
-def _nut_oracle(g, case, p):
+def _nut_oracle(g, case, p, dsm_only=False):
     """the oracle chain of the closure's LES viscosity on the context's own
-    grid: (nut in the interior cells, fallback mask)"""
+    grid: (nut in the interior cells, fallback mask); dsm_only: the dynamic
+    Smagorinsky value in every cell"""
     from oracle import closure as C
     from channel3d_amd import lib as L
     ux, uy, uz = (g.download(f) for f in (L.UX, L.UY, L.UZ))
@@ -288,6 +289,8 @@ def _nut_oracle(g, case, p):
     ss = sum(w * b * b for w, b in zip(wt, Sij))
     nu = -ts / (2 * ss)
     fb = ~(nu >= 0) | ~np.isfinite(nu)
+    if dsm_only:
+        fb = np.ones_like(fb)
     d2 = ((dx * dY * dz) ** (2 / 3))[None, :, None]
     nu = np.where(fb, np.maximum(cs2, 0)[None, :, None] * d2 * sn, nu)
     return case["xnu"] + nu, fb
@@ -336,6 +339,40 @@ def test_les_viscosity_from_closure():
         g.step()
     d = g.diagnostics()
     assert np.isfinite(d["vmax_x"]) and d["divmax2"] < 1e-10
+    g.set_les_closure(None)
+    g.close()
+    cl.close()
+
+
+def test_les_viscosity_dynamic_smagorinsky_mode():
+    """mode 1 (dynamic Smagorinsky in every cell): nut against the oracle's
+    FP64 dynamic Smagorinsky chain to 1e-9 (no bf16 in this path), every cell
+    counted as a fallback; mode 0 restores the network's viscosity"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from channel3d_amd import lib as L
+    cl = Closure(1 << 12)
+    p = C.init_params(4)
+    cl.set_params(p)
+    g = _smooth_dns(LESC, seed=13)
+    g.set_les_closure(cl)
+    cl.set_mode(1)
+    g.Clc_SGS_Vis()
+    g.sync()
+    nut = g.download(L.NUT)[1:-2, 1:-2, 1:-2]
+    ref, _ = _nut_oracle(g, LESC, p, dsm_only=True)
+    err = np.abs(nut - ref).max() / np.abs(ref).max()
+    print(f"dynamic Smagorinsky mode: nut max rel err {err:.2e}", flush=True)
+    assert err < 1e-9
+    assert cl.fallbacks() == LESC["nxc"] * LESC["nyc"] * LESC["nzc"]
+    for _ in range(3):
+        g.step()
+    d = g.diagnostics()
+    assert np.isfinite(d["vmax_x"]) and d["divmax2"] < 1e-10
+    cl.set_mode(0)
+    g.Clc_SGS_Vis()
+    g.sync()
+    _check_nut(g, LESC, p, g.download(L.NUT), cl.fallbacks())
     g.set_les_closure(None)
     g.close()
     cl.close()
