@@ -273,23 +273,32 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 
 // ---------------------------------------------------------------------------
 // The same Thomas sweeps without writing the forward pass to memory.  The
-// line is cut into segments of C rows.  The forward pass loads a segment's
-// right-hand side (C loads in flight), runs the recurrence and keeps only
-// the value at the segment's end, in LDS.  The backward pass walks the
-// segments from the top: it recomputes the segment's forward values from
-// its checkpoint (the same operations on the same operands, so the same
-// values), then substitutes backward and updates u.  The top segment's
-// forward values are still in registers and are not recomputed.  Traffic:
-// the right-hand side is read twice, u once, u written once (4 words per
-// point instead of 5); the results equal k_adi_y's bit for bit.
+// line is cut into segments of C rows.  The forward pass runs the recurrence
+// segment by segment and keeps only the value at each segment's end, in LDS.
+// The backward pass walks the segments from the top: it recomputes the
+// segment's forward values from its checkpoint (the same operations on the
+// same operands, so the same values), then substitutes backward and updates
+// u.  The top segment's forward values are still in registers and are not
+// recomputed.  Traffic: the right-hand side is read twice, u once, u written
+// once (4 words per point instead of 5); the results equal k_adi_y's bit
+// for bit.
+//
+// Loads are issued one segment ahead: the forward pass loads segment s+1
+// while it runs the recurrence of segment s, the backward pass loads the
+// right-hand side and u of the segment below while it substitutes through
+// the current one, so a wavefront always has C rows of loads in flight
+// behind its serial recurrence.  A workgroup is 256 consecutive lines of the
+// flattened (i, k) index (i fastest): each row access of a wavefront is one
+// run of 64 consecutive words.
 // ---------------------------------------------------------------------------
 // WALL: 0 no wall row, 1 row 1 is the wall condition x = duy, 2 the wall
 // condition x = 0 (a solve for the increment alone).  SET: u = x instead of
 // u += x, with u of type TU (an FP32 solve writing an FP64 increment).
-// Batches: blockIdx.z selects the field, bf reals apart in r and u.
+// Batches: blockIdx.y selects the field, bf reals apart in r and u.
+constexpr int YNT = 256;
 template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
           typename TU = T>
-__global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
+__global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
                                                   const double* __restrict__ tvecm,
                                                   const double* __restrict__ tarr,
                                                   const T* __restrict__ r, TU* __restrict__ u,
@@ -297,33 +306,42 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
                                                   const DevScalars* __restrict__ sc,
                                                   long long bf) {
   extern __shared__ unsigned char ck_raw[];
-  T* ck = (T*)ck_raw;  // [segment][256 threads]
-  const int tid = threadIdx.x + 64 * threadIdx.y;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int k = blockIdx.y * 4 + threadIdx.y;
-  if (i >= g.nx || k >= g.nz) return;
-  r += bf * blockIdx.z;
-  u += bf * blockIdx.z;
+  T* ck = (T*)ck_raw;  // [segment][YNT threads]
+  const int tid = threadIdx.x;
+  const long long line = (long long)blockIdx.x * YNT + tid;
+  if (line >= (long long)g.nx * g.nz) return;
+  const int i = (int)(line % g.nx), k = (int)(line / g.nx);
+  r += bf * blockIdx.y;
+  u += bf * blockIdx.y;
   constexpr bool WALLROW = WALL != 0;
-  auto wall = [&]() -> T { return WALL == 1 ? duy[i + (long long)g.nx * k] : T(0); };
   const int ny = g.ny;
   const int nseg = (ny + C - 1) / C;
   const long long base = (long long)i + g.plane * k;
   const long long sj = g.nx;
   const T shift = SHIFT ? (T)sc->dp1ns : T(0);
-  auto ld = [&](int j) { return SHIFT ? r[base + sj * j] - shift : r[base + sj * j]; };
-  T y[C];
-  T f = T(0);
-  // forward: segment s holds rows s*C+1 .. min(s*C+C, ny)
-  for (int s = 0; s < nseg; ++s) {
+  const T wallv = WALL == 1 ? duy[i + (long long)g.nx * k] : T(0);
+  // rows s*C+1 .. s*C+C of the right-hand side (0 past the last row)
+  auto load = [&](T* y, int s) {
     const int lo = s * C + 1;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      y[q] = j <= ny ? ld(j) : T(0);
+      y[q] = j <= ny ? (SHIFT ? r[base + sj * j] - shift : r[base + sj * j]) : T(0);
     }
-    if (s == 0 && WALLROW) y[0] = wall();
+  };
+  auto loadu = [&](TU* uv, int s) {
+    const int lo = s * C + 1;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j = lo + q;
+      uv[q] = (!SET && j <= ny) ? C3D_LD_ONCE(&u[base + sj * j]) : TU(0);  // once
+    }
+  };
+  // forward recurrence of segment s from f (the value at the row below it)
+  auto fwd = [&](T* y, int s, T& f) {
+    const int lo = s * C + 1;
     if (s == 0) {
+      if (WALLROW) y[0] = wallv;
       f = y[0] / (T)tvecm[1];
       y[0] = f;
     }
@@ -335,40 +353,43 @@ __global__ __launch_bounds__(256) void k_adi_y_ck(Geom g, const double* __restri
         y[q] = f;
       }
     }
-    if (s + 1 < nseg) ck[s * 256 + tid] = f;  // forward value at row (s+1)*C
+  };
+  T ya[C], yb[C];
+  TU ua[C];
+  T f = T(0);
+  load(ya, 0);
+  for (int s = 0; s < nseg; s += 2) {
+    if (s + 1 < nseg) load(yb, s + 1);
+    fwd(ya, s, f);
+    if (s + 1 < nseg) {
+      ck[s * YNT + tid] = f;  // forward value at row (s+1)*C
+      if (s + 2 < nseg) load(ya, s + 2);
+      fwd(yb, s + 1, f);
+      if (s + 2 < nseg) ck[(s + 1) * YNT + tid] = f;
+    }
   }
-  // backward; y holds the top segment's forward values
+  // the top segment's forward values into ya
+  if ((nseg - 1) & 1) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) ya[q] = yb[q];
+  }
   T x = f;
+  loadu(ua, nseg - 1);
   for (int s = nseg - 1; s >= 0; --s) {
     const int lo = s * C + 1;
-    if (s < nseg - 1) {
-      T fp = s > 0 ? ck[(s - 1) * 256 + tid] : T(0);
-#pragma unroll
-      for (int q = 0; q < C; ++q) y[q] = ld(lo + q);  // a full segment: lo + C - 1 < ny
-      if (s == 0) {
-        if (WALLROW) y[0] = wall();
-        fp = y[0] / (T)tvecm[1];
-        y[0] = fp;
-      }
-#pragma unroll
-      for (int q = 0; q < C; ++q) {
-        if (s > 0 || q > 0) {
-          fp = (y[q] - (T)ta[lo + q] * fp) / (T)tvecm[lo + q];
-          y[q] = fp;
-        }
-      }
-    }
-    TU uv[C];
-#pragma unroll
-    for (int q = 0; q < C; ++q) {
-      const int j = lo + q;
-      uv[q] = (!SET && j <= ny) ? C3D_LD_ONCE(&u[base + sj * j]) : TU(0);  // once
-    }
+    if (s > 0) load(yb, s - 1);  // the segment below: a full one
 #pragma unroll
     for (int q = C - 1; q >= 0; --q) {
       const int j = lo + q;
-      if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
-      if (j <= ny) C3D_ST_ONCE(SET ? (TU)x : uv[q] + (TU)x, &u[base + sj * j]);
+      if (j < ny) x = ya[q] - (T)tarr[j + 1] * x;
+      if (j <= ny) C3D_ST_ONCE(SET ? (TU)x : ua[q] + (TU)x, &u[base + sj * j]);
+    }
+    if (s > 0) {
+      loadu(ua, s - 1);
+      T fp = s > 1 ? ck[(s - 2) * YNT + tid] : T(0);
+      fwd(yb, s - 1, fp);
+#pragma unroll
+      for (int q = 0; q < C; ++q) ya[q] = yb[q];
     }
   }
 }
@@ -381,9 +402,10 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
   dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4), by(64, 4);
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   const DevScalars* sc = (const DevScalars*)c->d_sc;
-  constexpr int CSEG = 32;
-  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * 256 * sizeof(T);
+  constexpr int CSEG = 16;
+  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * YNT * sizeof(T);
   if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
+    const dim3 gy((unsigned)(((long long)g.nx * g.nz + YNT - 1) / YNT)), by(YNT);
     auto launch = [&](auto ntc) {
       constexpr bool NT_ = decltype(ntc)::value;
       if (comp == 1)
@@ -534,8 +556,12 @@ int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy
 // r = rhs - (1 - b Dxx)(1 - b Dzz) M du, in FP64, stored as TR.  M is the y
 // matrix the sweeps invert: rows 1..ny, diagonal bm(j), lower am(j) (j > 1),
 // upper cm(j) (j < ny).  WALL: row 1 is the wall condition du = 0, its
-// residual is 0.  One thread per cell; the 27 neighbours of a cell come from
-// the caches.  blockIdx.z = q nz + k for field q (bf reals apart).
+// residual is 0.  A thread owns an (i, j) column and marches a chunk of
+// kchunk z planes: each plane's x-y part zs(k) = sum_x wx (M du)(x, k) is
+// formed once and rolls through registers, so every du value is loaded for
+// three planes' outputs once (9 loads per cell instead of 27), and the sum
+// over z keeps the order wzo zs(k-1) + wzc zs(k) + wzo zs(k+1).
+// blockIdx.z = q nch + chunk for field q (bf reals apart).
 template <typename TR, bool WALL>
 __global__ __launch_bounds__(256) void k_helm_resid(Geom g, const double* __restrict__ am,
                                                     const double* __restrict__ bm,
@@ -543,40 +569,50 @@ __global__ __launch_bounds__(256) void k_helm_resid(Geom g, const double* __rest
                                                     double wxo, double wzc, double wzo,
                                                     const double* __restrict__ rhs,
                                                     const double* __restrict__ du,
-                                                    TR* __restrict__ r, long long bf) {
+                                                    TR* __restrict__ r, long long bf,
+                                                    int kchunk) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int jc = blockIdx.y * 4 + threadIdx.y + 1;
-  const int q = blockIdx.z / g.nz;
-  const int k = blockIdx.z - q * g.nz;
+  const int nch = (g.nz + kchunk - 1) / kchunk;
+  const int q = blockIdx.z / nch;
+  const int k0 = (blockIdx.z - q * nch) * kchunk;
   if (i >= g.nx || jc > g.ny) return;
   rhs += bf * q;
   du += bf * q;
   r += bf * q;
-  const long long c0 = fidx(g, i, jc, k);
+  const int k1 = min(g.nz, k0 + kchunk);
   if (WALL && jc == 1) {
-    r[c0] = TR(0);
+    for (int k = k0; k < k1; ++k) r[fidx(g, i, jc, k)] = TR(0);
     return;
   }
   const int im = i == 0 ? g.nx - 1 : i - 1, ip = i == g.nx - 1 ? 0 : i + 1;
-  const int km = k == 0 ? g.nz - 1 : k - 1, kp = k == g.nz - 1 ? 0 : k + 1;
   const double a = jc > 1 ? am[jc] : 0.0, b = bm[jc], cc = jc < g.ny ? cm[jc] : 0.0;
-  const int ii[3] = {im, i, ip};
-  const int kk[3] = {km, k, kp};
-  double acc = 0.0;
-#pragma unroll
-  for (int dk = 0; dk < 3; ++dk) {
+  auto plane = [&](int k) {
+    const int ii[3] = {im, i, ip};
     double zs = 0.0;
 #pragma unroll
     for (int di = 0; di < 3; ++di) {
-      const long long p = fidx(g, ii[di], jc, kk[dk]);
+      const long long p = fidx(g, ii[di], jc, k);
       double my = b * du[p];
       if (jc > 1) my += a * du[p - g.nx];
       if (jc < g.ny) my += cc * du[p + g.nx];
       zs += (di == 1 ? wxc : wxo) * my;
     }
-    acc += (dk == 1 ? wzc : wzo) * zs;
+    return zs;
+  };
+  double zm = plane(k0 == 0 ? g.nz - 1 : k0 - 1);
+  double zc = plane(k0);
+  for (int k = k0; k < k1; ++k) {
+    const double zp = plane(k == g.nz - 1 ? 0 : k + 1);
+    double acc = 0.0;
+    acc += wzo * zm;
+    acc += wzc * zc;
+    acc += wzo * zp;
+    const long long c0 = fidx(g, i, jc, k);
+    r[c0] = (TR)(rhs[c0] - acc);
+    zm = zc;
+    zc = zp;
   }
-  r[c0] = (TR)(rhs[c0] - acc);
 }
 
 template <typename T, typename TIN>
@@ -619,13 +655,13 @@ template <typename T, bool SET>
 static int y_batch(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, const void* rhs, void* du,
                    int nb, long long bf) {
   const Geom& g = c->g;
-  constexpr int CSEG = 32;
-  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * 256 * sizeof(T);
+  constexpr int CSEG = 16;
+  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * YNT * sizeof(T);
   if (ck_bytes > 64 * 1024) {
     set_error("helmholtz_batch: too many rows for the checkpointed sweep");
     return C3D_ERR_UNSUPPORTED;
   }
-  const dim3 gy((g.nx + 63) / 64, (g.nz + 3) / 4, nb), by(64, 4);
+  const dim3 gy((unsigned)(((long long)g.nx * g.nz + YNT - 1) / YNT), nb), by(YNT);
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   const DevScalars* sc = (const DevScalars*)c->d_sc;
   auto launch = [&](auto ntc, auto wallc) {
@@ -683,15 +719,17 @@ int run_helmholtz_batch(c3d_ctx* c, int comp, int ns, int nb, const void* rhs, v
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   for (int s = 0; s < sweeps; ++s) {
     prof_begin(c, ST_RESID, &ev);
-    const dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz * nb), blk(64, 4);
+    constexpr int KCH = 8;
+    const int nch = (g.nz + KCH - 1) / KCH;
+    const dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, nch * nb), blk(64, 4);
     if (comp == 1)
       hipLaunchKernelGGL((k_helm_resid<float, true>), grid, blk, 0, c->stream, g, yt[0], yt[3],
                          yt[4], wxc, wxo, wzc, wzo, (const double*)rhs, (const double*)du,
-                         (float*)c->bres, bf);
+                         (float*)c->bres, bf, KCH);
     else
       hipLaunchKernelGGL((k_helm_resid<float, false>), grid, blk, 0, c->stream, g, yt[0], yt[3],
                          yt[4], wxc, wxo, wzc, wzo, (const double*)rhs, (const double*)du,
-                         (float*)c->bres, bf);
+                         (float*)c->bres, bf, KCH);
     C3D_HIP(hipGetLastError());
     prof_end(c, ST_RESID, ev, N * (8 + 8 + 4));
     prof_begin(c, ST_ADI_XZ, &ev);
