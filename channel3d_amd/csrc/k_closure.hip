@@ -22,7 +22,9 @@
 #include "c3d_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <random>
+#include <type_traits>
 
 namespace c3d {
 
@@ -172,9 +174,9 @@ __global__ void k_cl_samples(int nI, int nJ, int nK, const double* __restrict__ 
 // bf16 A fragments from the FP32 masters: lane l of a forward fragment
 // (layer, o, kt) holds W[16kt + 4(l/16) + j][16o + l%16]; of a backward
 // fragment (layer, it, ot) W[16it + l%16][16ot + 4(l/16) + j]
-__global__ void k_cl_pack(const float* __restrict__ w, s4v* __restrict__ fwd,
-                          s4v* __restrict__ bwd) {
-  const int f = blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ void pack_fragment(const float* __restrict__ w,
+                                              s4v* __restrict__ fwd, s4v* __restrict__ bwd,
+                                              int f, int lane) {
   const int q = lane >> 4, r = lane & 15;
   if (f < NFWD) {
     int l = 3;
@@ -202,23 +204,27 @@ __global__ void k_cl_pack(const float* __restrict__ w, s4v* __restrict__ fwd,
   }
 }
 
+__global__ void k_cl_pack(const float* __restrict__ w, s4v* __restrict__ fwd,
+                          s4v* __restrict__ bwd) {
+  pack_fragment(w, fwd, bwd, blockIdx.x, threadIdx.x);
+}
+
 struct Fwd {
   s4v h0, h1[4], h2[4], h3[4];
   f4v z1[4], z2[4], z3[4], y;
 };
 
-// forward of the 16 samples s0 + lane%16 (rows past n are zero inputs)
-__device__ __forceinline__ void mlp_forward(const float* __restrict__ X, long long s, bool ok,
-                                            const s4v* __restrict__ fwd,
-                                            const float* __restrict__ w, Fwd& F) {
+// forward of 16 samples, lane l holding inputs 4(l/16) .. 4(l/16)+3 of sample
+// l%16 (zero for rows past the end); fwd: the bf16 forward fragments,
+// bias[l]: layer l's bias vector
+__device__ __forceinline__ void mlp_forward(f4v xv, const s4v* __restrict__ fwd,
+                                            const float* const* bias, Fwd& F) {
   const int lane = threadIdx.x & 63, q = lane >> 4;
-  f4v xv = {0.f, 0.f, 0.f, 0.f};
-  if (ok) xv = *(const f4v*)(X + s * NPAD + 4 * q);
   F.h0 = pack4(xv);
   const f4v zero = {0.f, 0.f, 0.f, 0.f};
   for (int o = 0; o < 4; ++o) {
     f4v a = mfma(fwd[(FWD_OFF[0] + o) * 64 + lane], F.h0, zero);
-    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[0] + 16 * o + 4 * q + i];
+    for (int i = 0; i < 4; ++i) a[i] += bias[0][16 * o + 4 * q + i];
     F.z1[o] = a;
     f4v h;
     for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
@@ -227,7 +233,7 @@ __device__ __forceinline__ void mlp_forward(const float* __restrict__ X, long lo
   for (int o = 0; o < 4; ++o) {
     f4v a = zero;
     for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[1] + o * 4 + kt) * 64 + lane], F.h1[kt], a);
-    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[1] + 16 * o + 4 * q + i];
+    for (int i = 0; i < 4; ++i) a[i] += bias[1][16 * o + 4 * q + i];
     F.z2[o] = a;
     f4v h;
     for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
@@ -236,7 +242,7 @@ __device__ __forceinline__ void mlp_forward(const float* __restrict__ X, long lo
   for (int o = 0; o < 4; ++o) {
     f4v a = zero;
     for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[2] + o * 4 + kt) * 64 + lane], F.h2[kt], a);
-    for (int i = 0; i < 4; ++i) a[i] += w[OFF_B[2] + 16 * o + 4 * q + i];
+    for (int i = 0; i < 4; ++i) a[i] += bias[2][16 * o + 4 * q + i];
     F.z3[o] = a;
     f4v h;
     for (int i = 0; i < 4; ++i) h[i] = silu(a[i]);
@@ -246,7 +252,7 @@ __device__ __forceinline__ void mlp_forward(const float* __restrict__ X, long lo
   for (int kt = 0; kt < 4; ++kt) a = mfma(fwd[(FWD_OFF[3] + kt) * 64 + lane], F.h3[kt], a);
   for (int i = 0; i < 4; ++i) {
     const int out = 4 * q + i;
-    a[i] += out < NOUT ? w[OFF_B[3] + out] : 0.0f;
+    a[i] += out < NOUT ? bias[3][out] : 0.0f;
   }
   F.y = a;
 }
@@ -262,7 +268,10 @@ __global__ __launch_bounds__(64) void k_cl_forward(const float* __restrict__ X, 
     const long long s = t * 16 + (lane & 15);
     const bool ok = s < n;
     Fwd F;
-    mlp_forward(X, s, ok, fwd, w, F);
+    const float* bias[4] = {w + OFF_B[0], w + OFF_B[1], w + OFF_B[2], w + OFF_B[3]};
+    f4v xv = {0.f, 0.f, 0.f, 0.f};
+    if (ok) xv = *(const f4v*)(X + s * NPAD + 4 * q);
+    mlp_forward(xv, fwd, bias, F);
     for (int i = 0; i < 4; ++i) {
       const int out = 4 * q + i;
       if (ok && out < NOUT) Y[s * NOUT + out] = F.y[i];
@@ -282,142 +291,281 @@ __device__ __forceinline__ s4v transpose_tile(s4v v, unsigned short (*tr)[20]) {
   return o;
 }
 
+// four tiles at once (one LDS round trip): out[e] = transpose of in[e]
+__device__ __forceinline__ void transpose4(const s4v* in, s4v* out,
+                                           unsigned short (*tr)[16][20]) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tr[e][4 * q + i][r] = (unsigned short)in[e][i];
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[e][j] = (short)tr[e][r][4 * q + j];
+  wave_sync();
+}
+
 // sum over the 16 lanes of a quarter (samples), result in every lane
 __device__ __forceinline__ float quarter_sum(float v) {
   for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-// Forward + backward over samples base + (idx ? idx[t] : t), t < n; the
-// gradient of the mean square error over ntot samples.  Per workgroup
-// partial sums of every parameter's gradient and of the squared error.
-__global__ __launch_bounds__(64) void k_cl_grad(const float* __restrict__ X,
-                                                const float* __restrict__ Tg, long long base,
-                                                const int* __restrict__ idx, long long n,
-                                                double ntot, const s4v* __restrict__ fwd,
-                                                const s4v* __restrict__ bwd,
-                                                const float* __restrict__ w,
-                                                float* __restrict__ partial,
-                                                double* __restrict__ lpart) {
-  __shared__ float gW[40][64][4];     // dW^T tiles in accumulator layout
-  __shared__ float gb[3 * NH + NOUT];
-  __shared__ unsigned short tr[16][20];
-  const int lane = threadIdx.x, q = lane >> 4, r = lane & 15;
-  for (int e = lane; e < 40 * 256; e += 64) (&gW[0][0][0])[e] = 0.0f;
-  for (int e = lane; e < 3 * NH + NOUT; e += 64) gb[e] = 0.0f;
-  wave_sync();
-  // tile index of the dW^T accumulators: layer l, (ot, it)
-  auto gidx = [](int l, int ot, int it) {
-    const int off[4] = {0, 4, 20, 36};
-    const int nit[4] = {1, 4, 4, 4};
-    return off[l] + ot * nit[l] + it;
-  };
+// Which samples a gradient evaluation reads: sample base + t (idx and key
+// unused), base + idx[t], or base + the keyed permutation of pos + t over
+// [0, span) (a training minibatch of a shuffled epoch, no index array).
+struct SampleMap {
+  const int* idx = nullptr;
+  long long pos = 0;
+  unsigned span = 0;
+  int half = 0;        // permutation domain 2^(2 half) >= span
+  unsigned key = 0;
+};
+
+// keyed bijection of [0, 2^(2h)): a four-round balanced Feistel network
+// on h-bit halves with an integer-hash round function, cycle-walked into
+// [0, n) (the orbit of x < n returns to [0, n), so the walk ends, and the
+// first value it meets defines a bijection of [0, n))
+__device__ __forceinline__ unsigned feistel_perm(unsigned x, unsigned n, int h, unsigned key) {
+  const unsigned mask = h >= 32 ? 0xFFFFFFFFu : (1u << h) - 1u;
+  do {
+    unsigned L = x >> h, R = x & mask;
+    for (int rd = 0; rd < 4; ++rd) {
+      unsigned f = (R + key + 0x7F4A7C15u * (unsigned)(rd + 1)) * 0x9E3779B1u;
+      f ^= f >> 15;
+      f *= 0x85EBCA77u;
+      f ^= f >> 13;
+      const unsigned t = R;
+      R = (L ^ f) & mask;
+      L = t;
+    }
+    x = (L << h) | R;
+  } while (x >= n);
+  return x;
+}
+
+// Forward + backward of the MSE gradient over n samples (mapped by sm; the
+// mean over ntot).  Workgroups of four wavefronts; each wavefront takes
+// 16-sample tiles in turn and keeps its weight-gradient tiles dW^T in MFMA
+// accumulators and its bias sums per lane, in registers, across all its
+// tiles.  At the end the four wavefronts add their sums into one LDS copy
+// of the parameter vector in a fixed order, and the workgroup writes it as
+// its partial (FP32) with its squared-error sum (FP64).
+constexpr int GWAVES = 4;
+__global__ __launch_bounds__(64 * GWAVES) void k_cl_grad(const float* __restrict__ X,
+                                                         const float* __restrict__ Tg,
+                                                         long long base, SampleMap sm,
+                                                         long long n, double ntot,
+                                                         const s4v* __restrict__ fwd,
+                                                         const s4v* __restrict__ bwd,
+                                                         const float* __restrict__ w,
+                                                         float* __restrict__ partial,
+                                                         double* __restrict__ lpart) {
+  __shared__ float red[NPARAM];
+  __shared__ unsigned short trs[GWAVES][4][16][20];
+  __shared__ double lsh[GWAVES];
+  // the step's bf16 fragments and biases, read from LDS at each use (kept in
+  // registers across the tile loop they would push the accumulators out)
+  __shared__ s4v frag[(NFWD + NBWD) * 64];
+  __shared__ float bsh[3 * NH + NOUT];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  {
+    // all loads issued before the LDS stores wait for them
+    constexpr int NF = (NFWD + NBWD) * 64, PER = (NF + 64 * GWAVES - 1) / (64 * GWAVES);
+    s4v tmp[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * 64 * GWAVES;
+      if (e < NF) tmp[k] = e < NFWD * 64 ? fwd[e] : bwd[e - NFWD * 64];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * 64 * GWAVES;
+      if (e < NF) frag[e] = tmp[k];
+    }
+  }
+  for (int e = threadIdx.x; e < 3 * NH + NOUT; e += 64 * GWAVES)
+    bsh[e] = w[e < NH ? OFF_B[0] + e : (e < 2 * NH ? OFF_B[1] + e - NH
+                                                   : (e < 3 * NH ? OFF_B[2] + e - 2 * NH
+                                                                 : OFF_B[3] + e - 3 * NH))];
+  __syncthreads();
+  const s4v* fw = frag;
+  const s4v* bw = frag + NFWD * 64;
+  const float* bias[4] = {bsh, bsh + NH, bsh + 2 * NH, bsh + 3 * NH};
+  unsigned short (*tr4)[16][20] = trs[wv];
+  unsigned short (*tr)[20] = trs[wv][0];
+  const f4v zero = {0.f, 0.f, 0.f, 0.f};
+  f4v acc[40];
+  f4v bacc[13];  // bias sums: layer 1..3 tiles 0..3 at [4 L + ot], layer 4 at [12]
+#pragma unroll
+  for (int e = 0; e < 40; ++e) acc[e] = zero;
+#pragma unroll
+  for (int e = 0; e < 13; ++e) bacc[e] = zero;
   const float coef = (float)(2.0 / (ntot * NOUT));
   double lsum = 0.0;
   const long long ntile = (n + 15) / 16;
-  const f4v zero = {0.f, 0.f, 0.f, 0.f};
-  for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
+  // the gathered inputs and targets of a tile; the next tile's are loaded
+  // while the current one runs
+  auto gather = [&](long long t, f4v& xv, f4v& tg, bool& ok) {
     const long long tl = t * 16 + r;
-    const bool ok = tl < n;
-    const long long s = ok ? base + (idx ? (long long)idx[tl] : tl) : 0;
+    ok = t < ntile && tl < n;
+    long long s = 0;
+    if (ok) {
+      if (sm.idx) s = base + sm.idx[tl];
+      else if (sm.span) s = base + feistel_perm((unsigned)(sm.pos + tl), sm.span, sm.half, sm.key);
+      else s = base + tl;
+    }
+    xv = zero;
+    tg = zero;
+    if (ok) {
+      xv = *(const f4v*)(X + s * NPAD + 4 * q);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (4 * q + i < NOUT) tg[i] = Tg[s * NOUT + 4 * q + i];
+    }
+  };
+  const long long tstride = (long long)gridDim.x * GWAVES;
+  long long t = (long long)blockIdx.x * GWAVES + wv;
+  f4v xn, tn;
+  bool okn;
+  gather(t, xn, tn, okn);
+  for (; t < ntile; t += tstride) {
+    const f4v xv = xn, tg = tn;
+    const bool ok = okn;
+    gather(t + tstride, xn, tn, okn);
     Fwd F;
-    mlp_forward(X, s, ok, fwd, w, F);
-    // output error
+    mlp_forward(xv, fw, bias, F);
     f4v dz4;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int out = 4 * q + i;
       float d = 0.0f;
-      if (ok && out < NOUT) d = F.y[i] - Tg[s * NOUT + out];
+      if (ok && out < NOUT) d = F.y[i] - tg[i];
       lsum += (double)d * (double)d;
       dz4[i] = coef * d;
     }
+    bacc[12] += dz4;
     const s4v dz4b = pack4(dz4);
-    // layer 4: dW4^T[out x in] += dz4^T H3, db4, dH3^T = W4 dz4^T
+    // layer 4: dW4^T += dz4^T H3, dH3^T = W4 dz4^T
     {
       const s4v A = transpose_tile(dz4b, tr);
-      for (int it = 0; it < 4; ++it) {
-        const s4v B = transpose_tile(F.h3[it], tr);
-        const int gi = gidx(3, 0, it);
-        f4v acc = *(f4v*)&gW[gi][lane][0];
-        *(f4v*)&gW[gi][lane][0] = mfma(A, B, acc);
-      }
-      for (int i = 0; i < 4; ++i) {
-        const float v = quarter_sum(dz4[i]);
-        if (r == 0 && 4 * q + i < NOUT) gb[3 * NH + 4 * q + i] += v;
-      }
+      s4v B[4];
+      transpose4(F.h3, B, tr4);
+#pragma unroll
+      for (int it = 0; it < 4; ++it) acc[36 + it] = mfma(A, B[it], acc[36 + it]);
     }
     s4v dzb[4];
     f4v dz[4];
+#pragma unroll
     for (int it = 0; it < 4; ++it) {
-      const f4v dh = mfma(bwd[(BWD_OFF[3] + it) * 64 + lane], dz4b, zero);
+      const f4v dh = mfma(bw[(BWD_OFF[3] + it) * 64 + lane], dz4b, zero);
+#pragma unroll
       for (int i = 0; i < 4; ++i) dz[it][i] = dh[i] * silu_grad(F.z3[it][i]);
       dzb[it] = pack4(dz[it]);
+      bacc[8 + it] += dz[it];
     }
-    // layers 3 and 2: dW^T += dz^T H_in, db, dH_in^T = W dz^T
+    // layers 3 and 2: dW^T += dz^T H_in, dH_in^T = W dz^T
+#pragma unroll
     for (int l = 2; l >= 1; --l) {
       const s4v* hin = l == 2 ? F.h2 : F.h1;
-      s4v At[4];
-      for (int ot = 0; ot < 4; ++ot) At[ot] = transpose_tile(dzb[ot], tr);
-      for (int it = 0; it < 4; ++it) {
-        const s4v B = transpose_tile(hin[it], tr);
-        for (int ot = 0; ot < 4; ++ot) {
-          const int gi = gidx(l, ot, it);
-          f4v acc = *(f4v*)&gW[gi][lane][0];
-          *(f4v*)&gW[gi][lane][0] = mfma(At[ot], B, acc);
-        }
-      }
-      for (int ot = 0; ot < 4; ++ot)
-        for (int i = 0; i < 4; ++i) {
-          const float v = quarter_sum(dz[ot][i]);
-          if (r == 0) gb[l * NH + 16 * ot + 4 * q + i] += v;
-        }
+      const int off = l == 2 ? 20 : 4;
+      s4v At[4], Bt[4];
+      transpose4(dzb, At, tr4);
+      transpose4(hin, Bt, tr4);
+#pragma unroll
+      for (int it = 0; it < 4; ++it)
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot)
+          acc[off + ot * 4 + it] = mfma(At[ot], Bt[it], acc[off + ot * 4 + it]);
       const f4v* zin = l == 2 ? F.z2 : F.z1;
       s4v nb[4];
       f4v nd[4];
+#pragma unroll
       for (int it = 0; it < 4; ++it) {
         f4v dh = zero;
-        for (int ot = 0; ot < 4; ++ot) dh = mfma(bwd[(BWD_OFF[l] + it * 4 + ot) * 64 + lane], dzb[ot], dh);
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) dh = mfma(bw[(BWD_OFF[l] + it * 4 + ot) * 64 + lane], dzb[ot], dh);
+#pragma unroll
         for (int i = 0; i < 4; ++i) nd[it][i] = dh[i] * silu_grad(zin[it][i]);
         nb[it] = pack4(nd[it]);
       }
+#pragma unroll
       for (int it = 0; it < 4; ++it) {
         dz[it] = nd[it];
         dzb[it] = nb[it];
+        bacc[4 * (l - 1) + it] += nd[it];
       }
     }
-    // layer 1: dW1^T += dz1^T X, db1
+    // layer 1: dW1^T += dz1^T X
     {
       const s4v B = transpose_tile(F.h0, tr);
-      for (int ot = 0; ot < 4; ++ot) {
-        const s4v A = transpose_tile(dzb[ot], tr);
-        const int gi = gidx(0, ot, 0);
-        f4v acc = *(f4v*)&gW[gi][lane][0];
-        *(f4v*)&gW[gi][lane][0] = mfma(A, B, acc);
-      }
-      for (int ot = 0; ot < 4; ++ot)
-        for (int i = 0; i < 4; ++i) {
-          const float v = quarter_sum(dz[ot][i]);
-          if (r == 0) gb[16 * ot + 4 * q + i] += v;
-        }
+      s4v At[4];
+      transpose4(dzb, At, tr4);
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(At[ot], B, acc[ot]);
     }
   }
-  wave_sync();
-  // accumulators -> flat layout: tile (l, ot, it), lane holds
-  // dW^T[out = 16ot + 4q + i][in = 16it + lane%16]
-  float* pp = partial + (long long)blockIdx.x * NPARAM;
-  for (int l = 0; l < 4; ++l)
-    for (int ot = 0; ot < OT[l]; ++ot)
-      for (int it = 0; it < KT[l]; ++it) {
-        const int gi = gidx(l, ot, it);
-        for (int i = 0; i < 4; ++i) {
-          const int out = 16 * ot + 4 * q + i, in = 16 * it + r;
-          if (out < OUTD[l] && in < IND[l]) pp[OFF_W[l] + in * OUTD[l] + out] = gW[gi][lane][i];
+  // bias sums over the 16 samples of a quarter
+#pragma unroll
+  for (int e = 0; e < 13; ++e)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bacc[e][i] = quarter_sum(bacc[e][i]);
+  // the wavefronts' sums into two LDS copies of the parameter vector (the
+  // fragment copy is free now): wavefronts 0 and 1 write copies A and B,
+  // then 2 and 3 add into them; the partial is A + B, i.e. (w0 + w2) + (w1 + w3)
+  __syncthreads();
+  float* const cpyB = (float*)frag;
+  auto put = [&](float* dst, auto firstc) {
+    constexpr bool first = decltype(firstc)::value;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int off = l == 0 ? 0 : (l == 1 ? 4 : (l == 2 ? 20 : 36));
+      const int nit = l == 0 ? 1 : 4, not_ = l == 3 ? 1 : 4;
+#pragma unroll
+      for (int ot = 0; ot < not_; ++ot)
+#pragma unroll
+        for (int it = 0; it < nit; ++it) {
+          const f4v a = acc[off + ot * nit + it];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int out = 16 * ot + 4 * q + i, in = 16 * it + r;
+            if (out < OUTD[l] && in < IND[l]) {
+              float& d = dst[OFF_W[l] + in * OUTD[l] + out];
+              d = first ? a[i] : d + a[i];
+            }
+          }
         }
-      }
-  for (int l = 0; l < 4; ++l)
-    for (int o = lane; o < OUTD[l]; o += 64) pp[OFF_B[l] + o] = gb[l * NH + o];
+    }
+    if (r == 0) {
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            float& d = dst[OFF_B[l] + 16 * ot + 4 * q + i];
+            d = first ? bacc[4 * l + ot][i] : d + bacc[4 * l + ot][i];
+          }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (4 * q + i < NOUT) {
+          float& d = dst[OFF_B[3] + 4 * q + i];
+          d = first ? bacc[12][i] : d + bacc[12][i];
+        }
+    }
+  };
+  if (wv < 2) put((wv & 1) ? cpyB : red, std::true_type{});
+  __syncthreads();
+  if (wv >= 2) put((wv & 1) ? cpyB : red, std::false_type{});
+  __syncthreads();
+  float* pp = partial + (long long)blockIdx.x * NPARAM;
+  for (int e = threadIdx.x; e < NPARAM; e += 64 * GWAVES) pp[e] = red[e] + cpyB[e];
   for (int o = 32; o > 0; o >>= 1) lsum += __shfl_down(lsum, o, 64);
-  if (lane == 0) lpart[blockIdx.x] = lsum;
+  if (lane == 0) lsh[wv] = lsum;
+  __syncthreads();
+  if (threadIdx.x == 0) lpart[blockIdx.x] = ((lsh[0] + lsh[1]) + lsh[2]) + lsh[3];
 }
 
 // grad[p] = sum of the workgroup partials (FP64, a fixed order: sixteen
@@ -465,6 +613,65 @@ __global__ void k_cl_adam(float* __restrict__ w, const float* __restrict__ g,
   v[p] = vp;
   const float mh = mp / c1, vh = vp / c2;
   w[p] = w[p] - lr * mh / (sqrtf(vh) + eps);
+}
+
+// One training step's update: each parameter's gradient is the FP64 sum of
+// the workgroup partials in k_cl_reduce's fixed order; Adam updates the
+// master, and the thread writes the master's bf16 value into the forward and
+// backward fragments that hold it (the layouts of k_cl_pack).  loss as
+// k_cl_reduce.
+__global__ __launch_bounds__(1024) void k_cl_reduce_adam(const float* __restrict__ partial,
+                                                         int nwg,
+                                                         const double* __restrict__ lpart,
+                                                         double ntot, float* __restrict__ w,
+                                                         float* __restrict__ m,
+                                                         float* __restrict__ v, float lr,
+                                                         float c1, float c2,
+                                                         float* __restrict__ loss,
+                                                         s4v* __restrict__ fwd,
+                                                         s4v* __restrict__ bwd) {
+  __shared__ double sh[16][64];
+  const int t = threadIdx.x, pl = t & 63, bg = t >> 6;
+  const int p = blockIdx.x * 64 + pl;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (p < NPARAM)
+    for (int b = bg, rr = 0; b < nwg; b += 16, rr = (rr + 1) & 3)
+      a[rr] += (double)partial[(long long)b * NPARAM + p];
+  sh[bg][pl] = (a[0] + a[1]) + (a[2] + a[3]);
+  __syncthreads();
+  if (bg == 0 && p < NPARAM && !(p < OFF_B[0] && p / NH >= NIN)) {
+    double sum = 0.0;
+    for (int qq = 0; qq < 16; ++qq) sum += sh[qq][pl];
+    const float gp = (float)sum;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mp = b1 * m[p] + (1.0f - b1) * gp;
+    const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+    m[p] = mp;
+    v[p] = vp;
+    const float wn = w[p] - lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+    w[p] = wn;
+    // the weight's places in the bf16 fragments (biases have none)
+    int l = 3;
+    while (l > 0 && p < OFF_W[l]) --l;
+    const int loc = p - OFF_W[l];
+    if (loc < IND[l] * OUTD[l]) {
+      const int in = loc / OUTD[l], out = loc % OUTD[l];
+      const short hb = (short)to_bf16(wn);
+      {
+        const int kt = in / 16, qq = (in % 16) / 4, j = in % 4, o = out / 16, r = out % 16;
+        ((short*)&fwd[(FWD_OFF[l] + o * KT[l] + kt) * 64 + 16 * qq + r])[j] = hb;
+      }
+      if (l >= 1) {
+        const int it = in / 16, r = in % 16, ot = out / 16, qq = (out % 16) / 4, j = out % 4;
+        ((short*)&bwd[(BWD_OFF[l] + it * OT[l] + ot) * 64 + 16 * qq + r])[j] = hb;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && t == 0 && loss) {
+    double s2 = 0.0;
+    for (int b = 0; b < nwg; ++b) s2 += lpart[b];
+    *loss = (float)(s2 / (ntot * NOUT));
+  }
 }
 
 // ------------------------------------------------ dynamic Smagorinsky --
@@ -708,8 +915,6 @@ struct c3d_closure {
   s4v* bwd = nullptr;
   float* partial = nullptr;        // GRAD_WG_MAX x NPARAM
   double* lpart = nullptr;
-  int* perm = nullptr;
-  long long perm_cap = 0;
   double* fb = nullptr;            // filter scratch
   long long fb_cap = 0;
   float* Y = nullptr;              // predictions
@@ -740,13 +945,25 @@ int cl_range(c3d_closure* cl, long long first, long long count, const char* what
   return C3D_OK;
 }
 
-// gradient over samples first + (idx ? idx[t] : t), t < count, into cl->g;
-// loss into *dloss (device)
-int cl_grad(c3d_closure* cl, long long first, const int* idx, long long count, float* dloss) {
+// workgroups of a gradient evaluation over count samples: two 16-sample
+// tiles per wavefront while that fills GRAD_WG_MAX workgroups
+int grad_wgs(long long count) {
   const long long ntile = (count + 15) / 16;
-  const int nwg = (int)std::min<long long>(GRAD_WG_MAX, ntile);
-  hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64), 0, cl->stream, cl->X, cl->T, first, idx,
-                     count, (double)count, cl->fwd, cl->bwd, cl->w, cl->partial, cl->lpart);
+  // two tiles per wavefront up to 1024 tiles, then more per wavefront (fewer
+  // partials for the reduction): measured best at 16384 and 65536 samples
+  const int tpw = (int)std::max<long long>(2, ntile / 1024);
+  return (int)std::max<long long>(1, std::min<long long>(GRAD_WG_MAX, (ntile + tpw * GWAVES - 1) /
+                                                                          (tpw * GWAVES)));
+}
+
+// gradient over the samples sm maps from first, t < count, into cl->g;
+// loss into *dloss (device)
+int cl_grad(c3d_closure* cl, long long first, const SampleMap& sm, long long count,
+            float* dloss) {
+  const int nwg = grad_wgs(count);
+  hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64 * GWAVES), 0, cl->stream, cl->X, cl->T,
+                     first, sm, count, (double)count, cl->fwd, cl->bwd, cl->w, cl->partial,
+                     cl->lpart);
   C3D_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_cl_reduce, dim3((NPARAM + 63) / 64), dim3(1024), 0, cl->stream,
                      cl->partial, nwg, cl->lpart, (double)count, cl->g, dloss);
@@ -815,7 +1032,7 @@ extern "C" int c3d_closure_destroy(c3d_closure* cl) {
   cl->users.clear();
   if (cl->stream) hipStreamSynchronize(cl->stream);
   void* bufs[] = {cl->X, cl->T, cl->S, cl->w, cl->m, cl->v, cl->g, cl->loss, cl->fwd, cl->bwd,
-                  cl->partial, cl->lpart, cl->perm, cl->fb, cl->Y};
+                  cl->partial, cl->lpart, cl->fb, cl->Y};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (cl->stream) hipStreamDestroy(cl->stream);
@@ -968,7 +1185,7 @@ extern "C" int c3d_closure_grad(c3d_closure* cl, long long first, long long coun
     if ((rc = cl_alloc((void**)&cl->loss, sizeof(float) * 1024))) return rc;
     cl->loss_cap = 1024;
   }
-  if ((rc = cl_grad(cl, first, nullptr, count, cl->loss))) return rc;
+  if ((rc = cl_grad(cl, first, SampleMap(), count, cl->loss))) return rc;
   if (grad) C3D_HIP(hipMemcpyAsync(grad, cl->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost,
                                    cl->stream));
   if (loss) C3D_HIP(hipMemcpyAsync(loss, cl->loss, sizeof(float), hipMemcpyDeviceToHost,
@@ -993,33 +1210,40 @@ extern "C" int c3d_closure_train(c3d_closure* cl, long long first, long long cou
     if ((rc = cl_alloc((void**)&cl->loss, sizeof(float) * steps))) return rc;
     cl->loss_cap = steps;
   }
-  if (cl->perm_cap < count) {
-    if (cl->perm) hipFree(cl->perm);
-    cl->perm = nullptr;
-    if ((rc = cl_alloc((void**)&cl->perm, sizeof(int) * count))) return rc;
-    cl->perm_cap = count;
+  if (count > 0x7FFFFFFFLL) {
+    set_error("c3d_closure_train: at most 2^31 - 1 samples per training range");
+    return C3D_ERR_ARG;
   }
-  std::vector<int> perm(count);
+  // Minibatches of a shuffled epoch: positions pos .. pos+batch-1 of a keyed
+  // permutation of the range (a new key per epoch from the seed), evaluated
+  // in the gradient kernel; an epoch's tail shorter than a batch is skipped.
+  int half = 1;
+  while ((1LL << (2 * half)) < count) ++half;
   std::mt19937 gen(seed);
-  long long pos = count;  // reshuffle on the first step
+  long long pos = count;  // a new epoch on the first step
+  unsigned key = 0;
+  const int nwg = grad_wgs(batch);
   for (int st = 0; st < steps; ++st) {
     if (pos + batch > count) {
-      for (long long t = 0; t < count; ++t) perm[t] = (int)t;
-      std::shuffle(perm.begin(), perm.end(), gen);
-      // the previous epoch's steps read the old order: finish them first
-      C3D_HIP(hipStreamSynchronize(cl->stream));
-      C3D_HIP(hipMemcpyAsync(cl->perm, perm.data(), sizeof(int) * count,
-                             hipMemcpyHostToDevice, cl->stream));
+      key = (unsigned)gen();
       pos = 0;
     }
-    if ((rc = cl_grad(cl, first, cl->perm + pos, batch, cl->loss + st))) return rc;
+    SampleMap sm;
+    sm.pos = pos;
+    sm.span = (unsigned)count;
+    sm.half = half;
+    sm.key = key;
+    hipLaunchKernelGGL(k_cl_grad, dim3(nwg), dim3(64 * GWAVES), 0, cl->stream, cl->X, cl->T,
+                       first, sm, (long long)batch, (double)batch, cl->fwd, cl->bwd, cl->w,
+                       cl->partial, cl->lpart);
+    C3D_HIP(hipGetLastError());
     ++cl->adam_t;
     const float c1 = 1.0f - std::pow(0.9f, (float)cl->adam_t);
     const float c2 = 1.0f - std::pow(0.999f, (float)cl->adam_t);
-    hipLaunchKernelGGL(k_cl_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream, cl->w,
-                       cl->g, cl->m, cl->v, lr, c1, c2);
+    hipLaunchKernelGGL(k_cl_reduce_adam, dim3((NPARAM + 63) / 64), dim3(1024), 0, cl->stream,
+                       cl->partial, nwg, cl->lpart, (double)batch, cl->w, cl->m, cl->v, lr, c1,
+                       c2, cl->loss + st, cl->fwd, cl->bwd);
     C3D_HIP(hipGetLastError());
-    if ((rc = cl_pack(cl))) return rc;
     pos += batch;
   }
   if (losses)

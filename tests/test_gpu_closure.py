@@ -378,6 +378,47 @@ def test_les_viscosity_dynamic_smagorinsky_mode():
     cl.close()
 
 
+def test_samples_and_nut_at_the_les96_shape():
+    """the learned-closure configuration's own shapes: samples of a
+    384x192x288 DNS state filtered 4x2x4 onto 96x96x72 against the oracle
+    (FP32 storage, 1e-6), and the closure's viscosity on a 96x96x72 LES
+    context against the oracle chain (the bars of _check_nut)"""
+    from oracle import closure as C
+    from channel3d_amd.closure import Closure
+    from tests import cases
+    dns = dict(cases.LES384)
+    dns.pop("les_type")
+    dns.pop("filter_type")
+    g = _smooth_dns(dns, seed=31)
+    g.step()
+    cl = Closure(96 * 96 * 72)
+    first, n = cl.add_samples(g)
+    assert n == 96 * 96 * 72
+    X, T, S = cl.samples(first, n)
+    Xo, To, So, _ = _oracle_samples(g, dns)
+    g.close()
+    for a, b, nm in ((X, Xo, "X"), (T, To, "T")):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"96x96x72 samples {nm}: max err {err:.2e} of the largest", flush=True)
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(b).max()), nm
+    assert np.allclose(S, So, rtol=1e-6, atol=0)
+    # a network trained briefly on these samples as the LES model
+    p = C.init_params(9)
+    cl.set_params(p)
+    cl.train(first, n, batch=4096, steps=50, lr=2e-3, seed=5)
+    p = cl.get_params()
+    les = dict(cases.LES96)
+    h = _smooth_dns(les, seed=32)
+    h.set_les_closure(cl)
+    h.Clc_SGS_Vis()
+    h.sync()
+    from channel3d_amd import lib as L
+    _check_nut(h, les, p, h.download(L.NUT), cl.fallbacks())
+    h.set_les_closure(None)
+    h.close()
+    cl.close()
+
+
 def test_les_closure_steps_replay_from_graphs():
     """LES steps with the closure as SGS model, launched and replayed from
     HIP graphs, agree bit for bit (the closure's scratch is allocated when
