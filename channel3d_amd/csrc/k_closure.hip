@@ -856,33 +856,40 @@ __global__ void k_cl_nut(Geom g, int nl, const float* __restrict__ Y,
                          const double* __restrict__ cs2, const double* __restrict__ dY,
                          double dX, double dZ, double xnu, int bcym, int bcyp,
                          double* __restrict__ nut, int* __restrict__ nfallback) {
+  __shared__ int nfb_block;
+  if (threadIdx.x == 0) nfb_block = 0;
+  __syncthreads();
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nl) return;
-  const int i = (int)(c % g.nx), J = (int)((c / g.nx) % g.ny), k = (int)(c / ((long long)g.nx * g.ny));
+  const bool live = c < nl;
+  const long long cc = live ? c : nl - 1;
+  const int i = (int)(cc % g.nx), J = (int)((cc / g.nx) % g.ny), k = (int)(cc / ((long long)g.nx * g.ny));
   double nu = -1.0;
   if (!DSM) {
     double t[6];
-    for (int q = 0; q < 6; ++q) t[q] = (double)Y[c * NOUT + q] * (double)scale[c];
+    for (int q = 0; q < 6; ++q) t[q] = (double)Y[cc * NOUT + q] * (double)scale[cc];
     const double tr = (t[0] + t[1] + t[2]) / 3.0;
     for (int q = 0; q < 3; ++q) t[q] -= tr;
     double ts = 0.0, ss = 0.0;
     for (int q = 0; q < 6; ++q) {
       const double wt = q < 3 ? 1.0 : 2.0;
-      const double S = w[q * (long long)nl + c];
+      const double S = w[q * (long long)nl + cc];
       ts += wt * t[q] * S;
       ss += wt * S * S;
     }
     nu = -ts / (2.0 * ss);
   }
-  const bool back = DSM || !(nu >= 0.0) || !isfinite(nu);
+  const bool back = live && (DSM || !(nu >= 0.0) || !isfinite(nu));
   if (back) {
-    const double sn = w[6 * (long long)nl + c];
+    const double sn = w[6 * (long long)nl + cc];
     nu = fmax(cs2[J], 0.0) * pow(dX * dY[J] * dZ, 2.0 / 3.0) * sn;
   }
-  // one counter update per wavefront
+  // fallback cells: one LDS update per wavefront, one counter update per
+  // workgroup (one per wavefront on a single address serialised the kernel)
   const unsigned long long m = __ballot(back);
-  if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)__ballot(1)) - 1) && m)
-    atomicAdd(nfallback, (int)__popcll(m));
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&nfb_block, (int)__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0 && nfb_block) atomicAdd(nfallback, nfb_block);
+  if (!live) return;
   const double v = xnu + nu;
   const int jc = J + 1;
   nut[fidx(g, i, jc, k)] = v;
@@ -1166,7 +1173,7 @@ extern "C" int c3d_closure_predict(c3d_closure* cl, long long first, long long c
     cl->y_cap = count;
   }
   const long long ntile = (count + 15) / 16;
-  const int nwg = (int)std::min<long long>(4096, ntile);
+  const int nwg = (int)std::min<long long>(1 << 20, ntile);
   hipLaunchKernelGGL(k_cl_forward, dim3(nwg), dim3(64), 0, cl->stream, cl->X + first * NPAD,
                      count, cl->fwd, cl->w, cl->Y);
   C3D_HIP(hipGetLastError());
@@ -1435,7 +1442,7 @@ int c3d::closure_nut(c3d_closure* cl, c3d_ctx* c, const void* ux, const void* uy
                        (const double*)fb, (const double*)dyc, (const double*)ddY, dX, dZ,
                        c->p.yly, X, T, S);
     const long long ntile = (nl + 15) / 16;
-    hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(4096, ntile)), dim3(64),
+    hipLaunchKernelGGL(k_cl_forward, dim3((unsigned)std::min<long long>(1 << 20, ntile)), dim3(64),
                        0, st, (const float*)X, nl, cl->fwd, cl->w, Y);
   }
   hipLaunchKernelGGL(k_cl_zero_int, dim3(1), dim3(1), 0, st, c->cln_fb);
