@@ -47,25 +47,26 @@ constexpr int NFWD = 40, NBWD = 36;
 constexpr float CLIPV = 10.0f;
 constexpr int GRAD_WG_MAX = 512;
 
+// round to nearest even (gfx950 converts in hardware, v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short to_bf16(float f) {
-  unsigned int b = __float_as_uint(f);
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  return (unsigned short)(b >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ __forceinline__ float from_bf16(unsigned short h) {
   return __uint_as_float((unsigned int)h << 16);
 }
 __device__ __forceinline__ s4v pack4(f4v v) {
   s4v r;
-  r.x = (short)to_bf16(v.x);
-  r.y = (short)to_bf16(v.y);
-  r.z = (short)to_bf16(v.z);
-  r.w = (short)to_bf16(v.w);
+  r.x = __builtin_bit_cast(short, (__bf16)v.x);
+  r.y = __builtin_bit_cast(short, (__bf16)v.y);
+  r.z = __builtin_bit_cast(short, (__bf16)v.z);
+  r.w = __builtin_bit_cast(short, (__bf16)v.w);
   return r;
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative with the hardware exponential (v_exp_f32 on
+// -z log2 e): a few ulp from expf, far below the bf16 rounding that follows
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ f4v mfma(s4v a, s4v b, f4v c) {
@@ -581,9 +582,17 @@ __global__ __launch_bounds__(1024) void k_cl_reduce(const float* __restrict__ pa
   const int t = threadIdx.x, pl = t & 63, bg = t >> 6;
   const int p = blockIdx.x * 64 + pl;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
+  // four partials per iteration, loaded before they are added (independent
+  // loads in flight); partial b goes to accumulator ((b - bg) / 16) % 4
   if (p < NPARAM)
-    for (int b = bg, r = 0; b < nwg; b += 16, r = (r + 1) & 3)
-      a[r] += (double)partial[(long long)b * NPARAM + p];
+    for (int b = bg; b < nwg; b += 64) {
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        v[u] = b + 16 * u < nwg ? partial[(long long)(b + 16 * u) * NPARAM + p] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] += (double)v[u];
+    }
   sh[bg][pl] = (a[0] + a[1]) + (a[2] + a[3]);
   __syncthreads();
   if (bg == 0 && p < NPARAM) {
@@ -634,9 +643,17 @@ __global__ __launch_bounds__(1024) void k_cl_reduce_adam(const float* __restrict
   const int t = threadIdx.x, pl = t & 63, bg = t >> 6;
   const int p = blockIdx.x * 64 + pl;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
+  // four partials per iteration, loaded before they are added (independent
+  // loads in flight); partial b goes to accumulator ((b - bg) / 16) % 4
   if (p < NPARAM)
-    for (int b = bg, rr = 0; b < nwg; b += 16, rr = (rr + 1) & 3)
-      a[rr] += (double)partial[(long long)b * NPARAM + p];
+    for (int b = bg; b < nwg; b += 64) {
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        v[u] = b + 16 * u < nwg ? partial[(long long)(b + 16 * u) * NPARAM + p] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] += (double)v[u];
+    }
   sh[bg][pl] = (a[0] + a[1]) + (a[2] + a[3]);
   __syncthreads();
   if (bg == 0 && p < NPARAM && !(p < OFF_B[0] && p / NH >= NIN)) {
