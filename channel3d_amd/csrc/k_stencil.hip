@@ -7,6 +7,8 @@
 // ghost planes exactly as the reference reads its halo arrays.
 #include "c3d_internal.h"
 
+#include <type_traits>
+
 
 namespace c3d {
 
@@ -86,71 +88,134 @@ __device__ __forceinline__ void block_sum_fix(long long* v, long long (*sh)[4]) 
 // MASK bit 0/1/2 selects the x/y/z component; the x component also writes
 // the per-block partial of the bulk-velocity integral s1tot (:46).
 //
-// A thread owns one (i, j) column and marches a run of z planes.  The
-// values its own column needs at k-1, k and k+1 (and the few off-column
-// ones the conservative fluxes need one plane up or down) roll through
-// registers, so every plane of every field is fetched once per run; x and
-// y neighbours of the current plane come from the tile's other threads
-// through the caches.  Expressions keep the reference's operation order.
+// A workgroup of 64 x 4 threads (one wavefront per row of j) owns a tile of
+// 64 x 4 cells and marches a run of z planes.  Each plane of ux, uy, uz, pr
+// over the tile plus a one-cell halo is loaded once, with coalesced row
+// loads, into LDS, so every neighbour read is an LDS read at a fixed offset.
+// Three plane buffers rotate: plane k is computed from the buffers of planes
+// k and k+1 while plane k+2 is in flight into registers; it is then written
+// into the buffer plane k-1 held, one barrier per plane.  The own column's
+// values at k-1 roll through registers.
+//
+// The order of the memory operations is chosen for the single vector-memory
+// counter, which also counts stores: the histories of plane k are requested
+// before plane k+2 (waiting for them does not wait for the prefetch), and the
+// plane's results are stored after plane k+2 has been staged (waiting for
+// the prefetch does not wait for the stores).  Tiles entirely inside the
+// domain run without the per-cell guard, so no path skips the stores and the
+// compiler's counter bookkeeping stays exact.  Expressions keep the
+// reference's operation order.
 // ---------------------------------------------------------------------------
-template <typename T, int MASK, int BYR, bool NTS = false>
-__global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
-                                                const T* __restrict__ ux, const T* __restrict__ uy,
-                                                const T* __restrict__ uz, const T* __restrict__ pr,
-                                                T* __restrict__ rx, T* __restrict__ ry,
-                                                T* __restrict__ rz, T* __restrict__ hx,
-                                                T* __restrict__ hy, T* __restrict__ hz,
-                                                double* __restrict__ partials, int kchunk) {
-  __shared__ long long shf[BYR][4];
-  const int cgx = (g.nx + BX - 1) / BX, cgy = (g.ny + BYR - 1) / BYR;
+constexpr int RHS_WY = 4;  // rows of j per RHS tile (one wavefront each)
+
+template <typename T, int MASK, bool NTS = false>
+__global__ __launch_bounds__(BX* RHS_WY, 1) void k_rhs(Geom g, Metrics<T> m, Coef<T> cf,
+                                                      const T* __restrict__ ux,
+                                                      const T* __restrict__ uy,
+                                                      const T* __restrict__ uz,
+                                                      const T* __restrict__ pr,
+                                                      T* __restrict__ rx, T* __restrict__ ry,
+                                                      T* __restrict__ rz, T* __restrict__ hx,
+                                                      T* __restrict__ hy, T* __restrict__ hz,
+                                                      double* __restrict__ partials, int kchunk) {
+  constexpr int WY = RHS_WY, LW = BX + 2, LP = LW * (WY + 2);  // one field's tile plane
+  constexpr int NT = BX * WY, NS = (LP + NT - 1) / NT;         // staging slots per thread
+  __shared__ T sb[3][4][LP];
+  __shared__ long long shf[WY][4];
+  const int tx = threadIdx.x;
+  const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one row per wavefront
+  const int tid = tx + BX * ty;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int cgx = (nx + BX - 1) / BX, cgy = (ny + WY - 1) / WY;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int cbx = lb % cgx, cby = (lb / cgx) % cgy, cbz = lb / (cgx * cgy);
-  const int i = cbx * BX + threadIdx.x;
-  const int jc = cby * BYR + threadIdx.y + 1;  // reference j
-  const int k0 = cbz * kchunk;
+  const int i0 = cbx * BX, j0 = cby * WY + 1;  // first cell of the tile (reference j)
+  const int k0 = cbz * kchunk, k1 = min(nz, k0 + kchunk);
+  const long long sk = g.plane;
+  // staging slot s of this thread: tile element e = tid + NT s, at halo row
+  // jj = e / LW (j = j0 - 1 + jj) and column ii = e % LW (i = i0 - 1 + ii,
+  // periodic); rows past the top ghost and columns past the wrap are loaded
+  // from in-range cells and never read
+  int soff[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int e = tid + NT * s;
+    const int jj = e / LW, ii = e - jj * LW;
+    int i = i0 - 1 + ii;
+    i = i < 0 ? nx - 1 : (i >= nx ? i % nx : i);
+    const int j = min(j0 - 1 + jj, ny + 1);
+    soff[s] = e < LP ? j * nx + i : -1;
+  }
+  T st[4][NS];
+  auto fetch = [&](int k) {
+    const long long pb = sk * k;
+    const T *a = ux + pb, *b = uy + pb, *c = uz + pb, *d = pr + pb;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (soff[s] >= 0) {
+        st[0][s] = a[soff[s]]; st[1][s] = b[soff[s]];
+        st[2][s] = c[soff[s]]; st[3][s] = d[soff[s]];
+      }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (soff[s] >= 0) {
+        const int e = tid + NT * s;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) sb[buf][f][e] = st[f][s];
+      }
+  };
+
+  const int i = i0 + tx, ic = min(i, nx - 1);
+  const int jc = j0 + ty, jrow = min(jc, ny), jp = jrow + 1;
+  const int e = (ty + 1) * LW + tx + 1;  // this cell in the tile plane
+  const T qdx1 = T(0.25) * cf.rdx, qdx3 = T(0.25) * cf.rdz;
+  const T hdx1 = T(0.5) * cf.rdx, hdx3 = T(0.5) * cf.rdz;
+  const T am1c = cf.rdx2, ap1c = cf.rdx2, ac1c = -(am1c + ap1c);
+  const T am3c = cf.rdz2, ap3c = cf.rdz2, ac3c = -(am3c + ap3c);
   long long s1[4] = {0, 0, 0, 0};
-  if (i < g.nx && jc <= g.ny) {
-    const int nx = g.nx, nz = g.nz;
-    const long long sj = nx, sk = g.plane;
-    const int im = wrapm(i, nx), ip = wrapp(i, nx);
-    const int jm = jc - 1, jp = jc + 1;
-    const T I1 = m.yint[jc], I2 = T(1) - I1, I3 = m.yint[jp], I4 = T(1) - I3;
-    const T qdx1 = T(0.25) * cf.rdx, qdx3 = T(0.25) * cf.rdz;
-    const T hdx1 = T(0.5) * cf.rdx, hdx3 = T(0.5) * cf.rdz;
-    const T sucaj = T(0.5) * m.rdyp[jc];
-    const T sucac = m.rdyc[jc], qsucac = T(0.25) * sucac;
-    const T am1c = cf.rdx2, ap1c = cf.rdx2, ac1c = -(am1c + ap1c);
-    const T am3c = cf.rdz2, ap3c = cf.rdz2, ac3c = -(am3c + ap3c);
-    const T ap2c = m.ap2c[jc], ac2c = m.ac2c[jc], am2c = m.am2c[jc];
-    const T ap2p = m.ap2p[jc], ac2p = m.ac2p[jc], am2p = m.am2p[jc];
-    const T s1coe = m.s1coe[jc];
-    const int k1 = min(nz, k0 + kchunk);
-    // rolling registers: *_m at k-1, *_c at k, *_p at k+1 of this column
-    int kprev = wrapm(k0, nz);
-    long long rq = sj * jc + sk * kprev;
-    T u_m = ux[rq + i], v_m = uy[rq + i], w_m = uz[rq + i], p_m = pr[rq + i];
-    T uxp_m = ux[rq + ip];                 // ux(ip, k-1)
-    T vyp_m = uy[rq + sj + i];             // uy(jp, k-1)
-    rq = sj * jc + sk * k0;
-    T u_c = ux[rq + i], v_c = uy[rq + i], w_c = uz[rq + i], p_c = pr[rq + i];
-    T w_xm = uz[rq + im], w_ym = uz[rq - sj + i];   // uz(im, k), uz(jm, k)
+  // own column at k-1, rolled: ux, uy, uz, pr, ux(ip), uy(jp)
+  T u_m, v_m, w_m, p_m, uxp_m, vyp_m;
+  {
+    const long long rq = sk * wrapm(k0, nz) + (long long)nx * jrow;
+    u_m = ux[rq + ic]; v_m = uy[rq + ic]; w_m = uz[rq + ic]; p_m = pr[rq + ic];
+    uxp_m = ux[rq + wrapp(ic, nx)]; vyp_m = uy[rq + nx + ic];
+  }
+  fetch(k0);
+  stage(1);
+  fetch(wrapp(k0, nz));
+  stage(2);
+  __syncthreads();
+  int bm = 0, bc = 1, bp = 2;  // buffers of planes k-1 (free), k, k+1
+  const bool full = i0 + BX <= nx && j0 + WY - 1 <= ny;
+  auto run = [&](auto fullv) {
+    constexpr bool FULL = decltype(fullv)::value;
+    const bool live = FULL || (i < nx && jc <= ny);
     for (int k = k0; k < k1; ++k) {
-      const int kp = wrapp(k, nz);
-      const long long rowc = sj * jc + sk * k;
-      const long long rowm = rowc - sj, rowp = rowc + sj;
-      const long long rowkp = sj * jc + sk * kp;
-      const long long c = rowc + i;
-      // plane k+1 of this column (and uz one step left / down)
-      const T u_p = ux[rowkp + i], v_p = uy[rowkp + i], w_p = uz[rowkp + i];
-      const T w_xm_p = uz[rowkp + im], w_ym_p = uz[rowkp - sj + i];
-      const T p_p = pr[rowkp + i];
-      // plane k neighbours
-      const T u_xp = ux[rowc + ip], u_xm = ux[rowc + im], u_yp = ux[rowp + i], u_ym = ux[rowm + i];
-      const T v_xp = uy[rowc + ip], v_xm = uy[rowc + im], v_yp = uy[rowp + i], v_ym = uy[rowm + i];
-      const T w_xp = uz[rowc + ip], w_yp = uz[rowp + i];
+      const bool more = k + 2 <= k1;  // plane k+2 is read by this run
+      const long long c = sk * k + (long long)nx * jrow + ic;
+      T hxo, hyo, hzo;  // histories: written and read once per step, non-temporal
+      if (MASK & 1) hxo = C3D_LD_ONCE(&hx[c]);
+      if (MASK & 2) hyo = C3D_LD_ONCE(&hy[c]);
+      if (MASK & 4) hzo = C3D_LD_ONCE(&hz[c]);
+      if (more) fetch(k + 2 >= nz ? k + 2 - nz : k + 2);
+      const T* U0 = sb[bc][0]; const T* V0 = sb[bc][1];
+      const T* W0 = sb[bc][2]; const T* P0 = sb[bc][3];
+      const T* W1 = sb[bp][2];
+      const T u_c = U0[e], v_c = V0[e], w_c = W0[e], p_c = P0[e];
+      const T u_p = sb[bp][0][e], v_p = sb[bp][1][e], w_p = W1[e];
+      const T w_xm_p = W1[e - 1], w_ym_p = W1[e - LW];
+      const T u_xp = U0[e + 1], u_xm = U0[e - 1], u_yp = U0[e + LW], u_ym = U0[e - LW];
+      const T v_xp = V0[e + 1], v_xm = V0[e - 1], v_yp = V0[e + LW], v_ym = V0[e - LW];
+      const T w_xp = W0[e + 1], w_xm = W0[e - 1], w_yp = W0[e + LW], w_ym = W0[e - LW];
+      const T I1 = m.yint[jrow], I2 = T(1) - I1, I3 = m.yint[jp], I4 = T(1) - I3;
+      T ox, oy, oz, cx, cy, cz;  // this plane's right-hand sides and histories
       if (MASK & 1) {
+        const T sucaj = T(0.5) * m.rdyp[jrow];
+        const T ap2c = m.ap2c[jrow], ac2c = m.ac2c[jrow], am2c = m.am2c[jrow];
         const T h11 = ((u_xp + u_c) * (u_xp + u_c) - (u_c + u_xm) * (u_c + u_xm)) * qdx1;
-        const T h12 = ((v_yp + uy[rowp + im]) * (I3 * u_c + I4 * u_yp) -
+        const T h12 = ((v_yp + V0[e + LW - 1]) * (I3 * u_c + I4 * u_yp) -
                        (v_c + v_xm) * (I1 * u_ym + I2 * u_c)) *
                       sucaj;
         const T h13 = ((w_p + w_xm_p) * (u_p + u_c) - (w_c + w_xm) * (u_c + u_m)) * qdx3;
@@ -158,41 +223,37 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T d22 = ap2c * u_yp + ac2c * u_c + am2c * u_ym;
         const T d33 = ap3c * u_p + ac3c * u_c + am3c * u_m;
         const T dcq = d11 + d22 + d33;
-        const T gradp = (p_c - pr[rowc + im]) * cf.rdx;
-        fix_add(s1, (double)(u_c * s1coe));
+        const T gradp = (p_c - P0[e - 1]) * cf.rdx;
         const T conv = -h11 - h12 - h13 + cf.gx;
-        // written and read once per step: non-temporal, so they do not evict
-        // the neighbour rows the next tiles read from L2
-        const T hold = C3D_LD_ONCE(&hx[c]);
-        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
-                                        T(2) * cf.pmBeta * dcq,
-                                    &rx[c]);
-        C3D_ST_ONCE(conv, &hx[c]);
+        if (live) fix_add(s1, (double)(u_c * m.s1coe[jrow]));
+        ox = cf.pmGamma * conv + cf.pmTheta * hxo - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        cx = conv;
       }
       if (MASK & 2) {
-        const T h21 = ((I1 * ux[rowm + ip] + I2 * u_xp) * (v_xp + v_c) -
+        const T sucac = m.rdyc[jrow], qsucac = T(0.25) * sucac;
+        const T ap2p = m.ap2p[jrow], ac2p = m.ac2p[jrow], am2p = m.am2p[jrow];
+        const T h21 = ((I1 * U0[e - LW + 1] + I2 * u_xp) * (v_xp + v_c) -
                        (I1 * u_ym + I2 * u_c) * (v_c + v_xm)) *
                       hdx1;
         const T h22 = ((v_yp + v_c) * (v_yp + v_c) - (v_c + v_ym) * (v_c + v_ym)) * qsucac;
-        const T h23 = ((I1 * w_ym_p + I2 * w_p) * (v_p + v_c) - (I1 * w_ym + I2 * w_c) * (v_c + v_m)) *
-                      hdx3;
+        const T h23 =
+            ((I1 * w_ym_p + I2 * w_p) * (v_p + v_c) - (I1 * w_ym + I2 * w_c) * (v_c + v_m)) *
+            hdx3;
         const T d11 = ap1c * v_xp + ac1c * v_c + am1c * v_xm;
         const T d22 = ap2p * v_yp + ac2p * v_c + am2p * v_ym;
         const T d33 = ap3c * v_p + ac3c * v_c + am3c * v_m;
         const T dcq = d11 + d22 + d33;
-        const T gradp = (p_c - pr[rowm + i]) * sucac;
+        const T gradp = (p_c - P0[e - LW]) * sucac;
         const T conv = -h21 - h22 - h23 + cf.gy;
-        // written and read once per step: non-temporal, so they do not evict
-        // the neighbour rows the next tiles read from L2
-        const T hold = C3D_LD_ONCE(&hy[c]);
-        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
-                                        T(2) * cf.pmBeta * dcq,
-                                    &ry[c]);
-        C3D_ST_ONCE(conv, &hy[c]);
+        oy = cf.pmGamma * conv + cf.pmTheta * hyo - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        cy = conv;
       }
       if (MASK & 4) {
+        const T sucaj = T(0.5) * m.rdyp[jrow];
+        const T ap2c = m.ap2c[jrow], ac2c = m.ac2c[jrow], am2c = m.am2c[jrow];
         const T h31 = ((u_xp + uxp_m) * (w_xp + w_c) - (u_c + u_m) * (w_c + w_xm)) * qdx1;
-        const T h32 = ((v_yp + vyp_m) * (I3 * w_c + I4 * w_yp) - (v_c + v_m) * (I1 * w_ym + I2 * w_c)) *
+        const T h32 = ((v_yp + vyp_m) * (I3 * w_c + I4 * w_yp) -
+                       (v_c + v_m) * (I1 * w_ym + I2 * w_c)) *
                       sucaj;
         const T h33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * qdx3;
         const T d11 = ap1c * w_xp + ac1c * w_c + am1c * w_xm;
@@ -201,24 +262,31 @@ __global__ __launch_bounds__(BX* BYR) void k_rhs(Geom g, Metrics<T> m, Coef<T> c
         const T dcq = d11 + d22 + d33;
         const T gradp = (p_c - p_m) * cf.rdz;
         const T conv = -h31 - h32 - h33 + cf.gz;
-        // written and read once per step: non-temporal, so they do not evict
-        // the neighbour rows the next tiles read from L2
-        const T hold = C3D_LD_ONCE(&hz[c]);
-        C3D_ST_ONCE(cf.pmGamma * conv + cf.pmTheta * hold - cf.pmAlpha * gradp +
-                                        T(2) * cf.pmBeta * dcq,
-                                    &rz[c]);
-        C3D_ST_ONCE(conv, &hz[c]);
+        oz = cf.pmGamma * conv + cf.pmTheta * hzo - cf.pmAlpha * gradp + T(2) * cf.pmBeta * dcq;
+        cz = conv;
       }
-      // roll
       u_m = u_c; v_m = v_c; w_m = w_c; p_m = p_c;
-      u_c = u_p; v_c = v_p; w_c = w_p; p_c = p_p;
       uxp_m = u_xp; vyp_m = v_yp;
-      w_xm = w_xm_p; w_ym = w_ym_p;
+      if (more) stage(bm);
+      if (live) {
+        // written once per step: non-temporal, so they do not evict the
+        // neighbour rows the next tiles read from L2
+        if (MASK & 1) { C3D_ST_ONCE(ox, &rx[c]); C3D_ST_ONCE(cx, &hx[c]); }
+        if (MASK & 2) { C3D_ST_ONCE(oy, &ry[c]); C3D_ST_ONCE(cy, &hy[c]); }
+        if (MASK & 4) { C3D_ST_ONCE(oz, &rz[c]); C3D_ST_ONCE(cz, &hz[c]); }
+      }
+      __syncthreads();
+      const int t = bm;
+      bm = bc; bc = bp; bp = t;
     }
-  }
+  };
+  if (full)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
   if (MASK & 1) {
     block_sum_fix(s1, shf);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
+    if (tid == 0) {
       long long* pf = (long long*)partials + 4LL * blockIdx.x;
       for (int q = 0; q < 4; ++q) pf[q] = s1[q];
     }
@@ -548,17 +616,10 @@ const Metrics<double>& mets<double>(c3d_ctx* c) { return c->md; }
 template <>
 const Metrics<float>& mets<float>(c3d_ctx* c) { return c->mf; }
 
-// rows of j per RHS workgroup.  The y-neighbour rows of a tile are re-read
-// by the tiles above and below, so taller tiles read less twice; measured at
-// 768x384x576 FP64, 4 rows (4.60 ms) beat 8 (4.75 ms) and 16, which lose
-// occupancy, and a 3-workgroups-per-CU register cap (4.78 ms) did not pay.
-static constexpr int RHS_ROWS = 4;
-static int rhs_rows() { return RHS_ROWS; }
-
 // z run of the RHS kernel: long enough to amortise the two lead-in planes,
 // short enough to give the device a few thousand workgroups
 static int rhs_kchunk(const Geom& g) {
-  const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + rhs_rows() - 1) / rhs_rows());
+  const int tiles = ((g.nx + BX - 1) / BX) * ((g.ny + RHS_WY - 1) / RHS_WY);
   int nch = (4096 + tiles - 1) / tiles;
   if (nch < 1) nch = 1;
   int kc = (g.nz + nch - 1) / nch;
@@ -567,7 +628,7 @@ static int rhs_kchunk(const Geom& g) {
 }
 static int rhs_nblocks(const Geom& g) {
   const int kc = rhs_kchunk(g);
-  return ((g.nx + BX - 1) / BX) * ((g.ny + rhs_rows() - 1) / rhs_rows()) * ((g.nz + kc - 1) / kc);
+  return ((g.nx + BX - 1) / BX) * ((g.ny + RHS_WY - 1) / RHS_WY) * ((g.nz + kc - 1) / kc);
 }
 
 template <typename T>
@@ -575,30 +636,27 @@ static int rhs_t(c3d_ctx* c, int mask, const void* ux, const void* uy, const voi
                  const void* pr, void* rx, void* ry, void* rz, void* hx, void* hy, void* hz) {
   const Geom& g = c->g;
   const int kc = rhs_kchunk(g);
-  const int rows = rhs_rows();
-  dim3 grid(rhs_nblocks(g)), blk(BX, rows);
+  dim3 grid(rhs_nblocks(g)), blk(BX, RHS_WY);
   Coef<T> cf = make_coef<T>(c);
   const Metrics<T>& m = mets<T>(c);
   auto U = [](const void* p) { return (const T*)p; };
   auto W = [](void* p) { return (T*)p; };
-#define RHS_LAUNCH(M, R)                                                                       \
+#define RHS_LAUNCH(M)                                                                          \
   if (g.nt)                                                                                    \
-    hipLaunchKernelGGL((k_rhs<T, M, R, true>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy), \
+    hipLaunchKernelGGL((k_rhs<T, M, true>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy),    \
                        U(uz), U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials,  \
                        kc);                                                                    \
   else                                                                                         \
-    hipLaunchKernelGGL((k_rhs<T, M, R, false>), grid, blk, 0, c->stream, g, m, cf, U(ux),       \
-                       U(uy), U(uz), U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz),          \
-                       c->d_partials, kc)
-#define RHS_ROWS_LAUNCH(M) RHS_LAUNCH(M, RHS_ROWS)
+    hipLaunchKernelGGL((k_rhs<T, M, false>), grid, blk, 0, c->stream, g, m, cf, U(ux), U(uy),   \
+                       U(uz), U(pr), W(rx), W(ry), W(rz), W(hx), W(hy), W(hz), c->d_partials,  \
+                       kc)
   switch (mask) {
-    case 1: RHS_ROWS_LAUNCH(1); break;
-    case 2: RHS_ROWS_LAUNCH(2); break;
-    case 4: RHS_ROWS_LAUNCH(4); break;
-    case 7: RHS_ROWS_LAUNCH(7); break;
+    case 1: RHS_LAUNCH(1); break;
+    case 2: RHS_LAUNCH(2); break;
+    case 4: RHS_LAUNCH(4); break;
+    case 7: RHS_LAUNCH(7); break;
     default: set_error("rhs: bad component mask"); return C3D_ERR_ARG;
   }
-#undef RHS_ROWS_LAUNCH
 #undef RHS_LAUNCH
   C3D_HIP(hipGetLastError());
   return C3D_OK;

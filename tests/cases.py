@@ -15,6 +15,9 @@ SMALL = dict(nxc=32, nyc=24, nzc=40, xlx=4 * math.pi, yly=2.0, zlz=2 * math.pi,
              xnu=2.0 / 5600.0, dt=0.005, gamma=1.5, ubulk=1.0)
 ODD = dict(nxc=48, nyc=37, nzc=36, xlx=3.0, yly=2.0, zlz=1.7, xnu=1.0 / 1000.0,
            dt=0.004, gamma=1.2, ubulk=1.0)
+# full and partial stencil tiles side by side (x: 64 + 32, y: rows of 8 and 4)
+TILES = dict(nxc=96, nyc=20, nzc=20, xlx=3.0, yly=2.0, zlz=1.5, xnu=1.0 / 1200.0, dt=0.004,
+             gamma=1.5, ubulk=1.0)
 # large implicit factor: beta*rdx^2 of order one, exercises the full-circulant path
 STIFF = dict(nxc=16, nyc=20, nzc=20, xlx=1.0, yly=2.0, zlz=1.0, xnu=0.05, dt=0.05,
              gamma=1.5, ubulk=1.0)

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 CASES = {"small": cases.SMALL, "odd": cases.ODD, "stiff": cases.STIFF,
-         "freeslip": cases.FREESLIP, "c1": cases.C1}
+         "freeslip": cases.FREESLIP, "c1": cases.C1, "tiles": cases.TILES}
 
 
 def relerr(a, b):
@@ -100,7 +100,7 @@ def test_metrics_identical(O, name):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["small", "odd", "freeslip"])
+@pytest.mark.parametrize("name", ["small", "odd", "freeslip", "tiles", "c1"])
 @pytest.mark.parametrize("ns", [1, 2, 3])
 def test_rhs_components(O, name, ns):
     case = CASES[name]
