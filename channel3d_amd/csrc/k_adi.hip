@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 // Batches: blockIdx.y selects the field, bf reals apart in r and u.
 constexpr int YNT = 256;
 template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
-          typename TU = T>
+          typename TU = T, bool FY = false>
 __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
                                                   const double* __restrict__ tvecm,
                                                   const double* __restrict__ tarr,
@@ -320,25 +320,35 @@ __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restri
   const long long sj = g.nx;
   const T shift = SHIFT ? (T)sc->dp1ns : T(0);
   const T wallv = WALL == 1 ? duy[i + (long long)g.nx * k] : T(0);
+  // The per-row guards (rows past ny) are needed in the top segment only; a
+  // guarded load or store is a branch of its own, and across such branches
+  // the compiler waits for every outstanding load (the prefetched segment
+  // included), so the other segments run unguarded (G false).  With FY (ny a
+  // multiple of C) the top segment is whole too.
+  using Gd = std::integral_constant<bool, !FY>;
+  using Un = std::false_type;
   // rows s*C+1 .. s*C+C of the right-hand side (0 past the last row)
-  auto load = [&](T* y, int s) {
+  auto load = [&](T* y, int s, auto gv) {
+    constexpr bool G = decltype(gv)::value;
     const int lo = s * C + 1;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      y[q] = j <= ny ? (SHIFT ? r[base + sj * j] - shift : r[base + sj * j]) : T(0);
+      y[q] = (!G || j <= ny) ? (SHIFT ? r[base + sj * j] - shift : r[base + sj * j]) : T(0);
     }
   };
-  auto loadu = [&](TU* uv, int s) {
+  auto loadu = [&](TU* uv, int s, auto gv) {
+    constexpr bool G = decltype(gv)::value;
     const int lo = s * C + 1;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      uv[q] = (!SET && j <= ny) ? C3D_LD_ONCE(&u[base + sj * j]) : TU(0);  // once
+      uv[q] = (!SET && (!G || j <= ny)) ? C3D_LD_ONCE(&u[base + sj * j]) : TU(0);  // once
     }
   };
   // forward recurrence of segment s from f (the value at the row below it)
-  auto fwd = [&](T* y, int s, T& f) {
+  auto fwd = [&](T* y, int s, T& f, auto gv) {
+    constexpr bool G = decltype(gv)::value;
     const int lo = s * C + 1;
     if (s == 0) {
       if (WALLROW) y[0] = wallv;
@@ -348,49 +358,82 @@ __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restri
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int j = lo + q;
-      if ((s > 0 || q > 0) && j <= ny) {
+      if ((s > 0 || q > 0) && (!G || j <= ny)) {
         f = (y[q] - (T)ta[j] * f) / (T)tvecm[j];
         y[q] = f;
       }
     }
   };
-  T ya[C], yb[C];
-  TU ua[C];
-  T f = T(0);
-  load(ya, 0);
-  for (int s = 0; s < nseg; s += 2) {
-    if (s + 1 < nseg) load(yb, s + 1);
-    fwd(ya, s, f);
-    if (s + 1 < nseg) {
-      ck[s * YNT + tid] = f;  // forward value at row (s+1)*C
-      if (s + 2 < nseg) load(ya, s + 2);
-      fwd(yb, s + 1, f);
-      if (s + 2 < nseg) ck[(s + 1) * YNT + tid] = f;
-    }
-  }
-  // the top segment's forward values into ya
-  if ((nseg - 1) & 1) {
-#pragma unroll
-    for (int q = 0; q < C; ++q) ya[q] = yb[q];
-  }
-  T x = f;
-  loadu(ua, nseg - 1);
-  for (int s = nseg - 1; s >= 0; --s) {
+  // substitution through segment s (forward values in y, u in uv)
+  T x = T(0);
+  auto back = [&](const T* y, const TU* uv, int s, auto gv) {
+    constexpr bool G = decltype(gv)::value;
     const int lo = s * C + 1;
-    if (s > 0) load(yb, s - 1);  // the segment below: a full one
 #pragma unroll
     for (int q = C - 1; q >= 0; --q) {
       const int j = lo + q;
-      if (j < ny) x = ya[q] - (T)tarr[j + 1] * x;
-      if (j <= ny) C3D_ST_ONCE(SET ? (TU)x : ua[q] + (TU)x, &u[base + sj * j]);
+      if (j < ny) x = y[q] - (T)tarr[j + 1] * x;
+      if (!G || j <= ny) C3D_ST_ONCE(SET ? (TU)x : uv[q] + (TU)x, &u[base + sj * j]);
     }
-    if (s > 0) {
-      loadu(ua, s - 1);
-      T fp = s > 1 ? ck[(s - 2) * YNT + tid] : T(0);
-      fwd(yb, s - 1, fp);
+  };
+  T ya[C], yb[C];
+  TU ua[C];
+  T f = T(0);
+  const int top = nseg - 1;
+  // forward: segments below the top in pairs, prefetching unconditionally
+  int s = 0;
+  if (top == 0) {
+    load(ya, 0, Gd{});
+    fwd(ya, 0, f, Gd{});
+  } else {
+    load(ya, 0, Un{});
+    for (; s + 2 < top; s += 2) {
+      load(yb, s + 1, Un{});
+      fwd(ya, s, f, Un{});
+      ck[s * YNT + tid] = f;  // forward value at row (s+1)*C
+      load(ya, s + 2, Un{});
+      fwd(yb, s + 1, f, Un{});
+      ck[(s + 1) * YNT + tid] = f;
+    }
+    // one or two segments left below the top, then the top one
+    if (s + 1 < top) {
+      load(yb, s + 1, Un{});
+      fwd(ya, s, f, Un{});
+      ck[s * YNT + tid] = f;
+      load(ya, top, Gd{});
+      fwd(yb, s + 1, f, Un{});
+      ck[(s + 1) * YNT + tid] = f;
+      fwd(ya, top, f, Gd{});
+    } else {
+      load(yb, top, Gd{});
+      fwd(ya, s, f, Un{});
+      ck[s * YNT + tid] = f;
+      fwd(yb, top, f, Gd{});
 #pragma unroll
       for (int q = 0; q < C; ++q) ya[q] = yb[q];
     }
+  }
+  // backward: the top segment's forward values are in ya
+  x = f;
+  loadu(ua, top, Gd{});
+  if (top > 0) {
+    load(yb, top - 1, Un{});
+    back(ya, ua, top, Gd{});
+    loadu(ua, top - 1, Un{});
+    T fp = top > 1 ? ck[(top - 2) * YNT + tid] : T(0);
+    fwd(yb, top - 1, fp, Un{});
+    for (s = top - 1; s > 0; --s) {
+      load(ya, s - 1, Un{});  // the segment below
+      back(yb, ua, s, Un{});
+      loadu(ua, s - 1, Un{});
+      fp = s > 1 ? ck[(s - 2) * YNT + tid] : T(0);
+      fwd(ya, s - 1, fp, Un{});
+#pragma unroll
+      for (int q = 0; q < C; ++q) yb[q] = ya[q];
+    }
+    back(yb, ua, 0, Un{});
+  } else {
+    back(ya, ua, 0, Gd{});
   }
 }
 
@@ -406,20 +449,30 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
   const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * YNT * sizeof(T);
   if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
     const dim3 gy((unsigned)(((long long)g.nx * g.nz + YNT - 1) / YNT)), by(YNT);
-    auto launch = [&](auto ntc) {
-      constexpr bool NT_ = decltype(ntc)::value;
+    auto launch = [&](auto ntc, auto fyc) {
+      constexpr bool NT_ = decltype(ntc)::value, FY = decltype(fyc)::value;
       if (comp == 1)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 1, false, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)duy_ym, sc, 0LL);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 1, false, NT_, false, T, FY>), gy, by, ck_bytes,
+                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+                           (const T*)duy_ym, sc, 0LL);
       else if (shift)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, true, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc, 0LL);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, true, NT_, false, T, FY>), gy, by, ck_bytes,
+                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+                           (const T*)nullptr, sc, 0LL);
       else
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, false, NT_>), gy, by, ck_bytes, c->stream, g,
-                           yt[0], yt[1], yt[2], (const T*)rhs, (T*)u, (const T*)nullptr, sc, 0LL);
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, false, NT_, false, T, FY>), gy, by, ck_bytes,
+                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+                           (const T*)nullptr, sc, 0LL);
     };
-    if (g.nt) launch(std::true_type{});
-    else launch(std::false_type{});
+    // whole segments only (ny a multiple of the segment): the guard-free sweep
+    const bool fy = g.ny % CSEG == 0;
+    if (g.nt) {
+      if (fy) launch(std::true_type{}, std::true_type{});
+      else launch(std::true_type{}, std::false_type{});
+    } else {
+      if (fy) launch(std::false_type{}, std::true_type{});
+      else launch(std::false_type{}, std::false_type{});
+    }
     C3D_HIP(hipGetLastError());
     return C3D_OK;
   }
