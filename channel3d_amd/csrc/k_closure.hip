@@ -600,10 +600,14 @@ __global__ __launch_bounds__(1024) void k_cl_reduce(const float* __restrict__ pa
     for (int q = 0; q < 16; ++q) s += sh[q][pl];
     grad[p] = (float)s;
   }
-  if (blockIdx.x == 0 && t == 0 && loss) {
+  // the loss: the second wavefront of block 0 (free after the partial
+  // sums) adds the workgroup losses lane-strided, then across the lanes in a
+  // fixed shuffle tree -- deterministic, and no serial chain of nwg loads
+  if (blockIdx.x == 0 && bg == 1 && loss) {
     double s2 = 0.0;
-    for (int b = 0; b < nwg; ++b) s2 += lpart[b];
-    *loss = (float)(s2 / (ntot * NOUT));
+    for (int b = pl; b < nwg; b += 64) s2 += lpart[b];
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_down(s2, o, 64);
+    if (pl == 0) *loss = (float)(s2 / (ntot * NOUT));
   }
 }
 
@@ -684,10 +688,14 @@ __global__ __launch_bounds__(1024) void k_cl_reduce_adam(const float* __restrict
       }
     }
   }
-  if (blockIdx.x == 0 && t == 0 && loss) {
+  // the loss: the second wavefront of block 0 (free after the partial
+  // sums) adds the workgroup losses lane-strided, then across the lanes in a
+  // fixed shuffle tree -- deterministic, and no serial chain of nwg loads
+  if (blockIdx.x == 0 && bg == 1 && loss) {
     double s2 = 0.0;
-    for (int b = 0; b < nwg; ++b) s2 += lpart[b];
-    *loss = (float)(s2 / (ntot * NOUT));
+    for (int b = pl; b < nwg; b += 64) s2 += lpart[b];
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_down(s2, o, 64);
+    if (pl == 0) *loss = (float)(s2 / (ntot * NOUT));
   }
 }
 
@@ -969,13 +977,15 @@ int cl_range(c3d_closure* cl, long long first, long long count, const char* what
   return C3D_OK;
 }
 
-// workgroups of a gradient evaluation over count samples: two 16-sample
-// tiles per wavefront while that fills GRAD_WG_MAX workgroups
+// workgroups of a gradient evaluation over count samples (16-sample tiles),
+// at most GRAD_WG_MAX
 int grad_wgs(long long count) {
   const long long ntile = (count + 15) / 16;
-  // two tiles per wavefront up to 1024 tiles, then more per wavefront (fewer
-  // partials for the reduction): measured best at 16384 and 65536 samples
-  const int tpw = (int)std::max<long long>(2, ntile / 1024);
+  // as many tiles per wavefront as give one workgroup per CU (256 CUs x 4
+  // wavefronts = 1024 tiles a round; the kernel runs one wavefront per SIMD):
+  // measured best at 16384 (1 tile, 489 M samples/s against 476 M with 2)
+  // and 65536 samples (4 tiles, 1.27 G against 0.93 G with 2 or 8)
+  const int tpw = (int)std::max<long long>(1, ntile / 1024);
   return (int)std::max<long long>(1, std::min<long long>(GRAD_WG_MAX, (ntile + tpw * GWAVES - 1) /
                                                                           (tpw * GWAVES)));
 }
