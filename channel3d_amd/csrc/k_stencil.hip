@@ -456,7 +456,16 @@ __global__ void k_update_uy_ym(int n, T uybc, T* __restrict__ uy_ym, T* __restri
 // Projection: FluidVelUpdate (m_TScheme.f90:281-306) and PressureUpdate_FIMP
 // (m_TSchemeFIMP_inc.f90:1371-1386).  With ghost_from_interior the Neumann
 // ghost phi(0)=phi(1) of SetBC_and_UpdateHalo_pr is applied on the fly.
+//
+// A thread walks a run of PKCH z planes of its (i, j) column.  Plane k+1's
+// words are requested before plane k's results are stored (the hardware
+// counts loads and stores on one counter, so a load issued after a store
+// cannot be waited for without also waiting for the store), and every load
+// is unconditional (the ghost row is read and then selected).  phi one plane
+// down is the previous plane's own value.
 // ---------------------------------------------------------------------------
+constexpr int PKCH = 16;
+
 template <typename T, bool VEL, bool PRS, bool GHOST, bool NTS = false>
 __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2, T rdzEta2,
                                                  T alpha, const T* __restrict__ phi,
@@ -468,25 +477,58 @@ __global__ __launch_bounds__(BX* BY) void k_proj(Geom g, Metrics<T> m, T rdxEta2
   if (i >= g.nx || jc > g.ny) return;
   const int im = wrapm(i, g.nx);
   const T sucacEta2 = m.rdyc[jc] * alpha;
-  const int k1 = min(g.nz, (cbz + 1) * KCH);
-  for (int k = cbz * KCH; k < k1; ++k) {
-    const int km = wrapm(k, g.nz);
-    const long long rowc = (long long)g.nx * jc + g.plane * k;
-    const long long c = rowc + i;
-    const T locphi = phi[c];
+  const int k0 = cbz * PKCH, k1 = min(g.nz, k0 + PKCH);
+  const long long row = (long long)g.nx * jc;
+  struct Plane {
+    T ph, phx, phy, u, v, w, p;
+  };
+  auto fetch = [&](int k, Plane& a) {
+    const long long c = row + g.plane * k + i;
+    a.ph = phi[c];
     if (VEL) {
-      const T phx = phi[rowc + im];
-      const T phy = (GHOST && jc == 1) ? locphi : phi[c - g.nx];
-      const T phz = phi[(long long)g.nx * jc + g.plane * km + i];
+      a.phx = phi[c - i + im];
+      a.phy = phi[c - g.nx];  // the ghost row when jc == 1
       // u, v, w and p are read and written once: non-temporal
-      C3D_ST_ONCE(C3D_LD_ONCE(&ux[c]) - (locphi - phx) * rdxEta2,
-                                  &ux[c]);
-      C3D_ST_ONCE(C3D_LD_ONCE(&uy[c]) - (locphi - phy) * sucacEta2,
-                                  &uy[c]);
-      C3D_ST_ONCE(C3D_LD_ONCE(&uz[c]) - (locphi - phz) * rdzEta2,
-                                  &uz[c]);
+      a.u = C3D_LD_ONCE(&ux[c]);
+      a.v = C3D_LD_ONCE(&uy[c]);
+      a.w = C3D_LD_ONCE(&uz[c]);
     }
-    if (PRS) C3D_ST_ONCE(C3D_LD_ONCE(&pr[c]) + locphi, &pr[c]);
+    if (PRS) a.p = C3D_LD_ONCE(&pr[c]);
+  };
+  auto update = [&](int k, const Plane& a, T phz) {
+    // keep the scheduler from moving this plane's stores above the loads of
+    // the next plane issued just before
+    __builtin_amdgcn_sched_barrier(0);
+    const long long c = row + g.plane * k + i;
+    const T locphi = a.ph;
+    if (VEL) {
+      const T phy = (GHOST && jc == 1) ? locphi : a.phy;
+      C3D_ST_ONCE(a.u - (locphi - a.phx) * rdxEta2, &ux[c]);
+      C3D_ST_ONCE(a.v - (locphi - phy) * sucacEta2, &uy[c]);
+      C3D_ST_ONCE(a.w - (locphi - phz) * rdzEta2, &uz[c]);
+    }
+    if (PRS) C3D_ST_ONCE(a.p + locphi, &pr[c]);
+  };
+  T phz = VEL ? phi[row + g.plane * wrapm(k0, g.nz) + i] : T(0);  // phi one plane down
+  // two plane buffers in alternating roles (a copy between them would wait
+  // for the loads, and with them for the stores issued since)
+  Plane a, b;
+  fetch(k0, a);
+  int k = k0;
+  for (; k + 2 < k1; k += 2) {
+    fetch(k + 1, b);
+    update(k, a, phz);
+    phz = a.ph;
+    fetch(k + 2, a);
+    update(k + 1, b, phz);
+    phz = b.ph;
+  }
+  if (k + 1 < k1) {  // a holds plane k; one more plane after it
+    fetch(k + 1, b);
+    update(k, a, phz);
+    update(k + 1, b, a.ph);
+  } else {
+    update(k, a, phz);
   }
 }
 
@@ -803,14 +845,15 @@ static int proj_t(c3d_ctx* c, const void* phi, void* ux, void* uy, void* uz, voi
   const Geom& g = c->g;
   const T a = (T)c->pmAlpha;
   const T rdxE = (T)(c->rdx * c->pmAlpha), rdzE = (T)(c->rdz * c->pmAlpha);
-#define PROJ_LAUNCH(V, P, G)                                                                     \
-  if (g.nt)                                                                                      \
-    hipLaunchKernelGGL((k_proj<T, V, P, G, true>), cell_grid(g), dim3(BX, BY), 0, c->stream, g, \
-                       mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,        \
-                       (T*)pr);                                                                  \
-  else                                                                                           \
-    hipLaunchKernelGGL((k_proj<T, V, P, G, false>), cell_grid(g), dim3(BX, BY), 0, c->stream,   \
-                       g, mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,     \
+  const dim3 grid(ncell_blocks(g, PKCH));
+#define PROJ_LAUNCH(V, P, G)                                                                   \
+  if (g.nt)                                                                                    \
+    hipLaunchKernelGGL((k_proj<T, V, P, G, true>), grid, dim3(BX, BY), 0, c->stream, g,        \
+                       mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,      \
+                       (T*)pr);                                                                \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_proj<T, V, P, G, false>), grid, dim3(BX, BY), 0, c->stream, g,       \
+                       mets<T>(c), rdxE, rdzE, a, (const T*)phi, (T*)ux, (T*)uy, (T*)uz,      \
                        (T*)pr)
   if (vel && prs && ghost) PROJ_LAUNCH(true, true, true);
   else if (vel && prs) PROJ_LAUNCH(true, true, false);
