@@ -26,11 +26,6 @@ constexpr int LBX = 64, LBY = 4, LKCH = 8;
 __device__ __forceinline__ int wm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ __forceinline__ int wp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
 
-template <typename T>
-__device__ __forceinline__ T pw(T x, T e) { return pow(x, e); }
-template <>
-__device__ __forceinline__ float pw<float>(float x, float e) { return powf(x, e); }
-
 // 1-D grid of 64 x 4 (i, j) tiles times runs of LKCH z planes
 #define LES_CELL()                                                            \
   const int cgx_ = (g.nx + LBX - 1) / LBX, cgy_ = (g.ny + LBY - 1) / LBY;     \
@@ -158,8 +153,12 @@ __global__ __launch_bounds__(LBX* LBY) void k_sgs_wale(Geom g, Metrics<T> m, Les
     Sd33 = Sd33 - SdTrace;
     const T SdijSdij =
         Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + T(2) * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23);
-    const T v = CwDelta2 * pw<T>(SdijSdij, T(1.5)) /
-                    (pw<T>(SijSij, T(2.5)) + pw<T>(SdijSdij, T(1.25))) + q.xnu;
+    // the reference's powers (**1.5, **2.5, **1.25) of these non-negative
+    // sums through square roots: x^1.5 = x sqrt(x), x^2.5 = x^2 sqrt(x),
+    // x^1.25 = x sqrt(sqrt(x)), a few ulp from pow at a fraction of its cost
+    const T sdh = sqrt(SdijSdij), sh = sqrt(SijSij);
+    const T v = CwDelta2 * (SdijSdij * sdh) /
+                    (SijSij * SijSij * sh + SdijSdij * sqrt(sdh)) + q.xnu;
     A(nut, i, jc, k) = v;
     nut_ghosts(g, q, nut, i, jc, k, v);
   }
