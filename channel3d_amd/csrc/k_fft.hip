@@ -320,6 +320,12 @@ struct DivArgs {
   double* partials;
 };
 
+// calls of the first pass's input accessor per thread (its butterflies per
+// thread times the radix)
+__host__ __device__ constexpr int first_pass_loads(int N, int lines) {
+  return ((lines * (N / pick_radix(N)) + FNT - 1) / FNT) * pick_radix(N);
+}
+
 template <typename T, int N, bool PCORR, bool NTS = false>
 __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
                                                        typename CT<T>::t* __restrict__ spec,
@@ -352,10 +358,22 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
   }
   __syncthreads();
   double dm = 0.0;
+  // the p corrections are kept in registers and stored after the transform:
+  // a store between the accessor calls would make every later call's loads
+  // wait for it (loads and stores share one completion counter), so a
+  // thread's loads would go out one element at a time.  The first pass calls
+  // the accessor from fully unrolled loops, so the slot index is a constant.
+  constexpr int NLD = PCORR ? first_pass_loads(N, LINES) : 1;
+  T pnew[NLD][2];
+  long long pat[NLD];
+  int nld = 0;
   auto ld = [&](int l, int m) -> T2 {
 #pragma clang fp contract(off)
     const long long rowc = rb[l];
-    if (rowc < 0) return mk<T2>(0, 0);
+    if (rowc < 0) {
+      if (PCORR) pat[nld++] = -1;  // no row: nothing to store
+      return mk<T2>(0, 0);
+    }
     const long long rowkp = rkp[l];
     const T sucaj = rdy[l];
     T val[2];
@@ -367,12 +385,21 @@ __global__ __launch_bounds__(FNT) void k_fft_x_fwd_div(Geom g, DivArgs<T> d,
                      (d.uz[rowkp + i] - d.uz[c]) * d.rdz;
       dm = fmax(dm, (double)fabs(rdiv));
       val[h] = d.sudtal * rdiv;
-      if (PCORR) d.pr[c] = d.pr[c] + d.xnuhm * rdiv;
+      if (PCORR) pnew[nld][h] = d.pr[c] + d.xnuhm * rdiv;
     }
+    if (PCORR) pat[nld++] = rowc + 2 * m;
     return mk<T2>(val[0], val[1]);
   };
   auto st = [&](int l, int m, const T2& v) { sm[l * LDN + m] = v; };
   stockham_io<T2, N, LINES, LDN, FNT, -1, false, true, 1>(sm, tw, ld, st);
+  if (PCORR) {
+#pragma unroll
+    for (int q = 0; q < NLD; ++q)
+      if (q < nld && pat[q] >= 0) {
+        d.pr[pat[q]] = pnew[q][0];
+        d.pr[pat[q] + 1] = pnew[q][1];
+      }
+  }
   // block max of |div| (the transform's barriers order the wmax writes)
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
   if ((tid & 63) == 0) wmax[tid >> 6] = dm;
