@@ -436,10 +436,13 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
   __syncthreads();
   // Z_k = (X_k + conj X_{N-k}) + i conj(w_k) (X_k - conj X_{N-k}), the
   // complex sequence whose inverse transform interleaves the real row
+  // A row past the last one reads the last row and yields zero: no branch
+  // surrounds the loads, so the compiler need not wait for one element's
+  // loads before it issues the next element's.
   auto ld = [&](int l, int k) -> T2 {
     const int row = row0 + l;
-    if (row >= nrows) return mk<T2>(0, 0);
-    const T2* sp = spec + (long long)g.P * row;
+    const bool live = row < nrows;
+    const T2* sp = spec + (long long)g.P * (live ? row : nrows - 1);
     T2 xk = sp[k], xn = sp[N - k];
     if (k == 0) {
       xk.y = 0;
@@ -450,7 +453,7 @@ __global__ __launch_bounds__(FNT) void k_fft_x_inv(Geom g, const typename CT<T>:
     T2 w = twr[k];
     w.y = -w.y;
     const T2 wb = cmul(w, b);
-    return mk<T2>(a.x - wb.y, a.y + wb.x);
+    return live ? mk<T2>(a.x - wb.y, a.y + wb.x) : mk<T2>(0, 0);
   };
   auto st = [&](int l, int m, const T2& z) {
     const long long b = rb[l];
