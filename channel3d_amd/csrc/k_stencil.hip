@@ -554,12 +554,16 @@ __global__ __launch_bounds__(BX* BY) void k_diag(Geom g, Metrics<T> m, T rdx, T 
     const int ip = wrapp(i, g.nx);
     const T rdyp = m.rdyp[jc];
     const double dy = dyp[jc];
-    const int k1 = min(g.nz, (cbz + 1) * KCHD);
-    for (int k = cbz * KCHD; k < k1; ++k) {
+    const int k0 = cbz * KCHD, k1 = min(g.nz, (cbz + 1) * KCHD);
+    // uz one plane up is the next plane's own value; unrolled so that
+    // several planes' loads are in flight together
+    T w = uz[(long long)g.nx * jc + g.plane * k0 + i];
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
       const int kp = wrapp(k, g.nz);
       const long long rowc = (long long)g.nx * jc + g.plane * k;
       const long long c = rowc + i;
-      const T u = ux[c], v = uy[c], w = uz[c];
+      const T u = ux[c], v = uy[c];
       const T uxp = ux[rowc + ip], vyp = uy[c + g.nx];
       const T wzp = uz[(long long)g.nx * jc + g.plane * kp + i];
       if (WHAT & 1) {
@@ -576,6 +580,7 @@ __global__ __launch_bounds__(BX* BY) void k_diag(Geom g, Metrics<T> m, T rdx, T 
         vz = fmax(vz, (double)fabs(w));
       }
       if (WHAT & 8) fix_add(us, (double)u * dy);
+      w = wzp;
     }
   }
   const int b = blockIdx.x;
