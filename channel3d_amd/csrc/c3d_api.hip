@@ -781,6 +781,14 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
         return fail(C3D_ERR_ARG);
       }
     }
+    if (const char* tl = std::getenv("C3D_TRIDIAG_LANES")) {
+      const std::string v(tl);
+      if (v == "1" || v == "2") c->tri_lanes = v[0] - '0';
+      else {
+        set_error("c3d_create: C3D_TRIDIAG_LANES must be 1 or 2");
+        return fail(C3D_ERR_ARG);
+      }
+    }
     if ((rc = build_adi_tables(c))) return fail(rc);
   }
   rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);

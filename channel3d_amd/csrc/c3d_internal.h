@@ -220,6 +220,9 @@ struct c3d_ctx {
   void* d_tabs = nullptr;
   // y solver (YS_*): checkpointed sweeps, or the sweeps storing the forward pass
   int ymode = 0;
+  // lanes per wavenumber pair of the checkpointed PPE sweep: 0 by the line
+  // count, 1 or 2 forced (tests of both forms)
+  int tri_lanes = 0;
 
   // state and scratch
   void* state[C3D_NSTATE] = {};

@@ -11,6 +11,8 @@
 // own consecutive kx, so every access of a sweep is coalesced.
 #include "c3d_internal.h"
 
+#include <type_traits>
+
 namespace c3d {
 
 template <typename T>
@@ -234,6 +236,153 @@ __device__ __forceinline__ void k_tridiag_ck_body(const Geom& g, const Metrics<T
   }
 }
 
+// The same elimination with the real and imaginary parts of a wavenumber
+// pair on two neighbouring lanes (lane 2n the real part, 2n+1 the
+// imaginary part), both computing the shared pivots -- the same operations,
+// so the same values -- and with each segment's right-hand side loaded one
+// segment ahead.  Twice the lanes of k_tridiag_ck_body for the same lines:
+// for grids whose line count fills only a fraction of the device (at
+// 192x160 or 96x72 a lane per pair leaves most SIMDs without a wavefront).
+// Only the top segment has rows past ny, so only its loads and stores carry
+// the row guard (a guarded access is a branch of its own, and the compiler
+// then waits for every outstanding load, the prefetched segment included).
+template <typename T, int C, bool NTS = false>
+__device__ __forceinline__ void k_tridiag_ck2_body(const Geom& g, const Metrics<T>& m,
+                                                   typename CT2<T>::t* __restrict__ spec,
+                                                   T* __restrict__ ckp) {
+  const int part = threadIdx.x & 1;  // 0: real, 1: imaginary part
+  const int kx = blockIdx.x * 32 + (threadIdx.x >> 1);
+  const int kz = blockIdx.y * 4 + threadIdx.y;
+  if (kx > g.nk || kz >= g.nz) return;
+  const int ny = g.ny;
+  const int nseg = (ny + C - 1) / C, top = nseg - 1;
+  const long long P = g.P;
+  // this lane's component of row j: sp[P2 * j]
+  T* sp = (T*)spec + 2 * (kx + P * (long long)ny * kz) + part;
+  const long long P2 = 2 * P;
+  const long long cs = P * (long long)g.nz;  // checkpoint plane stride
+  T* ck = ckp + kx + P * (long long)kz;
+  const T lx = m.wavex[kx], lz = m.wavez[kz];
+  const bool pin = (kx == 0 && kz == 0);
+  using Gd = std::true_type;
+  using Un = std::false_type;
+  auto load = [&](T* y, int s, auto gv) {
+    constexpr bool G = decltype(gv)::value;
+    const int lo = s * C;
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (!G || lo + q < ny) y[q] = sp[P2 * (lo + q)];
+  };
+  T ar[C];
+  T vecm = T(1), cprev = T(0), f = T(0);
+  // one segment's forward recurrence from (f, vecm, cprev); rows lo..lo+C-1
+  auto run = [&](T* y, int lo, auto gv) {
+    constexpr bool G = decltype(gv)::value;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int j0 = lo + q;
+      if (!G || j0 < ny) {
+        if (j0 == 0) {
+          vecm = pin ? T(1) : (m.ac2Pr[1] + lx) + lz;
+          cprev = pin ? T(0) : m.ap2Pr[1];
+          const T fj = pin ? T(0) : y[q];
+          f = fj / vecm;
+          ar[q] = T(0);
+        } else {
+          const int jr = j0 + 1;
+          const T arr = cprev / vecm;
+          const T a = m.am2Pr[jr];
+          vecm = ((m.ac2Pr[jr] + lx) + lz) - a * arr;
+          f = (y[q] - a * f) / vecm;
+          cprev = m.ap2Pr[jr];
+          ar[q] = arr;
+        }
+        y[q] = f;
+      }
+    }
+  };
+  auto save = [&](int s) {  // checkpoint at the end of segment s
+    ck[cs * (3 * s + part)] = f;
+    ck[cs * (3 * s + 2)] = vecm;  // both lanes hold the same pivot
+  };
+  T x = T(0), arnext = T(0);
+  // backward substitution through segment s (forward values in y)
+  auto back = [&](const T* y, int s, auto gv) {
+    constexpr bool G = decltype(gv)::value;
+    const int lo = s * C;
+#pragma unroll
+    for (int q = C - 1; q >= 0; --q) {
+      const int j0 = lo + q;
+      if (!G || j0 < ny) {
+        if (j0 < ny - 1) {
+          const T a1 = q + 1 < C ? ar[q + 1] : arnext;
+          x = y[q] - a1 * x;
+        }
+        C3D_ST_ONCE(x, &sp[P2 * j0]);  // written once
+      }
+    }
+    arnext = ar[0];
+  };
+  T ya[C], yb[C];
+  // forward: the segments below the top in pairs, each load a segment ahead
+  if (top == 0) {
+    load(ya, 0, Gd{});
+    run(ya, 0, Gd{});
+  } else {
+    load(ya, 0, Un{});
+    int s = 0;
+    for (; s + 2 < top; s += 2) {
+      load(yb, s + 1, Un{});
+      run(ya, s * C, Un{});
+      save(s);
+      load(ya, s + 2, Un{});
+      run(yb, (s + 1) * C, Un{});
+      save(s + 1);
+    }
+    if (s + 1 < top) {
+      load(yb, s + 1, Un{});
+      run(ya, s * C, Un{});
+      save(s);
+      load(ya, top, Gd{});
+      run(yb, (s + 1) * C, Un{});
+      save(s + 1);
+      run(ya, top * C, Gd{});
+    } else {
+      load(yb, top, Gd{});
+      run(ya, s * C, Un{});
+      save(s);
+      run(yb, top * C, Gd{});
+#pragma unroll
+      for (int q = 0; q < C; ++q) ya[q] = yb[q];
+    }
+  }
+  // backward; ya and ar hold the top segment
+  x = f;
+  if (top > 0) load(yb, top - 1, Un{});
+  back(ya, top, Gd{});
+  for (int s = top - 1; s >= 0; --s) {
+    // yb holds segment s as stored; the segment below goes into ya
+    if (s > 0) load(ya, s - 1, Un{});
+    if (s > 0) {
+      f = ck[cs * (3 * (s - 1) + part)];
+      vecm = ck[cs * (3 * (s - 1) + 2)];
+      cprev = m.ap2Pr[s * C];
+    }
+    run(yb, s * C, Un{});
+    back(yb, s, Un{});
+#pragma unroll
+    for (int q = 0; q < C; ++q) yb[q] = ya[q];
+  }
+}
+
+template <typename T, int C, bool NTS = false>
+__global__ __launch_bounds__(256) void k_tridiag_ck2(Geom g, Metrics<T> m,
+                                                     typename CT2<T>::t* __restrict__ spec,
+                                                     T* __restrict__ ckp, long long bsp,
+                                                     long long bck) {
+  k_tridiag_ck2_body<T, C, NTS>(g, m, spec + bsp * blockIdx.z, ckp + bck * blockIdx.z);
+}
+
 // At least 4 waves per SIMD: the FP64 body fits 128 VGPRs without spilling
 // (136 unconstrained, 3 waves), and at 768x384x576 the 1008 workgroups then
 // run in one round instead of 1.3 -- 0.99 ms per solve instead of 1.21 ms.
@@ -264,7 +413,19 @@ static int tridiag_t(c3d_ctx* c, void* spec, void* ck, const Batch& b) {
   constexpr int CSEG = 16;
   // checkpoints: 3 reals per line and segment, in the spectral scratch
   const long long need = 3LL * ((g.ny + CSEG - 1) / CSEG) * g.P * g.nz;
-  if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
+  // two lanes per wavenumber pair where a lane per pair leaves the SIMDs
+  // short of wavefronts (fewer than two per SIMD of the 256-CU device)
+  const long long lines = (long long)(g.nk + 1) * g.nz * b.n;
+  const bool two = c->tri_lanes ? c->tri_lanes == 2 : lines * 2 <= 2048LL * 64;
+  if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz && two) {
+    const dim3 grid2((g.nk + 1 + 31) / 32, (g.nz + 3) / 4, b.n);
+    if (g.nt)
+      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, true>), grid2, blk, 0, c->stream, g, m,
+                         (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
+    else
+      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, false>), grid2, blk, 0, c->stream, g, m,
+                         (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
+  } else if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
     if (g.nt)
       hipLaunchKernelGGL((k_tridiag_ck<T, CSEG, true>), grid, blk, 0, c->stream, g, m,
                          (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);

@@ -201,6 +201,36 @@ def test_ppe(O, name):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["small", "odd", "c1"])
+def test_tridiag_lane_forms_bitwise(O, name, monkeypatch):
+    """the checkpointed PPE sweep with one lane per wavenumber pair and with
+    the real and imaginary parts on two lanes: bit for bit the same, and at
+    the oracle within the bar"""
+    from channel3d_amd import Channel, lib as L
+    from channel3d_amd import synth
+    case = CASES[name]
+    o, g0 = make_pair(O, case, laminar=False)
+    nx, ny, nz = o.nxc, o.nyc, o.nzc
+    dyp = o.metric(O.M_DYP, 1, ny)
+    src = synth.ppe_rhs(nx, ny, nz, seed=7)
+    w = np.broadcast_to(dyp[None, :, None], src.shape)
+    src = np.asfortranarray(src - (src * w).sum() / w.sum())
+    res = []
+    for lanes in ("1", "2"):
+        monkeypatch.setenv("C3D_TRIDIAG_LANES", lanes)
+        g = Channel(case, yp=o.yp)
+        monkeypatch.delenv("C3D_TRIDIAG_LANES")
+        g.upload(L.ARR1, src)
+        g.ppe(L.ARR1, L.HALOARR)
+        res.append(interior(g.download(L.HALOARR)))
+        g.close()
+    assert np.array_equal(res[0], res[1])
+    phi = np.zeros_like(o.field(O.HALO))
+    o.call("clcPPE", np.asfortranarray(src.copy()), phi)
+    assert relerr(res[0], interior(phi)) < TOL
+    g0.close()
+
+
 def test_ppe_twenty_seeded_sources(O):
     """config 1's standalone solves: 20 sources ~ N(0,1), seeds 0..19, with
     the dyp-weighted mean removed, at 64^3, each against the oracle's clcPPE"""
