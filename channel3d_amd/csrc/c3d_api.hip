@@ -582,7 +582,8 @@ extern "C" int c3d_destroy(c3d_ctx* c) {
   void* bufs[] = {c->tmpA, c->spec, c->sarr, c->d_partials, c->d_sc, c->d_tw, c->d_met, c->d_tabs,
                   c->d_dyp, c->d_stat, c->d_tw32, c->d_met32, c->resid32, c->d_mean64,
                   c->d_lesmet, c->d_spec1d, c->spec2, c->d_spec2d, c->cln_d, c->cln_f,
-                  c->cln_fb, c->bspec, c->bsarr, c->bmean, c->bres, c->btmp};
+                  c->cln_fb, c->bspec, c->bsarr, c->bmean, c->bres, c->btmp,
+                  c->les_arr[0], c->les_arr[1]};
   for (void* p : bufs)
     if (p) hipFree(p);
   if (c->staging) hipHostFree(c->staging);
@@ -1453,10 +1454,14 @@ static int substep_les(c3d_ctx* c, int ns) {
                            s[C3D_ARR2], s[C3D_HALOARR], s[C3D_HISTX], s[C3D_HISTY], s[C3D_HISTZ],
                            1)))
     return rc;
-  // the constant-flow-rate shift of RhsX is applied as the ux solve loads it
-  if ((rc = launch_les_y(c, 0, s[C3D_UX], s[C3D_ARR1], nullptr, c->p.is_ux_const))) return rc;
-  if ((rc = launch_les_y(c, 1, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
-  if ((rc = launch_les_y(c, 2, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
+  // the three wall-normal solves in one launch (the ux solve's elimination
+  // scratch in tmpA, the others' in the LES scratch fields); the
+  // constant-flow-rate shift of RhsX is applied as the ux solve loads it
+  {
+    void* const u[3] = {s[C3D_UX], s[C3D_UY], s[C3D_UZ]};
+    void* const r[3] = {s[C3D_ARR1], s[C3D_ARR2], s[C3D_HALOARR]};
+    if ((rc = launch_les_y3(c, u, r, s[C3D_DUY_YM], c->p.is_ux_const))) return rc;
+  }
   if ((rc = launch_bc_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
   if (c->ppe_sweeps < 0 && fft_size_compiled(c->g.nk) && !c->g.fftrt) {
     if ((rc = launch_prsrc_fft_x(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], nullptr))) return rc;
@@ -1690,6 +1695,9 @@ extern "C" int c3d_set_les(c3d_ctx* c, int les_type, int filter_type) {
     if ((rc = dalloc(c, &c->state[C3D_NUT], fbytes))) return rc;
     C3D_HIP(hipMemset(c->state[C3D_NUT], 0, fbytes));  // nut = zero (m_Variables.f90:71-74)
   }
+  for (int d = 0; d < 2; ++d)
+    if (!c->les_arr[d] && (rc = dalloc(c, &c->les_arr[d], (size_t)c->g.nfield * c->esize)))
+      return rc;
   if (!c->d_lesmet) {
     const size_t L = (size_t)c->g.ny + 3;
     if ((rc = dalloc(c, &c->d_lesmet, 4 * L * c->esize))) return rc;

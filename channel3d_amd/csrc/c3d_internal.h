@@ -290,6 +290,7 @@ struct c3d_ctx {
   int* cln_fb = nullptr;
   double les_fcoe[9] = {};
   void* d_lesmet = nullptr;
+  void* les_arr[2] = {nullptr, nullptr};  // elimination scratch of the uy and uz LES solves
   c3d::LesMet<double> lmd{};
   c3d::LesMet<float> lmf{};
 
@@ -398,6 +399,8 @@ int launch_rhs_les(c3d_ctx* c, int mask, const void* ux, const void* uy, const v
                    int want_bulk);
 // comp 0 ux, 1 uy (wall row from duy_ym), 2 uz; the right-hand side is scratch
 int launch_les_y(c3d_ctx* c, int comp, void* u, void* rhs, const void* duy_ym, int add_dpmdxns);
+int launch_les_y3(c3d_ctx* c, void* const u[3], void* const rhs[3], const void* duy_ym,
+                  int add_dpmdxns);
 // pressure += (1 - pmBetaT d/dy nut d/dy) phi; phi's wall ghosts must be set
 int launch_prupd_les(c3d_ctx* c, void* pr, const void* phi);
 

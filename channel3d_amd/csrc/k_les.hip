@@ -411,14 +411,11 @@ __device__ __forceinline__ void les_row(const Geom& g, const LesMet<T>& lm, cons
 }
 
 template <typename T, int COMP, bool ADDD>
-__global__ __launch_bounds__(64) void k_les_y(Geom g, Metrics<T> m, LesMet<T> lm, T bt,
-                                              const T* __restrict__ nut, T* __restrict__ rhs,
-                                              T* __restrict__ arr, T* __restrict__ u,
-                                              const T* __restrict__ duy,
-                                              const DevScalars* __restrict__ sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int k = blockIdx.y;
-  if (i >= g.nx) return;
+__device__ __forceinline__ void les_y_line(const Geom& g, const Metrics<T>& m, const LesMet<T>& lm,
+                                           T bt, const T* __restrict__ nut,
+                                           T* __restrict__ rhs, T* __restrict__ arr,
+                                           T* __restrict__ u, const T* __restrict__ duy,
+                                           const DevScalars* __restrict__ sc, int i, int k) {
   const int ny = g.ny;
   const int im = wm(i, g.nx), km = wm(k, g.nz);
   const T dd = ADDD ? (T)sc->dpmdxns : T(0);
@@ -465,6 +462,45 @@ __global__ __launch_bounds__(64) void k_les_y(Geom g, Metrics<T> m, LesMet<T> lm
     u[q] = u[q] + fj;
     fnext = fj;
   }
+}
+
+template <typename T, int COMP, bool ADDD>
+__global__ __launch_bounds__(64) void k_les_y(Geom g, Metrics<T> m, LesMet<T> lm, T bt,
+                                              const T* __restrict__ nut, T* __restrict__ rhs,
+                                              T* __restrict__ arr, T* __restrict__ u,
+                                              const T* __restrict__ duy,
+                                              const DevScalars* __restrict__ sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= g.nx) return;
+  les_y_line<T, COMP, ADDD>(g, m, lm, bt, nut, rhs, arr, u, duy, sc, i, blockIdx.y);
+}
+
+// The three component solves of one LES substep in one launch: blockIdx.z is
+// the component, each with its own velocity, right-hand side and elimination
+// scratch. The solves are independent (each reads only nut and its own
+// arrays), and a single component's launch fills few CUs on small grids, so
+// three sequential launches would each wait out one line's serial sweep.
+template <typename T>
+struct LesY3 {
+  T* u[3];
+  T* rhs[3];
+  T* arr[3];
+  const T* duy;
+};
+
+template <typename T, bool ADDD>
+__global__ __launch_bounds__(64) void k_les_y3(Geom g, Metrics<T> m, LesMet<T> lm, T bt,
+                                               const T* __restrict__ nut, LesY3<T> a,
+                                               const DevScalars* __restrict__ sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= g.nx) return;
+  const int k = blockIdx.y;
+  if (blockIdx.z == 0)
+    les_y_line<T, 0, ADDD>(g, m, lm, bt, nut, a.rhs[0], a.arr[0], a.u[0], nullptr, sc, i, k);
+  else if (blockIdx.z == 1)
+    les_y_line<T, 1, false>(g, m, lm, bt, nut, a.rhs[1], a.arr[1], a.u[1], a.duy, sc, i, k);
+  else
+    les_y_line<T, 2, false>(g, m, lm, bt, nut, a.rhs[2], a.arr[2], a.u[2], nullptr, sc, i, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -594,6 +630,46 @@ int launch_les_y(c3d_ctx* c, int comp, void* u, void* rhs, const void* duy_ym, i
   if (rc) return rc;
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   prof_end(c, ST_ADI_Y, ev, 4.0 * N * c->esize);
+  return C3D_OK;
+}
+
+template <typename T>
+static int les_y3_t(c3d_ctx* c, void* const u[3], void* const rhs[3], const void* duy, int addd) {
+  const Geom& g = c->g;
+  LesY3<T> a;
+  void* const arr[3] = {c->tmpA, c->les_arr[0], c->les_arr[1]};
+  for (int d = 0; d < 3; ++d) {
+    a.u[d] = (T*)u[d];
+    a.rhs[d] = (T*)rhs[d];
+    a.arr[d] = (T*)arr[d];
+  }
+  a.duy = (const T*)duy;
+  dim3 grid((g.nx + 63) / 64, g.nz, 3), blk(64);
+  const T bt = (T)c->pmBetaT;
+  const T* nut = (const T*)c->state[C3D_NUT];
+  if (addd)
+    hipLaunchKernelGGL((k_les_y3<T, true>), grid, blk, 0, c->stream, g, met<T>(c), lesmet<T>(c),
+                       bt, nut, a, (const DevScalars*)c->d_sc);
+  else
+    hipLaunchKernelGGL((k_les_y3<T, false>), grid, blk, 0, c->stream, g, met<T>(c), lesmet<T>(c),
+                       bt, nut, a, (const DevScalars*)c->d_sc);
+  C3D_HIP(hipGetLastError());
+  return C3D_OK;
+}
+
+int launch_les_y3(c3d_ctx* c, void* const u[3], void* const rhs[3], const void* duy_ym,
+                  int add_dpmdxns) {
+  if (!duy_ym || !c->les_arr[0] || !c->les_arr[1]) {
+    set_error("LES y-solves: duy_ym and the LES scratch fields are required");
+    return C3D_ERR_ARG;
+  }
+  hipEvent_t ev;
+  prof_begin(c, ST_ADI_Y, &ev);
+  int rc = c->prec == C3D_FP64 ? les_y3_t<double>(c, u, rhs, duy_ym, add_dpmdxns)
+                               : les_y3_t<float>(c, u, rhs, duy_ym, add_dpmdxns);
+  if (rc) return rc;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  prof_end(c, ST_ADI_Y, ev, 12.0 * N * c->esize);
   return C3D_OK;
 }
 
