@@ -790,6 +790,14 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
         return fail(C3D_ERR_ARG);
       }
     }
+    if (const char* yt = std::getenv("C3D_YSOLVE_THREADS")) {
+      const std::string v(yt);
+      if (v == "64" || v == "256") c->y_threads = std::stoi(v);
+      else {
+        set_error("c3d_create: C3D_YSOLVE_THREADS must be 64 or 256");
+        return fail(C3D_ERR_ARG);
+      }
+    }
     if ((rc = build_adi_tables(c))) return fail(rc);
   }
   rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);

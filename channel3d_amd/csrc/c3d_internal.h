@@ -223,6 +223,7 @@ struct c3d_ctx {
   // lanes per wavenumber pair of the checkpointed PPE sweep: 0 by the line
   // count, 1 or 2 forced (tests of both forms)
   int tri_lanes = 0;
+  int y_threads = 0;  // lines per workgroup of the checkpointed y sweep (0: by grid size)
 
   // state and scratch
   void* state[C3D_NSTATE] = {};

@@ -295,10 +295,13 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 // condition x = 0 (a solve for the increment alone).  SET: u = x instead of
 // u += x, with u of type TU (an FP32 solve writing an FP64 increment).
 // Batches: blockIdx.y selects the field, bf reals apart in r and u.
+// NTH lines per workgroup (each thread owns one line and one LDS checkpoint
+// column, so NTH changes only the grouping): 256, or 64 on grids with few
+// lines, where 256-line workgroups would leave CUs idle.
 constexpr int YNT = 256;
 template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
-          typename TU = T, bool FY = false>
-__global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
+          typename TU = T, bool FY = false, int NTH = YNT>
+__global__ __launch_bounds__(NTH) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
                                                   const double* __restrict__ tvecm,
                                                   const double* __restrict__ tarr,
                                                   const T* __restrict__ r, TU* __restrict__ u,
@@ -306,9 +309,9 @@ __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restri
                                                   const DevScalars* __restrict__ sc,
                                                   long long bf) {
   extern __shared__ unsigned char ck_raw[];
-  T* ck = (T*)ck_raw;  // [segment][YNT threads]
+  T* ck = (T*)ck_raw;  // [segment][NTH threads]
   const int tid = threadIdx.x;
-  const long long line = (long long)blockIdx.x * YNT + tid;
+  const long long line = (long long)blockIdx.x * NTH + tid;
   if (line >= (long long)g.nx * g.nz) return;
   const int i = (int)(line % g.nx), k = (int)(line / g.nx);
   r += bf * blockIdx.y;
@@ -390,24 +393,24 @@ __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restri
     for (; s + 2 < top; s += 2) {
       load(yb, s + 1, Un{});
       fwd(ya, s, f, Un{});
-      ck[s * YNT + tid] = f;  // forward value at row (s+1)*C
+      ck[s * NTH + tid] = f;  // forward value at row (s+1)*C
       load(ya, s + 2, Un{});
       fwd(yb, s + 1, f, Un{});
-      ck[(s + 1) * YNT + tid] = f;
+      ck[(s + 1) * NTH + tid] = f;
     }
     // one or two segments left below the top, then the top one
     if (s + 1 < top) {
       load(yb, s + 1, Un{});
       fwd(ya, s, f, Un{});
-      ck[s * YNT + tid] = f;
+      ck[s * NTH + tid] = f;
       load(ya, top, Gd{});
       fwd(yb, s + 1, f, Un{});
-      ck[(s + 1) * YNT + tid] = f;
+      ck[(s + 1) * NTH + tid] = f;
       fwd(ya, top, f, Gd{});
     } else {
       load(yb, top, Gd{});
       fwd(ya, s, f, Un{});
-      ck[s * YNT + tid] = f;
+      ck[s * NTH + tid] = f;
       fwd(yb, top, f, Gd{});
 #pragma unroll
       for (int q = 0; q < C; ++q) ya[q] = yb[q];
@@ -420,13 +423,13 @@ __global__ __launch_bounds__(YNT) void k_adi_y_ck(Geom g, const double* __restri
     load(yb, top - 1, Un{});
     back(ya, ua, top, Gd{});
     loadu(ua, top - 1, Un{});
-    T fp = top > 1 ? ck[(top - 2) * YNT + tid] : T(0);
+    T fp = top > 1 ? ck[(top - 2) * NTH + tid] : T(0);
     fwd(yb, top - 1, fp, Un{});
     for (s = top - 1; s > 0; --s) {
       load(ya, s - 1, Un{});  // the segment below
       back(yb, ua, s, Un{});
       loadu(ua, s - 1, Un{});
-      fp = s > 1 ? ck[(s - 2) * YNT + tid] : T(0);
+      fp = s > 1 ? ck[(s - 2) * NTH + tid] : T(0);
       fwd(ya, s - 1, fp, Un{});
 #pragma unroll
       for (int q = 0; q < C; ++q) yb[q] = ya[q];
@@ -446,32 +449,42 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
   double* const* yt = comp == 1 ? tb.d_yp : tb.d_yc;
   const DevScalars* sc = (const DevScalars*)c->d_sc;
   constexpr int CSEG = 16;
-  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * YNT * sizeof(T);
+  const long long lines = (long long)g.nx * g.nz;
+  // 64-line workgroups below 512 workgroups of 256 lines (two per CU)
+  const int nth = c->y_threads ? c->y_threads : lines < 512LL * YNT ? 64 : YNT;
+  const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * nth * sizeof(T);
   if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
-    const dim3 gy((unsigned)(((long long)g.nx * g.nz + YNT - 1) / YNT)), by(YNT);
-    auto launch = [&](auto ntc, auto fyc) {
+    const dim3 gy((unsigned)((lines + nth - 1) / nth)), by(nth);
+    auto launch = [&](auto ntc, auto fyc, auto nthc) {
       constexpr bool NT_ = decltype(ntc)::value, FY = decltype(fyc)::value;
+      constexpr int N_ = decltype(nthc)::value;
       if (comp == 1)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 1, false, NT_, false, T, FY>), gy, by, ck_bytes,
-                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 1, false, NT_, false, T, FY, N_>), gy, by,
+                           ck_bytes, c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
                            (const T*)duy_ym, sc, 0LL);
       else if (shift)
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, true, NT_, false, T, FY>), gy, by, ck_bytes,
-                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, true, NT_, false, T, FY, N_>), gy, by,
+                           ck_bytes, c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
                            (const T*)nullptr, sc, 0LL);
       else
-        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, false, NT_, false, T, FY>), gy, by, ck_bytes,
-                           c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
+        hipLaunchKernelGGL((k_adi_y_ck<T, CSEG, 0, false, NT_, false, T, FY, N_>), gy, by,
+                           ck_bytes, c->stream, g, yt[0], yt[1], yt[2], (const T*)rhs, (T*)u,
                            (const T*)nullptr, sc, 0LL);
     };
     // whole segments only (ny a multiple of the segment): the guard-free sweep
     const bool fy = g.ny % CSEG == 0;
+    using N64 = std::integral_constant<int, 64>;
+    using N256 = std::integral_constant<int, YNT>;
+    auto by_nth = [&](auto ntc, auto fyc) {
+      if (nth == 64) launch(ntc, fyc, N64{});
+      else launch(ntc, fyc, N256{});
+    };
     if (g.nt) {
-      if (fy) launch(std::true_type{}, std::true_type{});
-      else launch(std::true_type{}, std::false_type{});
+      if (fy) by_nth(std::true_type{}, std::true_type{});
+      else by_nth(std::true_type{}, std::false_type{});
     } else {
-      if (fy) launch(std::false_type{}, std::true_type{});
-      else launch(std::false_type{}, std::false_type{});
+      if (fy) by_nth(std::false_type{}, std::true_type{});
+      else by_nth(std::false_type{}, std::false_type{});
     }
     C3D_HIP(hipGetLastError());
     return C3D_OK;

@@ -364,6 +364,28 @@ def test_fused_substeps(O, name):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["small", "odd", "c1"])
+def test_ysweep_workgroup_forms_bitwise(O, name, monkeypatch):
+    """the checkpointed wall-normal sweeps of clcU1Hat..clcU3Hat with 64- and
+    256-line workgroups: bit for bit the same substeps, and at the oracle
+    within the bar"""
+    from channel3d_amd import lib as L
+    case = CASES[name]
+    res = []
+    for nth in ("64", "256"):
+        monkeypatch.setenv("C3D_YSOLVE_THREADS", nth)
+        o, g = make_pair(O, case, seed=1)
+        monkeypatch.delenv("C3D_YSOLVE_THREADS")
+        for ns in (1, 2, 3):
+            o.substep(ns)
+            g.substep(ns)
+        compare_state(O, o, g, TOL)
+        res.append([g.download(gid) for gid in (L.UX, L.UY, L.UZ, L.PR)])
+        g.close()
+    for a, b in zip(*res):
+        assert np.array_equal(ymid(a), ymid(b))
+
+
 def test_per_routine_sequence_matches_fused(O):
     """ChannelIterate's call sequence through the per-routine entries gives the
     fused substep's result"""
