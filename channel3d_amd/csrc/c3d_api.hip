@@ -1434,11 +1434,13 @@ static int substep4(c3d_ctx* c, int ns) {
   if ((rc = launch_rhs4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_PR], s[C3D_ARR1], s[C3D_ARR2],
                         s[C3D_HALOARR], s[C3D_HISTX], s[C3D_HISTY], s[C3D_HISTZ], flow)))
     return rc;
-  // clcU1Hat..U3Hat (:120-122); RhsX's flow-rate shift is applied as the y
-  // sweep loads it
-  if ((rc = launch_adi(c, 0, ns, s[C3D_UX], s[C3D_ARR1], nullptr, flow))) return rc;
-  if ((rc = launch_adi(c, 1, ns, s[C3D_UY], s[C3D_ARR2], s[C3D_DUY_YM], 0))) return rc;
-  if ((rc = launch_adi(c, 2, ns, s[C3D_UZ], s[C3D_HALOARR], nullptr, 0))) return rc;
+  // clcU1Hat..U3Hat (:120-122) in one launch; RhsX's flow-rate shift is
+  // applied as the y sweep loads it
+  {
+    void* const u[3] = {s[C3D_UX], s[C3D_UY], s[C3D_UZ]};
+    void* const r[3] = {s[C3D_ARR1], s[C3D_ARR2], s[C3D_HALOARR]};
+    if ((rc = launch_adi_y3(c, ns, u, r, s[C3D_DUY_YM], flow))) return rc;
+  }
   // :126-130
   if ((rc = launch_bc4_vel(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_UY_YM], 0))) return rc;
   if ((rc = launch_prsrc4(c, s[C3D_UX], s[C3D_UY], s[C3D_UZ], s[C3D_ARR1], 0.0))) return rc;

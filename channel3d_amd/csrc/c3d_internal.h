@@ -368,6 +368,8 @@ int launch_diag_finalize(c3d_ctx* c, int nblocks, int what, int cfl_slot);
 // (y solve only) subtracts dp1ns from RhsX as the sweep loads it
 int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy_ym,
                int add_dpmdxns);
+int launch_adi_y3(c3d_ctx* c, int ns, void* const u[3], void* const rhs[3], const void* duy_ym,
+                  int add_dpmdxns);
 
 // ---- launchers (k_stencil4.hip, 4th-order path) ----
 template <typename T>

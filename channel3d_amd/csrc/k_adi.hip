@@ -299,23 +299,18 @@ __global__ __launch_bounds__(256) void k_adi_y(Geom g, const double* __restrict_
 // column, so NTH changes only the grouping): 256, or 64 on grids with few
 // lines, where 256-line workgroups would leave CUs idle.
 constexpr int YNT = 256;
-template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
-          typename TU = T, bool FY = false, int NTH = YNT>
-__global__ __launch_bounds__(NTH) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
-                                                  const double* __restrict__ tvecm,
-                                                  const double* __restrict__ tarr,
-                                                  const T* __restrict__ r, TU* __restrict__ u,
-                                                  const T* __restrict__ duy,
-                                                  const DevScalars* __restrict__ sc,
-                                                  long long bf) {
-  extern __shared__ unsigned char ck_raw[];
-  T* ck = (T*)ck_raw;  // [segment][NTH threads]
-  const int tid = threadIdx.x;
-  const long long line = (long long)blockIdx.x * NTH + tid;
-  if (line >= (long long)g.nx * g.nz) return;
+// One line's sweep: ck is the workgroup's LDS checkpoint array
+// [segment][NTH threads], tid the thread's column, line < nx*nz.
+template <typename T, int C, int WALL, bool SHIFT, bool NTS, bool SET, typename TU, bool FY,
+          int NTH>
+__device__ __forceinline__ void adi_y_ck_line(const Geom& g, const double* __restrict__ ta,
+                                              const double* __restrict__ tvecm,
+                                              const double* __restrict__ tarr,
+                                              const T* __restrict__ r, TU* __restrict__ u,
+                                              const T* __restrict__ duy,
+                                              const DevScalars* __restrict__ sc,
+                                              T* __restrict__ ck, int tid, long long line) {
   const int i = (int)(line % g.nx), k = (int)(line / g.nx);
-  r += bf * blockIdx.y;
-  u += bf * blockIdx.y;
   constexpr bool WALLROW = WALL != 0;
   const int ny = g.ny;
   const int nseg = (ny + C - 1) / C;
@@ -440,6 +435,68 @@ __global__ __launch_bounds__(NTH) void k_adi_y_ck(Geom g, const double* __restri
   }
 }
 
+// Batches: blockIdx.y selects the field, bf reals apart in r and u.
+template <typename T, int C, int WALL, bool SHIFT, bool NTS = false, bool SET = false,
+          typename TU = T, bool FY = false, int NTH = YNT>
+__global__ __launch_bounds__(NTH) void k_adi_y_ck(Geom g, const double* __restrict__ ta,
+                                                  const double* __restrict__ tvecm,
+                                                  const double* __restrict__ tarr,
+                                                  const T* __restrict__ r, TU* __restrict__ u,
+                                                  const T* __restrict__ duy,
+                                                  const DevScalars* __restrict__ sc,
+                                                  long long bf) {
+  extern __shared__ unsigned char ck_raw[];
+  const int tid = threadIdx.x;
+  const long long line = (long long)blockIdx.x * NTH + tid;
+  if (line >= (long long)g.nx * g.nz) return;
+  adi_y_ck_line<T, C, WALL, SHIFT, NTS, SET, TU, FY, NTH>(g, ta, tvecm, tarr, r + bf * blockIdx.y,
+                                                          u + bf * blockIdx.y, duy, sc,
+                                                          (T*)ck_raw, tid, line);
+}
+
+// The three components' sweeps of a y-only substep (the 4th-order
+// Crank-Nicolson solves) in one launch, blockIdx.y = component: ux (SHIFT:
+// the constant-flow-rate shift), uy (the wall row from duy, the face tables),
+// uz.  Each component reads only its own right-hand side and velocity, so
+// the three are independent; one component's launch leaves most of the GPU
+// waiting on each line's serial sweep when the grid has few lines.
+template <typename T>
+struct AdiY3 {
+  const double* t[3][3];  // per component: ta, tvecm, tarr
+  const T* r[3];
+  T* u[3];
+  const T* duy;
+};
+
+template <typename T, int C, bool SHIFT, bool NTS, bool FY, int NTH>
+__global__ __launch_bounds__(NTH) void k_adi_y3(Geom g, AdiY3<T> a,
+                                                const DevScalars* __restrict__ sc) {
+  extern __shared__ unsigned char ck_raw[];
+  const int tid = threadIdx.x;
+  const long long line = (long long)blockIdx.x * NTH + tid;
+  if (line >= (long long)g.nx * g.nz) return;
+  T* ck = (T*)ck_raw;
+  if (blockIdx.y == 0)
+    adi_y_ck_line<T, C, 0, SHIFT, NTS, false, T, FY, NTH>(g, a.t[0][0], a.t[0][1], a.t[0][2],
+                                                          a.r[0], a.u[0], nullptr, sc, ck, tid,
+                                                          line);
+  else if (blockIdx.y == 1)
+    adi_y_ck_line<T, C, 1, false, NTS, false, T, FY, NTH>(g, a.t[1][0], a.t[1][1], a.t[1][2],
+                                                          a.r[1], a.u[1], a.duy, sc, ck, tid,
+                                                          line);
+  else
+    adi_y_ck_line<T, C, 0, false, NTS, false, T, FY, NTH>(g, a.t[2][0], a.t[2][1], a.t[2][2],
+                                                          a.r[2], a.u[2], nullptr, sc, ck, tid,
+                                                          line);
+}
+
+// lines per workgroup of the checkpointed sweep: 64 below 512 workgroups
+// of 256 lines (two per CU), else 256
+static int y_threads_for(const c3d_ctx* c) {
+  const long long lines = (long long)c->g.nx * c->g.nz;
+  return c->y_threads ? c->y_threads : lines < 512LL * YNT ? 64 : YNT;
+}
+
 // y solve + update of one component
 template <typename T>
 static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, void* u,
@@ -450,8 +507,7 @@ static int y_solve(c3d_ctx* c, const c3d_ctx::AdiTab& tb, int comp, void* rhs, v
   const DevScalars* sc = (const DevScalars*)c->d_sc;
   constexpr int CSEG = 16;
   const long long lines = (long long)g.nx * g.nz;
-  // 64-line workgroups below 512 workgroups of 256 lines (two per CU)
-  const int nth = c->y_threads ? c->y_threads : lines < 512LL * YNT ? 64 : YNT;
+  const int nth = y_threads_for(c);
   const size_t ck_bytes = (size_t)((g.ny + CSEG - 1) / CSEG) * nth * sizeof(T);
   if (c->ymode == YS_CHECKPOINT && ck_bytes <= 64 * 1024) {
     const dim3 gy((unsigned)((lines + nth - 1) / nth)), by(nth);
@@ -608,6 +664,82 @@ int launch_adi(c3d_ctx* c, int comp, int ns, void* u, void* rhs, const void* duy
   }
   return c->prec == C3D_FP64 ? adi_t<double>(c, comp, ns, u, rhs, duy_ym, add_dpmdxns)
                              : adi_t<float>(c, comp, ns, u, rhs, duy_ym, add_dpmdxns);
+}
+
+// the three y-only solves of a 4th-order substep in one launch; false when
+// the checkpointed sweep does not apply (the caller then runs launch_adi per
+// component)
+constexpr int Y3SEG = 16;
+static size_t y3_ck_bytes(const c3d_ctx* c) {
+  return (size_t)((c->g.ny + Y3SEG - 1) / Y3SEG) * y_threads_for(c) * c->esize;
+}
+
+template <typename T>
+static void adi_y3_t(c3d_ctx* c, int ns, void* const u[3], void* const rhs[3], const void* duy,
+                     bool shift) {
+  const Geom& g = c->g;
+  const auto& tb = c->adi[ns - 1];
+  constexpr int CSEG = Y3SEG;
+  const long long lines = (long long)g.nx * g.nz;
+  const int nth = y_threads_for(c);
+  const size_t ck_bytes = y3_ck_bytes(c);
+  AdiY3<T> a;
+  for (int d = 0; d < 3; ++d) {
+    double* const* yt = d == 1 ? tb.d_yp : tb.d_yc;
+    for (int q = 0; q < 3; ++q) a.t[d][q] = yt[q];
+    a.r[d] = (const T*)rhs[d];
+    a.u[d] = (T*)u[d];
+  }
+  a.duy = (const T*)duy;
+  const dim3 gy((unsigned)((lines + nth - 1) / nth), 3), by(nth);
+  const DevScalars* sc = (const DevScalars*)c->d_sc;
+  auto launch = [&](auto shc, auto ntc, auto fyc, auto nthc) {
+    constexpr bool SH = decltype(shc)::value, NT_ = decltype(ntc)::value,
+                   FY = decltype(fyc)::value;
+    constexpr int N_ = decltype(nthc)::value;
+    hipLaunchKernelGGL((k_adi_y3<T, CSEG, SH, NT_, FY, N_>), gy, by, ck_bytes, c->stream, g, a,
+                       sc);
+  };
+  using N64 = std::integral_constant<int, 64>;
+  using N256 = std::integral_constant<int, YNT>;
+  auto by_nth = [&](auto shc, auto ntc, auto fyc) {
+    if (nth == 64) launch(shc, ntc, fyc, N64{});
+    else launch(shc, ntc, fyc, N256{});
+  };
+  auto by_fy = [&](auto shc, auto ntc) {
+    if (g.ny % CSEG == 0) by_nth(shc, ntc, std::true_type{});
+    else by_nth(shc, ntc, std::false_type{});
+  };
+  auto by_nt = [&](auto shc) {
+    if (g.nt) by_fy(shc, std::true_type{});
+    else by_fy(shc, std::false_type{});
+  };
+  if (shift) by_nt(std::true_type{});
+  else by_nt(std::false_type{});
+}
+
+int launch_adi_y3(c3d_ctx* c, int ns, void* const u[3], void* const rhs[3], const void* duy_ym,
+                  int add_dpmdxns) {
+  if (ns < 1 || ns > 3 || c->order != 4 || !duy_ym) {
+    set_error("adi_y3: 4th-order substep 1..3 with duy_ym");
+    return C3D_ERR_ARG;
+  }
+  if (c->ymode == YS_CHECKPOINT && y3_ck_bytes(c) <= 64 * 1024) {
+    const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+    hipEvent_t ev;
+    prof_begin(c, ST_ADI_Y, &ev);
+    if (c->prec == C3D_FP64) adi_y3_t<double>(c, ns, u, rhs, duy_ym, add_dpmdxns != 0);
+    else adi_y3_t<float>(c, ns, u, rhs, duy_ym, add_dpmdxns != 0);
+    C3D_HIP(hipGetLastError());
+    prof_end(c, ST_ADI_Y, ev, 9.0 * N * c->esize);
+    return C3D_OK;
+  }
+  int rc;
+  for (int d = 0; d < 3; ++d)
+    if ((rc = launch_adi(c, d, ns, u[d], rhs[d], d == 1 ? duy_ym : nullptr,
+                         d == 0 ? add_dpmdxns : 0)))
+      return rc;
+  return C3D_OK;
 }
 
 // ===========================================================================
