@@ -490,11 +490,12 @@ __global__ __launch_bounds__(NTH) void k_adi_y3(Geom g, AdiY3<T> a,
                                                           line);
 }
 
-// lines per workgroup of the checkpointed sweep: 64 below 512 workgroups
-// of 256 lines (two per CU), else 256
+// lines per workgroup of the checkpointed sweep: 64 below 256 workgroups of
+// 256 lines (one per CU), else 256 (measured: 64 faster at 64x64 and 192x160
+// lines, 256 faster at 384x288)
 static int y_threads_for(const c3d_ctx* c) {
   const long long lines = (long long)c->g.nx * c->g.nz;
-  return c->y_threads ? c->y_threads : lines < 512LL * YNT ? 64 : YNT;
+  return c->y_threads ? c->y_threads : lines < 256LL * YNT ? 64 : YNT;
 }
 
 // y solve + update of one component
