@@ -798,6 +798,14 @@ extern "C" int c3d_create(const c3d_params* p, const double* yp, c3d_ctx** out) 
         return fail(C3D_ERR_ARG);
       }
     }
+    if (const char* zt = std::getenv("C3D_FFTZ_TILE")) {
+      const std::string v(zt);
+      if (v == "8" || v == "16") c->z_tile = std::stoi(v);
+      else {
+        set_error("c3d_create: C3D_FFTZ_TILE must be 8 or 16");
+        return fail(C3D_ERR_ARG);
+      }
+    }
     if ((rc = build_adi_tables(c))) return fail(rc);
   }
   rc = c->prec == C3D_FP64 ? upload_twiddles<double>(c) : upload_twiddles<float>(c);

@@ -224,6 +224,7 @@ struct c3d_ctx {
   // count, 1 or 2 forced (tests of both forms)
   int tri_lanes = 0;
   int y_threads = 0;  // lines per workgroup of the checkpointed y sweep (0: by grid size)
+  int z_tile = 0;     // kx lines per workgroup of the z transform (0: 8)
 
   // state and scratch
   void* state[C3D_NSTATE] = {};

@@ -386,6 +386,28 @@ def test_ysweep_workgroup_forms_bitwise(O, name, monkeypatch):
         assert np.array_equal(ymid(a), ymid(b))
 
 
+@pytest.mark.parametrize("name", ["small", "odd", "c1"])
+def test_fftz_tile_forms_bitwise(O, name, monkeypatch):
+    """the PPE's z transforms with 8- and 16-line tiles (16 is the default
+    for FP64 lengths from 256 up, forced here on the small grids): bit for
+    bit the same substeps, and at the oracle within the bar"""
+    from channel3d_amd import lib as L
+    case = CASES[name]
+    res = []
+    for tile in ("8", "16"):
+        monkeypatch.setenv("C3D_FFTZ_TILE", tile)
+        o, g = make_pair(O, case, seed=2)
+        monkeypatch.delenv("C3D_FFTZ_TILE")
+        for ns in (1, 2, 3):
+            o.substep(ns)
+            g.substep(ns)
+        compare_state(O, o, g, TOL)
+        res.append([g.download(gid) for gid in (L.UX, L.UY, L.UZ, L.PR)])
+        g.close()
+    for a, b in zip(*res):
+        assert np.array_equal(ymid(a), ymid(b))
+
+
 def test_per_routine_sequence_matches_fused(O):
     """ChannelIterate's call sequence through the per-routine entries gives the
     fused substep's result"""
