@@ -338,7 +338,10 @@ int c3d_write_spectra2d(c3d_ctx* ctx, const char* path_spec2d);
  *                      ghosts must be set (SetBC_and_UpdateHalo_pr)
  *   c3d_step           evaluates nut once before the substeps
  *                      (m_ChannelSystem.f90:89); c3d_substep does not.
- * 2nd-order contexts only. */
+ * The first call that switches LES on allocates three field-sized device
+ * arrays (nut and the elimination scratch of the fused step's uy and uz
+ * wall-normal solves); they stay until c3d_destroy, so the step itself
+ * still allocates nothing.  2nd-order contexts only. */
 int c3d_set_les(c3d_ctx* ctx, int les_type, int filter_type);
 /* Clc_SGS_Vis (m_LESModels.f90:47-62): nut from the velocity (halo fields
  * with valid ghosts), including SetBC_and_UpdateHalo_SGS's wall ghosts */
