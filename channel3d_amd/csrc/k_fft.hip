@@ -878,42 +878,51 @@ static int fft_x_inv_t(c3d_ctx* c, const void* spec, void* out, double scale, co
   return C3D_OK;
 }
 
-// z transform of TK kx lines per workgroup
+// z transform of TK kx lines per workgroup (tw: the twiddles of T, bsp: the
+// batch stride of the spectra)
 template <typename T, int N, bool NTS, int TK>
-static void launch_z(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
+static void launch_z(c3d_ctx* c, void* spec, const void* tw, long long bsp, int nb,
+                     int inverse) {
   using T2 = typename CT<T>::t;
   const Geom& g = c->g;
-  const dim3 grid((g.nk + 1 + TK - 1) / TK, g.ny, b.n);
+  const dim3 grid((g.nk + 1 + TK - 1) / TK, g.ny, nb);
   if (inverse)
     hipLaunchKernelGGL((k_fft_z<T, N, 1, NTS, TK>), grid, dim3(FNT), 0, c->stream, g, (T2*)spec,
-                       (const T2*)c->tw_z, b.spec);
+                       (const T2*)tw, bsp);
   else
     hipLaunchKernelGGL((k_fft_z<T, N, -1, NTS, TK>), grid, dim3(FNT), 0, c->stream, g,
-                       (T2*)spec, (const T2*)c->tw_z, b.spec);
+                       (T2*)spec, (const T2*)tw, bsp);
 }
 
-// 16-line tiles (256-byte runs per z row, one workgroup per CU) where the tile
-// fits the 160 KiB of LDS: at 768x384x576 (FP64, 147.7 KiB) the forward and
-// inverse z transforms took 0.637 and 0.625 ms with 8-line tiles, 0.597 and
-// 0.589 ms with 16.  The default takes them from length 256 up (FP64).
+// 16-line tiles (one workgroup per CU; 256-byte runs per z row in FP64, 128
+// in FP32) where the tile fits the 160 KiB of LDS.  At 768x384x576 the
+// forward / inverse z transforms took 0.637 / 0.625 ms with 8-line tiles and
+// 0.597 / 0.589 ms with 16 in FP64 (147.7 KiB), 0.470 / 0.482 and
+// 0.366 / 0.355 ms in FP32.  The default takes them from length 256 up.
 template <typename T, int N>
 constexpr bool z16_fits() {
   return 16 * (N + 1) * (int)sizeof(typename CT<T>::t) <= 160 * 1024;
+}
+
+template <typename T, bool NTS, int N>
+static void launch_z_tiled(c3d_ctx* c, void* spec, const void* tw, long long bsp, int nb,
+                           int inverse) {
+  if constexpr (z16_fits<T, N>()) {
+    if (c->z_tile == 16 || (c->z_tile == 0 && N >= 256)) {
+      launch_z<T, N, NTS, 16>(c, spec, tw, bsp, nb, inverse);
+      return;
+    }
+  }
+  launch_z<T, N, NTS, ZTK>(c, spec, tw, bsp, nb, inverse);
 }
 
 template <typename T, bool NTS>
 static int fft_z_t(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   const Geom& g = c->g;
   switch (g.fftrt ? 0 : g.nz) {
-#define X(n_)                                                                \
-  case n_:                                                                   \
-    if constexpr (z16_fits<T, n_>()) {                                       \
-      if (c->z_tile == 16 || (c->z_tile == 0 && n_ >= 256 && sizeof(T) == 8)) { \
-        launch_z<T, n_, NTS, 16>(c, spec, inverse, b);                       \
-        break;                                                               \
-      }                                                                      \
-    }                                                                        \
-    launch_z<T, n_, NTS, ZTK>(c, spec, inverse, b);                          \
+#define X(n_)                                                                                  \
+  case n_:                                                                                     \
+    launch_z_tiled<T, NTS, n_>(c, spec, c->tw_z, b.spec, b.n, inverse);                        \
     break;
     C3D_FFT_SIZES(X)
 #undef X
@@ -969,16 +978,10 @@ static int fft_x_inv32(c3d_ctx* c, const void* spec, double* out, double scale, 
 
 static int fft_z32(c3d_ctx* c, void* spec, int inverse, const Batch& b) {
   const Geom& g = c->g;
-  dim3 grid((g.nk + 1 + ZTK - 1) / ZTK, g.ny, b.n);
   switch (g.fftrt ? 0 : g.nz) {
 #define X(n_)                                                                                  \
   case n_:                                                                                     \
-    if (inverse)                                                                               \
-      hipLaunchKernelGGL((k_fft_z<float, n_, 1>), grid, dim3(FNT), 0, c->stream, g,           \
-                         (float2*)spec, (const float2*)c->tw32_z, b.spec32);                   \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_fft_z<float, n_, -1>), grid, dim3(FNT), 0, c->stream, g,          \
-                         (float2*)spec, (const float2*)c->tw32_z, b.spec32);                   \
+    launch_z_tiled<float, false, n_>(c, spec, c->tw32_z, b.spec32, b.n, inverse);              \
     break;
     C3D_FFT_SIZES(X)
 #undef X

@@ -408,6 +408,33 @@ def test_fftz_tile_forms_bitwise(O, name, monkeypatch):
         assert np.array_equal(ymid(a), ymid(b))
 
 
+@pytest.mark.parametrize("form", ["fp32_context", "fp32_refine1"])
+def test_fftz_tile_forms_bitwise_fp32(O, form, monkeypatch):
+    """the FP32 z transforms (an FP32 context's, and the FP32 passes of the
+    FP64 context's refined pressure solve) with 8- and 16-line tiles: bit
+    for bit the same pressure solution"""
+    from channel3d_amd import Channel, lib as L
+    from channel3d_amd import synth
+    case = cases.C1
+    nx, ny, nz = case["nxc"], case["nyc"], case["nzc"]
+    src = np.asfortranarray(synth.ppe_rhs(nx, ny, nz, seed=11))
+    res = []
+    for tile in ("8", "16"):
+        monkeypatch.setenv("C3D_FFTZ_TILE", tile)
+        if form == "fp32_context":
+            g = Channel(case, precision=L.FP32)
+        else:
+            g = Channel(case)
+            g.set_ppe_refine(1)
+        monkeypatch.delenv("C3D_FFTZ_TILE")
+        g.upload(L.ARR1, src)
+        g.ppe(L.ARR1, L.HALOARR)
+        res.append(interior(g.download(L.HALOARR)))
+        g.close()
+    assert np.all(np.isfinite(res[0]))
+    assert np.array_equal(res[0], res[1])
+
+
 def test_per_routine_sequence_matches_fused(O):
     """ChannelIterate's call sequence through the per-routine entries gives the
     fused substep's result"""
