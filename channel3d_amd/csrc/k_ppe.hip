@@ -252,7 +252,7 @@ __device__ __forceinline__ void k_tridiag_ck2_body(const Geom& g, const Metrics<
                                                    T* __restrict__ ckp) {
   const int part = threadIdx.x & 1;  // 0: real, 1: imaginary part
   const int kx = blockIdx.x * 32 + (threadIdx.x >> 1);
-  const int kz = blockIdx.y * 4 + threadIdx.y;
+  const int kz = blockIdx.y * blockDim.y + threadIdx.y;
   if (kx > g.nk || kz >= g.nz) return;
   const int ny = g.ny;
   const int nseg = (ny + C - 1) / C, top = nseg - 1;
@@ -418,12 +418,15 @@ static int tridiag_t(c3d_ctx* c, void* spec, void* ck, const Batch& b) {
   const long long lines = (long long)(g.nk + 1) * g.nz * b.n;
   const bool two = c->tri_lanes ? c->tri_lanes == 2 : lines * 2 <= 2048LL * 64;
   if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz && two) {
-    const dim3 grid2((g.nk + 1 + 31) / 32, (g.nz + 3) / 4, b.n);
+    // one wavefront (32 wavenumber pairs of one kz) per workgroup: the
+    // grids that take this form have few lines, and single-wave workgroups
+    // spread them over more CUs
+    const dim3 grid2((g.nk + 1 + 31) / 32, g.nz, b.n), blk2(64, 1);
     if (g.nt)
-      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, true>), grid2, blk, 0, c->stream, g, m,
+      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, true>), grid2, blk2, 0, c->stream, g, m,
                          (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
     else
-      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, false>), grid2, blk, 0, c->stream, g, m,
+      hipLaunchKernelGGL((k_tridiag_ck2<T, CSEG, false>), grid2, blk2, 0, c->stream, g, m,
                          (typename CT2<T>::t*)spec, (T*)ck, b.spec, b.ck);
   } else if (c->ymode != YS_STORE && need <= (long long)g.P * g.ny * g.nz) {
     if (g.nt)
